@@ -1,0 +1,203 @@
+// Shared host/device declarations for the gfx950 transform kernels.
+//
+// Every transform is built from one kernel shape: a workgroup stages C
+// "columns" of R points each into LDS, runs an R-point Stockham autosort FFT
+// over them with a runtime radix schedule (2/3/4/5/7/8/11/13/16), and writes the
+// columns back. How columns map to HBM addresses is a 4-digit mixed-radix
+// decomposition of the flat column index, which expresses batched single-pass
+// transforms, the passes of a multi-pass (global Stockham) transform, and the
+// strided axis passes of FFTN/RFFT2 with the same code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace gofft {
+
+constexpr int kMaxStages = 16;   // LDS stages per sub-FFT (R <= 2^16 needs <= 16)
+constexpr int kMaxCols = 64;     // columns per workgroup
+
+// Division by a runtime constant d for n < 2^20, d < 2^20 (see fastdiv_make).
+struct FastDiv {
+  uint32_t d;
+  uint32_t pad;
+  uint64_t m;
+};
+
+inline FastDiv fastdiv_make(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  f.pad = 0;
+  f.m = ((1ull << 40) + d - 1) / d;
+  return f;
+}
+
+__host__ __device__ inline uint32_t fastdiv_q(uint32_t n, const FastDiv& f) {
+  return (uint32_t)(((uint64_t)n * f.m) >> 40);
+}
+
+// One LDS stage of an R-point sub-FFT: radix-`radix` Stockham DIT over
+// sub-transforms of size `ns` (product of the earlier radices).
+struct Stage {
+  int radix;
+  int ns;
+  int nb;      // R / radix butterflies per column
+  int tw_off;  // offset of this stage's twiddles in the stage table
+  FastDiv nsd;
+};
+
+// Column addressing: flat column g -> digits d0 = g % D0, d1 = (g / D0) % D1,
+// d2 = (g / (D0*D1)) % D2, d3 = rest. Element r of column g sits at
+// sum(d_i * a_i) + r * g_stride.
+struct ColMap {
+  long long D0, D1, D2;
+  long long a0, a1, a2, a3;
+  long long g;
+  int rowmajor;  // 1: walk each column contiguously (g == 1); 0: walk across columns
+  int pad;
+};
+
+// Sub-FFT description shared by all kernels.
+struct SubFFT {
+  int R;         // points per column
+  int C;         // columns per workgroup (power of two)
+  int log2C;
+  int tpc;       // threads per column (power of two)
+  int log2tpc;
+  int nstages;
+  FastDiv Rd;
+  Stage st[kMaxStages];
+};
+
+struct PassArgs {
+  const void* in;
+  void* out;
+  const void* stw;          // stage twiddles, precision of the data
+  const double2* ptw_hi;    // pass twiddle W_M^e = hi[e >> s] * lo[e & (2^s - 1)]
+  const double2* ptw_lo;
+  long long ncols;
+  long long tw_M;           // 0: no pass twiddle
+  int tw_shift;
+  int tw_digit;             // which column digit is the twiddle's k
+  int conj_in, conj_out;
+  double scale;             // applied on store when != 1
+  const long long* in_off;  // optional: digit d3 -> element offset (mixed-length batches)
+  const long long* out_off;
+  ColMap im, om;
+  SubFFT f;
+};
+
+// Real-input forward (RFFT / PSD / spectrogram), single pass: each column is
+// one real line of 2R samples seen as R complex pairs.
+struct RealFwdArgs {
+  const void* in;           // real samples (T)
+  void* out;                // complex bins (m+1 per line) or real PSD bins
+  const void* stw;
+  const void* utw;          // untangle twiddles W_n^k, k = 0..R (precision T)
+  const void* window;       // 2R window samples or null
+  long long nlines;
+  long long in_stride;      // real samples between consecutive lines
+  long long out_stride;     // output elements between consecutive lines
+  int mode;                 // 0 complex bins, 1 PSD
+  int odd_n;                // unused for even n (reserved)
+  double psd_scale;         // d / n
+  SubFFT f;
+};
+
+// Real-output inverse (IRFFT), single pass.
+struct RealInvArgs {
+  const void* in;           // complex bins
+  void* out;                // real samples
+  const void* stw;
+  const void* utw;
+  long long nlines;
+  long long in_stride;      // complex elements between consecutive lines
+  long long in_bins;        // bins present per line (missing ones read as 0)
+  long long out_stride;     // real samples between consecutive lines
+  double scale;             // 1/m
+  SubFFT f;
+};
+
+// Fused single-workgroup cyclic convolution (Rader / Bluestein when the
+// convolution length fits in LDS): gather/chirp -> FFT -> *B -> IFFT -> scatter.
+struct ConvArgs {
+  const void* in;
+  void* out;
+  const void* stw;
+  const void* spec;         // kernel spectrum B (length R, 1/R folded in)
+  const void* chirp;        // Bluestein chirp (length n) or null
+  const int* perm;          // Rader input gather perm[k] (length q) or null
+  const int* iperm;         // Rader output scatter iperm[k]
+  const long long* in_off;  // optional per-line element offsets (else l * stride)
+  const long long* out_off;
+  long long nlines;
+  long long in_stride, out_stride;
+  int n;                    // transform length
+  int kind;                 // 0 Rader cyclic, 1 Rader linear (read conv[k+q]), 2 Bluestein
+  int conj_io;              // inverse via conj(F(conj(x)))
+  double scale;             // post scale (1/n for IFFT)
+  SubFFT f;
+};
+
+// Launchers (defined in kernels.hip). Each returns a hipError_t.
+hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s);
+hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads, hipStream_t s);
+hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads, hipStream_t s);
+hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipStream_t s);
+
+// Element-wise helpers for the multi-pass convolution and real paths.
+// out[l*os + k] = in[l*is + k] * tab[k] (complex), k < n; conj on in/out optional.
+// Optional per-line offset tables (ioff/ooff) replace l*is / l*os.
+hipError_t launch_cmul_table(int prec, const void* in, void* out, const void* tab, long long n,
+                             long long nlines, long long is, long long os, int conj_in, int conj_out,
+                             double scale, const long long* ioff, const long long* ooff,
+                             hipStream_t s);
+// Rader gather: out[l*os + k] = k < q ? in[l*is + perm[k]] : 0, k < cl; conj optional.
+hipError_t launch_rader_gather(int prec, const void* in, void* out, const int* perm, long long q,
+                               long long cl, long long nlines, long long is, long long os,
+                               int conj_in, const long long* ioff, hipStream_t s);
+// Rader scatter: out[l*os + iperm[k]] = x0 + conv[l*cs + k + off], out[l*os] = sum(x).
+hipError_t launch_rader_scatter(int prec, const void* x, const void* conv, void* out, const int* iperm,
+                                long long n, long long off, long long nlines, long long xs,
+                                long long cs, long long os, int conj_out, double scale,
+                                const long long* xoff, const long long* ooff, hipStream_t s);
+// Bluestein chirp in: out[l*os + j] = j < n ? in[l*is + j] * w[j] : 0, j < m.
+hipError_t launch_chirp_in(int prec, const void* in, void* out, const void* w, long long n,
+                           long long m, long long nlines, long long is, long long os, int conj_in,
+                           const long long* ioff, hipStream_t s);
+// RFFT untangle of a packed m-point spectrum Z into m+1 bins (mode 1: PSD).
+hipError_t launch_untangle(int prec, const void* Z, void* out, const void* utw, long long m,
+                           long long nlines, long long zs, long long os, int mode,
+                           double psd_scale, hipStream_t s);
+// IRFFT pre-pass: Hermitian half spectrum -> packed m-point spectrum Z.
+hipError_t launch_herm_pack(int prec, const void* X, void* Z, const void* utw, long long m,
+                            long long nlines, long long xs, long long bins, long long zs,
+                            hipStream_t s);
+// Real lines -> complex lines (imaginary 0): out[l*os + i] = (in[l*is + i], 0), i < n.
+hipError_t launch_promote(int prec, const void* in, void* out, long long n, long long nlines,
+                          long long is, long long os, hipStream_t s);
+// Complex line copy: out[l*os + k] = in[l*is + k], k < n.
+hipError_t launch_copy_lines(int prec, const void* in, void* out, long long n, long long nlines,
+                             long long is, long long os, hipStream_t s);
+// Hermitian mirror of `bins` supplied bins into a full length-n spectrum.
+hipError_t launch_herm_full(int prec, const void* in, void* out, long long n, long long nlines,
+                            long long is, long long bins, long long os, hipStream_t s);
+// Real part of complex lines, optionally scaled: out[l*os + i] = re(in[l*is + i]) * scale.
+hipError_t launch_take_real(int prec, const void* in, void* out, long long n, long long nlines,
+                            long long is, long long os, double scale, hipStream_t s);
+// One-sided PSD of bins: out = |X|^2 * scale, doubled except DC (and k = n/2 when n even).
+hipError_t launch_power(int prec, const void* in, void* out, long long bins, long long n,
+                        long long nlines, long long is, long long os, double scale, hipStream_t s);
+// Spectrogram frames: out[t*seg + i] = x[t*step + i] * w[i].
+hipError_t launch_frames(int prec, const void* x, const void* w, void* out, long long seg,
+                         long long step, long long frames, hipStream_t s);
+// Strided axis lines <-> contiguous lines: line l = (outer, inner), element i at
+// outer*n*st + inner + i*st. dir 0 gathers to contiguous, 1 scatters back.
+hipError_t launch_axis_copy(int prec, const void* in, void* out, long long n, long long st,
+                            long long lines, int dir, hipStream_t s);
+// Fill n real elements of T with a value (graph-capturable, unlike a memset).
+hipError_t launch_fill(int prec, void* out, long long n, double value, hipStream_t s);
+// Scale a real or complex array in place or out of place (n elements of T).
+hipError_t launch_scale(int prec, const void* in, void* out, long long n, double scale,
+                        hipStream_t s);
+
+}  // namespace gofft

@@ -1,0 +1,1129 @@
+// gfx950 transform kernels: LDS-staged Stockham autosort FFTs.
+//
+// Layout conventions: complex data is interleaved (re, im) in the working
+// precision (double2 for the plain complex128 path, float2 for the complex64
+// path). A workgroup owns C columns of R points; LDS holds them as
+// lds[c * R + r]. Every LDS stage reads all its butterfly inputs into
+// registers, synchronises, then writes the outputs (in-place Stockham), so LDS
+// needs exactly C*R points.
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace gofft {
+
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { typedef double2 type; };
+template <> struct Vec2<float> { typedef float2 type; };
+
+template <typename T2> __device__ __forceinline__ T2 mkc(decltype(T2{}.x) x, decltype(T2{}.x) y) {
+  T2 r;
+  r.x = x;
+  r.y = y;
+  return r;
+}
+template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { return mkc<T2>(a.x + b.x, a.y + b.y); }
+template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { return mkc<T2>(a.x - b.x, a.y - b.y); }
+template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+  return mkc<T2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T2> __device__ __forceinline__ T2 cscale(T2 a, decltype(T2{}.x) s) { return mkc<T2>(a.x * s, a.y * s); }
+template <typename T2> __device__ __forceinline__ T2 mul_negi(T2 a) { return mkc<T2>(a.y, -a.x); }
+template <typename T2> __device__ __forceinline__ T2 cconj(T2 a) { return mkc<T2>(a.x, -a.y); }
+
+template <typename T2> __device__ __forceinline__ T2 from_d2(double2 w);
+template <> __device__ __forceinline__ double2 from_d2<double2>(double2 w) { return w; }
+template <> __device__ __forceinline__ float2 from_d2<float2>(double2 w) {
+  return make_float2((float)w.x, (float)w.y);
+}
+
+// ---------------------------------------------------------------------------
+// Forward DFT butterflies in registers, natural order in and out.
+
+template <typename T2> __device__ __forceinline__ void dft2(T2* v) {
+  T2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <typename T2> __device__ __forceinline__ void dft4(T2& v0, T2& v1, T2& v2, T2& v3) {
+  T2 t0 = cadd(v0, v2), t1 = csub(v0, v2), t2 = cadd(v1, v3), t3 = mul_negi(csub(v1, v3));
+  v0 = cadd(t0, t2);
+  v1 = cadd(t1, t3);
+  v2 = csub(t0, t2);
+  v3 = csub(t1, t3);
+}
+
+// cos/sin(2*pi*m/P) for m = 1 .. (P-1)/2.
+template <int P> struct PrimeConst;
+template <> struct PrimeConst<3> {
+  __device__ static double c(int m) { return -0.499999999999999999973; }
+  __device__ static double s(int m) { return 0.866025403784438646787; }
+};
+template <> struct PrimeConst<5> {
+  __device__ static double c(int m) { return m == 1 ? 0.309016994374947424076 : -0.809016994374947424104; }
+  __device__ static double s(int m) { return m == 1 ? 0.951056516295153572111 : 0.587785252292473129135; }
+};
+template <> struct PrimeConst<7> {
+  __device__ static double c(int m) {
+    return m == 1 ? 0.623489801858733530527 : m == 2 ? -0.222520933956314404342 : -0.900968867902419126320;
+  }
+  __device__ static double s(int m) {
+    return m == 1 ? 0.781831482468029808730 : m == 2 ? 0.974927912181823606982 : 0.433883739117558120293;
+  }
+};
+template <> struct PrimeConst<11> {
+  __device__ static double c(int m) {
+    const double t[5] = {0.841253532831181168863, 0.415415013001886425489, -0.142314838273285140460,
+                         -0.654860733945285064127, -0.959492973614497389901};
+    return t[m - 1];
+  }
+  __device__ static double s(int m) {
+    const double t[5] = {0.540640817455597582101, 0.909631995354518371418, 0.989821441880932732359,
+                         0.755749574354258283704, 0.281732556841429697652};
+    return t[m - 1];
+  }
+};
+template <> struct PrimeConst<13> {
+  __device__ static double c(int m) {
+    const double t[6] = {0.885456025653209895872, 0.568064746731155802487, 0.120536680255323053250,
+                         -0.354604887042535626003, -0.748510748171101098684, -0.970941817426052027193};
+    return t[m - 1];
+  }
+  __device__ static double s(int m) {
+    const double t[6] = {0.464723172043768545663, 0.822983865893656394580, 0.992708874098053992835,
+                         0.935016242685414823450, 0.663122658240795202322, 0.239315664287557766952};
+    return t[m - 1];
+  }
+};
+
+// Odd prime P: pair inputs (m, P-m) into sums S_m and differences D_m, then
+// y_k = x0 + sum_m cos(2pi km/P) S_m - i * sum_m sin(2pi km/P) D_m and
+// y_{P-k} the conjugate combination.
+template <int P, typename T2> __device__ __forceinline__ void dft_odd(T2* v) {
+  typedef decltype(T2{}.x) T;
+  constexpr int H = (P - 1) / 2;
+  T2 S[H], D[H];
+#pragma unroll
+  for (int m = 1; m <= H; ++m) {
+    S[m - 1] = cadd(v[m], v[P - m]);
+    D[m - 1] = csub(v[m], v[P - m]);
+  }
+  T2 x0 = v[0];
+  T2 y0 = x0;
+#pragma unroll
+  for (int m = 0; m < H; ++m) y0 = cadd(y0, S[m]);
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    T2 r = x0;
+    T2 q = mkc<T2>(0, 0);
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+      int e = (k * m) % P;
+      T c = (T)PrimeConst<P>::c(e <= H ? e : P - e);
+      T s = (T)(e <= H ? PrimeConst<P>::s(e) : -PrimeConst<P>::s(P - e));
+      r = mkc<T2>(r.x + c * S[m - 1].x, r.y + c * S[m - 1].y);
+      q = mkc<T2>(q.x + s * D[m - 1].x, q.y + s * D[m - 1].y);
+    }
+    // -i * q
+    T2 iq = mul_negi(q);
+    v[k] = cadd(r, iq);
+    v[P - k] = csub(r, iq);
+  }
+  v[0] = y0;
+}
+
+template <typename T2> __device__ __forceinline__ void dft8(T2* v) {
+  typedef decltype(T2{}.x) T;
+  const T r = (T)0.707106781186547524401;
+  T2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  T2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4(e0, e1, e2, e3);
+  dft4(o0, o1, o2, o3);
+  o1 = mkc<T2>(r * (o1.x + o1.y), r * (o1.y - o1.x));
+  o2 = mul_negi(o2);
+  o3 = mkc<T2>(r * (o3.y - o3.x), -r * (o3.x + o3.y));
+  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+}
+
+template <typename T2> __device__ __forceinline__ void dft16(T2* v) {
+  typedef decltype(T2{}.x) T;
+  // n = 4*n1 + n2: DFT4 over n1, twiddle W16^(n2*k1), DFT4 over n2.
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) dft4(v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]);
+  const T c1 = (T)0.923879532511286756101, s1 = (T)0.382683432365089771723;
+  const T r2 = (T)0.707106781186547524401;
+  // position of A[n2][k1] is n2 + 4*k1
+  // n2=1: k1=1,2,3 -> W^1, W^2, W^3
+  v[5] = mkc<T2>(c1 * v[5].x + s1 * v[5].y, c1 * v[5].y - s1 * v[5].x);
+  v[9] = mkc<T2>(r2 * (v[9].x + v[9].y), r2 * (v[9].y - v[9].x));
+  v[13] = mkc<T2>(s1 * v[13].x + c1 * v[13].y, s1 * v[13].y - c1 * v[13].x);
+  // n2=2: W^2, W^4, W^6
+  v[6] = mkc<T2>(r2 * (v[6].x + v[6].y), r2 * (v[6].y - v[6].x));
+  v[10] = mul_negi(v[10]);
+  v[14] = mkc<T2>(r2 * (v[14].y - v[14].x), -r2 * (v[14].x + v[14].y));
+  // n2=3: W^3, W^6, W^9
+  v[7] = mkc<T2>(s1 * v[7].x + c1 * v[7].y, s1 * v[7].y - c1 * v[7].x);
+  v[11] = mkc<T2>(r2 * (v[11].y - v[11].x), -r2 * (v[11].x + v[11].y));
+  // W^9 = exp(-2pi i 9/16) = -c1 + i s1
+  v[15] = mkc<T2>(-c1 * v[15].x - s1 * v[15].y, -c1 * v[15].y + s1 * v[15].x);
+  T2 y[16];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    T2 a0 = v[4 * k1], a1 = v[4 * k1 + 1], a2 = v[4 * k1 + 2], a3 = v[4 * k1 + 3];
+    dft4(a0, a1, a2, a3);
+    y[k1] = a0;
+    y[k1 + 4] = a1;
+    y[k1 + 8] = a2;
+    y[k1 + 12] = a3;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = y[i];
+}
+
+template <int RAD, typename T2> __device__ __forceinline__ void dft(T2* v) {
+  if constexpr (RAD == 2) dft2(v);
+  else if constexpr (RAD == 4) dft4(v[0], v[1], v[2], v[3]);
+  else if constexpr (RAD == 8) dft8(v);
+  else if constexpr (RAD == 16) dft16(v);
+  else dft_odd<RAD>(v);
+}
+
+// ---------------------------------------------------------------------------
+// One LDS stage over all columns of the workgroup. Thread (c, lt) owns the
+// butterflies j = lt, lt + tpc, ... of column c.
+
+template <typename T2, int RAD, int MAXP>
+__device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage& s, const T2* stw,
+                                          int c, int lt) {
+  constexpr int NB = MAXP / RAD > 0 ? MAXP / RAD : 1;
+  T2* col = lds + c * f.R;
+  const int nb = s.nb, ns = s.ns, tpc = f.tpc;
+  T2 v[NB][RAD];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    int j = lt + m * tpc;
+    if (j < nb) {
+#pragma unroll
+      for (int i = 0; i < RAD; ++i) v[m][i] = col[j + i * nb];
+      if (ns > 1) {
+        int q = (int)fastdiv_q((uint32_t)j, s.nsd);
+        int k = j - q * ns;
+        const T2* tw = stw + s.tw_off + k * (RAD - 1);
+#pragma unroll
+        for (int i = 1; i < RAD; ++i) v[m][i] = cmul(v[m][i], tw[i - 1]);
+      }
+      dft<RAD>(v[m]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    int j = lt + m * tpc;
+    if (j < nb) {
+      int q = (int)fastdiv_q((uint32_t)j, s.nsd);
+      int k = j - q * ns;
+      T2* o = col + q * ns * RAD + k;
+#pragma unroll
+      for (int i = 0; i < RAD; ++i) o[i * ns] = v[m][i];
+    }
+  }
+  __syncthreads();
+}
+
+// Runs the whole R-point sub-FFT (forward) on every column in LDS.
+template <typename T2, int MAXP>
+__device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw) {
+  const int tid = threadIdx.x;
+  const int c = tid >> f.log2tpc;
+  const int lt = tid & (f.tpc - 1);
+  for (int si = 0; si < f.nstages; ++si) {
+    const Stage& s = f.st[si];
+    switch (s.radix) {
+      case 2: lds_stage<T2, 2, MAXP>(lds, f, s, stw, c, lt); break;
+      case 3: lds_stage<T2, 3, MAXP>(lds, f, s, stw, c, lt); break;
+      case 4: lds_stage<T2, 4, MAXP>(lds, f, s, stw, c, lt); break;
+      case 5: lds_stage<T2, 5, MAXP>(lds, f, s, stw, c, lt); break;
+      case 7: lds_stage<T2, 7, MAXP>(lds, f, s, stw, c, lt); break;
+      case 8: lds_stage<T2, 8, MAXP>(lds, f, s, stw, c, lt); break;
+      case 11: lds_stage<T2, 11, MAXP>(lds, f, s, stw, c, lt); break;
+      case 13: lds_stage<T2, 13, MAXP>(lds, f, s, stw, c, lt); break;
+      default: lds_stage<T2, 16, MAXP>(lds, f, s, stw, c, lt); break;
+    }
+  }
+}
+
+__device__ __forceinline__ long long col_offset(const ColMap& m, long long g, long long* dtw,
+                                                int tw_digit, const long long* off) {
+  long long d0 = g % m.D0;
+  long long t = g / m.D0;
+  long long d1 = t % m.D1;
+  t /= m.D1;
+  long long d2 = t % m.D2;
+  long long d3 = t / m.D2;
+  if (dtw) *dtw = tw_digit == 0 ? d0 : d1;
+  return d0 * m.a0 + d1 * m.a1 + d2 * m.a2 + (off ? off[d3] : d3 * m.a3);
+}
+
+__device__ __forceinline__ double2 pass_twiddle(const double2* hi, const double2* lo, long long e,
+                                                int shift) {
+  double2 a = hi[e >> shift];
+  double2 b = lo[e & ((1ll << shift) - 1)];
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Sum over aligned groups of `width` lanes (width a power of two <= 64); every
+// lane of a group ends with the group total.
+template <typename T2> __device__ __forceinline__ T2 group_sum(T2 v, int width) {
+  for (int o = 1; o < width; o <<= 1) {
+    v.x += __shfl_xor(v.x, o, 64);
+    v.y += __shfl_xor(v.y, o, 64);
+  }
+  return v;
+}
+
+extern __shared__ unsigned char g_dyn_lds[];
+
+// ---------------------------------------------------------------------------
+// Generic pass: load C columns (optional pass twiddle W_M^(r*k), optional
+// conjugation), R-point FFT in LDS, store (optional conjugation and scale).
+template <typename T, int MAXP>
+__global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
+  typedef typename Vec2<T>::type T2;
+  T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
+  __shared__ long long s_ib[kMaxCols], s_ob[kMaxCols], s_tk[kMaxCols];
+  const SubFFT& f = a.f;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int C = f.C, R = f.R;
+  const long long g0 = (long long)blockIdx.x * C;
+  if (tid < C) {
+    long long g = g0 + tid;
+    if (g < a.ncols) {
+      long long k = 0;
+      s_ib[tid] = col_offset(a.im, g, &k, a.tw_digit, a.in_off);
+      s_ob[tid] = col_offset(a.om, g, nullptr, 0, a.out_off);
+      s_tk[tid] = k;
+    } else {
+      s_ib[tid] = -1;
+      s_ob[tid] = -1;
+      s_tk[tid] = 0;
+    }
+  }
+  __syncthreads();
+  const T2* in = reinterpret_cast<const T2*>(a.in);
+  const int total = C * R;
+  for (int fi = tid; fi < total; fi += nthr) {
+    int c, r;
+    if (a.im.rowmajor) {
+      c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+      r = fi - c * R;
+    } else {
+      c = fi & (C - 1);
+      r = fi >> f.log2C;
+    }
+    long long base = s_ib[c];
+    T2 v = mkc<T2>(0, 0);
+    if (base >= 0) {
+      v = in[base + (long long)r * a.im.g];
+      if (a.tw_M) {
+        long long e = (long long)r * s_tk[c];
+        if (e) v = cmul(v, from_d2<T2>(pass_twiddle(a.ptw_hi, a.ptw_lo, e, a.tw_shift)));
+      }
+      if (a.conj_in) v.y = -v.y;
+    }
+    lds[c * R + r] = v;
+  }
+  __syncthreads();
+  lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
+  T2* out = reinterpret_cast<T2*>(a.out);
+  const T sc = (T)a.scale;
+  const bool do_scale = a.scale != 1.0;
+  for (int fi = tid; fi < total; fi += nthr) {
+    int c, r;
+    if (a.om.rowmajor) {
+      c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+      r = fi - c * R;
+    } else {
+      c = fi & (C - 1);
+      r = fi >> f.log2C;
+    }
+    long long base = s_ob[c];
+    if (base >= 0) {
+      T2 v = lds[c * R + r];
+      if (a.conj_out) v.y = -v.y;
+      if (do_scale) v = cscale(v, sc);
+      out[base + (long long)r * a.om.g] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Real forward, single pass: line l holds 2R real samples; Z = FFT_R(pairs),
+// then the untangle to R+1 bins (mode 0) or the one-sided PSD (mode 1).
+template <typename T, int MAXP>
+__global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
+  typedef typename Vec2<T>::type T2;
+  T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
+  const SubFFT& f = a.f;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int C = f.C, R = f.R;
+  const long long l0 = (long long)blockIdx.x * C;
+  const T* in = reinterpret_cast<const T*>(a.in);
+  const T* win = reinterpret_cast<const T*>(a.window);
+  const int total = C * R;
+  for (int fi = tid; fi < total; fi += nthr) {
+    int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+    int r = fi - c * R;
+    long long l = l0 + c;
+    T2 v = mkc<T2>(0, 0);
+    if (l < a.nlines) {
+      const T* p = in + l * a.in_stride + 2 * (long long)r;
+      v = mkc<T2>(p[0], p[1]);
+      if (win) v = mkc<T2>(v.x * win[2 * r], v.y * win[2 * r + 1]);
+    }
+    lds[c * R + r] = v;
+  }
+  __syncthreads();
+  lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
+  const T2* utw = reinterpret_cast<const T2*>(a.utw);
+  const int nb = R + 1;
+  const int tot2 = C * nb;
+  for (int fi = tid; fi < tot2; fi += nthr) {
+    int c = fi / nb;
+    int k = fi - c * nb;
+    long long l = l0 + c;
+    if (l >= a.nlines) continue;
+    const T2* Z = lds + c * R;
+    T2 X;
+    if (k == 0 || k == R) {
+      T2 z0 = Z[0];
+      X = mkc<T2>(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0);
+    } else {
+      T2 zk = Z[k], zmk = Z[R - k];
+      T xer = (zk.x + zmk.x) * (T)0.5;
+      T xei = (zk.y - zmk.y) * (T)0.5;
+      T xor_ = (zk.y + zmk.y) * (T)0.5;
+      T xoi = -(zk.x - zmk.x) * (T)0.5;
+      T2 w = utw[k];
+      T tr = w.x * xor_ - w.y * xoi;
+      T ti = w.x * xoi + w.y * xor_;
+      X = mkc<T2>(xer + tr, xei + ti);
+    }
+    if (a.mode == 0) {
+      reinterpret_cast<T2*>(a.out)[l * a.out_stride + k] = X;
+    } else {
+      T p = (X.x * X.x + X.y * X.y) * (T)a.psd_scale;
+      if (k != 0 && k != R) p *= (T)2;
+      reinterpret_cast<T*>(a.out)[l * a.out_stride + k] = p;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Real inverse, single pass: R+1 bins -> packed spectrum Z -> IFFT_R (via
+// conj(FFT(conj))) -> 2R real samples scaled by 1/R.
+template <typename T, int MAXP>
+__global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
+  typedef typename Vec2<T>::type T2;
+  T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
+  const SubFFT& f = a.f;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int C = f.C, R = f.R;
+  const long long l0 = (long long)blockIdx.x * C;
+  const T2* in = reinterpret_cast<const T2*>(a.in);
+  const T2* utw = reinterpret_cast<const T2*>(a.utw);
+  const int total = C * R;
+  for (int fi = tid; fi < total; fi += nthr) {
+    int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+    int r = fi - c * R;
+    long long l = l0 + c;
+    T2 v = mkc<T2>(0, 0);
+    if (l < a.nlines && r < a.in_bins) v = in[l * a.in_stride + r];
+    lds[c * R + r] = v;
+  }
+  __syncthreads();
+  // Pairs (k, R-k) for k = 0 .. R/2, each owned by one thread; k = 0 pairs DC
+  // with the Nyquist bin X[R] read from memory.
+  const int half = R / 2 + 1;
+  const int tot2 = C * half;
+  for (int fi = tid; fi < tot2; fi += nthr) {
+    int c = fi / half;
+    int k = fi - c * half;
+    long long l = l0 + c;
+    T2* X = lds + c * R;
+    if (k == 0) {
+      T x0 = X[0].x;
+      T xm = 0;
+      if (l < a.nlines && R < a.in_bins) xm = in[l * a.in_stride + R].x;
+      // conj for the inverse-by-forward trick
+      X[0] = mkc<T2>((x0 + xm) * (T)0.5, -((x0 - xm) * (T)0.5));
+      continue;
+    }
+    int mk_ = R - k;
+    T2 xk = X[k], xmk = X[mk_];
+    // Z[k] = xe + i*xo with xe = (Xk + conj Xmk)/2, d = (Xk - conj Xmk)/2,
+    // xo = conj(W^k) d; Z[m-k] symmetric.
+    T2 w = utw[k];
+    {
+      T xer = (xk.x + xmk.x) * (T)0.5, xei = (xk.y - xmk.y) * (T)0.5;
+      T dr = (xk.x - xmk.x) * (T)0.5, di = (xk.y + xmk.y) * (T)0.5;
+      T xor_ = w.x * dr + w.y * di, xoi = w.x * di - w.y * dr;
+      T2 zk = mkc<T2>(xer - xoi, xei + xor_);
+      if (k != mk_) {
+        T2 zmk = mkc<T2>(xer + xoi, -(xei - xor_));
+        X[mk_] = cconj(zmk);
+      }
+      X[k] = cconj(zk);
+    }
+  }
+  __syncthreads();
+  lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
+  T* out = reinterpret_cast<T*>(a.out);
+  const T sc = (T)a.scale;
+  for (int fi = tid; fi < total; fi += nthr) {
+    int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+    int r = fi - c * R;
+    long long l = l0 + c;
+    if (l >= a.nlines) continue;
+    T2 z = lds[c * R + r];
+    T* p = out + l * a.out_stride + 2 * (long long)r;
+    p[0] = z.x * sc;
+    p[1] = -z.y * sc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused convolution (Rader / Bluestein), C transforms per workgroup.
+template <typename T, int MAXP>
+__global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
+  typedef typename Vec2<T>::type T2;
+  T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
+  __shared__ T2 s_red[16];  // one partial per wave
+  const SubFFT& f = a.f;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int C = f.C, R = f.R;
+  const long long l0 = (long long)blockIdx.x * C;
+  const T2* in = reinterpret_cast<const T2*>(a.in);
+  const T2* chirp = reinterpret_cast<const T2*>(a.chirp);
+  const int n = a.n;
+  const int q = n - 1;
+  const int total = C * R;
+  for (int fi = tid; fi < total; fi += nthr) {
+    int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+    int r = fi - c * R;
+    long long l = l0 + c;
+    T2 v = mkc<T2>(0, 0);
+    if (l < a.nlines) {
+      const T2* x = in + (a.in_off ? a.in_off[l] : l * a.in_stride);
+      if (a.kind == 2) {
+        if (r < n) {
+          v = x[r];
+          if (a.conj_io) v.y = -v.y;
+          v = cmul(v, chirp[r]);
+        }
+      } else if (r < q) {
+        v = x[a.perm[r]];
+        if (a.conj_io) v.y = -v.y;
+      }
+    }
+    lds[c * R + r] = v;
+  }
+  __syncthreads();
+  const int c = tid >> f.log2tpc;
+  const int lt = tid & (f.tpc - 1);
+  T2 colsum = mkc<T2>(0, 0);
+  if (a.kind != 2) {
+    // Rader DC: sum of x[1..n-1] (all gathered samples) plus x[0].
+    T2 acc = mkc<T2>(0, 0);
+    for (int r = lt; r < q; r += f.tpc) acc = cadd(acc, lds[c * R + r]);
+    acc = group_sum(acc, f.tpc < 64 ? f.tpc : 64);
+    if (f.tpc > 64) {
+      if ((tid & 63) == 0) s_red[tid >> 6] = acc;
+      __syncthreads();
+      acc = mkc<T2>(0, 0);
+      const int w0 = (c << f.log2tpc) >> 6, nw = f.tpc >> 6;
+      for (int w = 0; w < nw; ++w) acc = cadd(acc, s_red[w0 + w]);
+    }
+    colsum = acc;
+  }
+  const T2* stw = reinterpret_cast<const T2*>(a.stw);
+  lds_fft<T2, MAXP>(lds, f, stw);
+  const T2* spec = reinterpret_cast<const T2*>(a.spec);
+  for (int r = lt; r < R; r += f.tpc) lds[c * R + r] = cconj(cmul(lds[c * R + r], spec[r]));
+  __syncthreads();
+  lds_fft<T2, MAXP>(lds, f, stw);
+  const long long l = l0 + c;
+  const bool live = l < a.nlines;
+  T2* out = reinterpret_cast<T2*>(a.out) + (live ? (a.out_off ? a.out_off[l] : l * a.out_stride) : 0);
+  const T sc = (T)a.scale;
+  if (a.kind == 2) {
+    for (int k = lt; k < n && live; k += f.tpc) {
+      T2 v = cmul(cconj(lds[c * R + k]), chirp[k]);
+      if (a.conj_io) v.y = -v.y;
+      out[k] = cscale(v, sc);
+    }
+  } else {
+    const T2* x = in + (live ? (a.in_off ? a.in_off[l] : l * a.in_stride) : 0);
+    T2 x0 = live ? x[0] : mkc<T2>(0, 0);
+    if (a.conj_io) x0.y = -x0.y;
+    const int off = a.kind == 1 ? q : 0;
+    for (int k = lt; k < q && live; k += f.tpc) {
+      T2 v = cadd(x0, cconj(lds[c * R + k + off]));
+      if (a.conj_io) v.y = -v.y;
+      out[a.iperm[k]] = cscale(v, sc);
+    }
+    if (lt == 0 && live) {
+      T2 v = cadd(x0, colsum);
+      if (a.conj_io) v.y = -v.y;
+      out[0] = cscale(v, sc);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise helpers.
+
+__device__ __forceinline__ long long line_base(const long long* off, long long l, long long stride) {
+  return off ? off[l] : l * stride;
+}
+
+template <typename T>
+__global__ void k_cmul_table(const void* in_, void* out_, const void* tab_, long long n,
+                             long long nlines, long long is, long long os, int conj_in, int conj_out,
+                             double scale, const long long* ioff, const long long* ooff) {
+  typedef typename Vec2<T>::type T2;
+  const T2* in = reinterpret_cast<const T2*>(in_);
+  const T2* tab = reinterpret_cast<const T2*>(tab_);
+  T2* out = reinterpret_cast<T2*>(out_);
+  long long total = n * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / n, k = i - l * n;
+    T2 v = in[line_base(ioff, l, is) + k];
+    if (conj_in) v.y = -v.y;
+    v = cmul(v, tab[k]);
+    if (conj_out) v.y = -v.y;
+    if (scale != 1.0) v = cscale(v, (T)scale);
+    out[line_base(ooff, l, os) + k] = v;
+  }
+}
+
+template <typename T>
+__global__ void k_rader_gather(const void* in_, void* out_, const int* perm, long long q, long long cl,
+                               long long nlines, long long is, long long os, int conj_in,
+                               const long long* ioff) {
+  typedef typename Vec2<T>::type T2;
+  const T2* in = reinterpret_cast<const T2*>(in_);
+  T2* out = reinterpret_cast<T2*>(out_);
+  long long total = cl * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / cl, k = i - l * cl;
+    T2 v = mkc<T2>(0, 0);
+    if (k < q) {
+      v = in[line_base(ioff, l, is) + perm[k]];
+      if (conj_in) v.y = -v.y;
+    }
+    out[l * os + k] = v;
+  }
+}
+
+// One workgroup per line: scatter x0 + conv, and the DC sum by a tree.
+template <typename T>
+__global__ void __launch_bounds__(256) k_rader_scatter(const void* x_, const void* conv_, void* out_,
+                                                       const int* iperm, long long n, long long off,
+                                                       long long xs, long long cs, long long os,
+                                                       int conj_out, double scale,
+                                                       const long long* xoff, const long long* ooff) {
+  typedef typename Vec2<T>::type T2;
+  __shared__ T2 red[256];
+  const long long l = blockIdx.x;
+  const T2* x = reinterpret_cast<const T2*>(x_) + line_base(xoff, l, xs);
+  const T2* conv = reinterpret_cast<const T2*>(conv_) + l * cs;
+  T2* out = reinterpret_cast<T2*>(out_) + line_base(ooff, l, os);
+  T2 x0 = x[0];
+  const T sc = (T)scale;
+  T2 acc = mkc<T2>(0, 0);
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) acc = cadd(acc, x[i]);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  // For an inverse (conj_out) the convolution ran on conj(x), so x0 and the DC
+  // sum enter conjugated and the whole output is conjugated back.
+  T2 sum = red[0];
+  if (conj_out) {
+    x0.y = -x0.y;
+    sum.y = -sum.y;
+  }
+  for (long long k = threadIdx.x; k < n - 1; k += blockDim.x) {
+    T2 v = cadd(x0, conv[k + off]);
+    if (conj_out) v.y = -v.y;
+    out[iperm[k]] = cscale(v, sc);
+  }
+  if (threadIdx.x == 0) {
+    if (conj_out) sum.y = -sum.y;
+    out[0] = cscale(sum, sc);
+  }
+}
+
+template <typename T>
+__global__ void k_chirp_in(const void* in_, void* out_, const void* w_, long long n, long long m,
+                           long long nlines, long long is, long long os, int conj_in,
+                           const long long* ioff) {
+  typedef typename Vec2<T>::type T2;
+  const T2* in = reinterpret_cast<const T2*>(in_);
+  const T2* w = reinterpret_cast<const T2*>(w_);
+  T2* out = reinterpret_cast<T2*>(out_);
+  long long total = m * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / m, j = i - l * m;
+    T2 v = mkc<T2>(0, 0);
+    if (j < n) {
+      v = in[line_base(ioff, l, is) + j];
+      if (conj_in) v.y = -v.y;
+      v = cmul(v, w[j]);
+    }
+    out[l * os + j] = v;
+  }
+}
+
+template <typename T>
+__global__ void k_untangle(const void* Z_, void* out_, const void* utw_, long long m, long long nlines,
+                           long long zs, long long os, int mode, double psd_scale) {
+  typedef typename Vec2<T>::type T2;
+  const T2* Zall = reinterpret_cast<const T2*>(Z_);
+  const T2* utw = reinterpret_cast<const T2*>(utw_);
+  long long nb = m + 1, total = nb * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / nb, k = i - l * nb;
+    const T2* Z = Zall + l * zs;
+    T2 X;
+    if (k == 0 || k == m) {
+      T2 z0 = Z[0];
+      X = mkc<T2>(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0);
+    } else {
+      T2 zk = Z[k], zmk = Z[m - k];
+      T xer = (zk.x + zmk.x) * (T)0.5, xei = (zk.y - zmk.y) * (T)0.5;
+      T xor_ = (zk.y + zmk.y) * (T)0.5, xoi = -(zk.x - zmk.x) * (T)0.5;
+      T2 w = utw[k];
+      X = mkc<T2>(xer + (w.x * xor_ - w.y * xoi), xei + (w.x * xoi + w.y * xor_));
+    }
+    if (mode == 0) {
+      reinterpret_cast<T2*>(out_)[l * os + k] = X;
+    } else {
+      T p = (X.x * X.x + X.y * X.y) * (T)psd_scale;
+      if (k != 0 && k != m) p *= (T)2;
+      reinterpret_cast<T*>(out_)[l * os + k] = p;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_herm_pack(const void* X_, void* Z_, const void* utw_, long long m, long long nlines,
+                            long long xs, long long bins, long long zs) {
+  typedef typename Vec2<T>::type T2;
+  const T2* Xall = reinterpret_cast<const T2*>(X_);
+  const T2* utw = reinterpret_cast<const T2*>(utw_);
+  T2* Zall = reinterpret_cast<T2*>(Z_);
+  long long total = m * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / m, k = i - l * m;
+    const T2* X = Xall + l * xs;
+    T2 z;
+    if (k == 0) {
+      T x0 = bins > 0 ? X[0].x : (T)0;
+      T xm = m < bins ? X[m].x : (T)0;
+      z = mkc<T2>((x0 + xm) * (T)0.5, (x0 - xm) * (T)0.5);
+    } else {
+      T2 xk = k < bins ? X[k] : mkc<T2>(0, 0);
+      T2 xmk = (m - k) < bins ? X[m - k] : mkc<T2>(0, 0);
+      T xer = (xk.x + xmk.x) * (T)0.5, xei = (xk.y - xmk.y) * (T)0.5;
+      T dr = (xk.x - xmk.x) * (T)0.5, di = (xk.y + xmk.y) * (T)0.5;
+      T2 w = utw[k];
+      T xor_ = w.x * dr + w.y * di, xoi = w.x * di - w.y * dr;
+      z = mkc<T2>(xer - xoi, xei + xor_);
+    }
+    Zall[l * zs + k] = z;
+  }
+}
+
+
+template <typename T>
+__global__ void k_promote(const T* in, void* out_, long long n, long long nlines, long long is,
+                          long long os) {
+  typedef typename Vec2<T>::type T2;
+  T2* out = reinterpret_cast<T2*>(out_);
+  long long total = n * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / n, k = i - l * n;
+    out[l * os + k] = mkc<T2>(in[l * is + k], 0);
+  }
+}
+
+template <typename T>
+__global__ void k_copy_lines(const void* in_, void* out_, long long n, long long nlines, long long is,
+                             long long os) {
+  typedef typename Vec2<T>::type T2;
+  const T2* in = reinterpret_cast<const T2*>(in_);
+  T2* out = reinterpret_cast<T2*>(out_);
+  long long total = n * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / n, k = i - l * n;
+    out[l * os + k] = in[l * is + k];
+  }
+}
+
+// full[k] = src[k] (k < bins, k <= n/2), full[n-k] = conj(src[k]) for 0 < k < n-k
+// (realplan.go:197-213).
+template <typename T>
+__global__ void k_herm_full(const void* in_, void* out_, long long n, long long nlines, long long is,
+                            long long bins, long long os) {
+  typedef typename Vec2<T>::type T2;
+  const T2* in = reinterpret_cast<const T2*>(in_);
+  T2* out = reinterpret_cast<T2*>(out_);
+  long long half = n / 2 + 1;
+  long long total = n * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / n, k = i - l * n;
+    T2 v = mkc<T2>(0, 0);
+    if (k < half) {
+      if (k < bins) v = in[l * is + k];
+    } else {
+      long long kk = n - k;  // 0 < kk < n - kk
+      if (kk < bins) v = cconj(in[l * is + kk]);
+    }
+    out[l * os + k] = v;
+  }
+}
+
+template <typename T>
+__global__ void k_take_real(const void* in_, T* out, long long n, long long nlines, long long is,
+                            long long os, double scale) {
+  typedef typename Vec2<T>::type T2;
+  const T2* in = reinterpret_cast<const T2*>(in_);
+  long long total = n * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / n, k = i - l * n;
+    out[l * os + k] = in[l * is + k].x * (T)scale;
+  }
+}
+
+template <typename T>
+__global__ void k_power(const void* in_, T* out, long long bins, long long n, long long nlines,
+                        long long is, long long os, double scale) {
+  typedef typename Vec2<T>::type T2;
+  const T2* in = reinterpret_cast<const T2*>(in_);
+  long long total = bins * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / bins, k = i - l * bins;
+    T2 X = in[l * is + k];
+    T p = (X.x * X.x + X.y * X.y) * (T)scale;
+    if (k != 0 && !(n % 2 == 0 && k == bins - 1)) p *= (T)2;
+    out[l * os + k] = p;
+  }
+}
+
+template <typename T>
+__global__ void k_frames(const T* x, const T* w, T* out, long long seg, long long step,
+                         long long frames) {
+  long long total = seg * frames;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long t = i / seg, k = i - t * seg;
+    out[i] = x[t * step + k] * w[k];
+  }
+}
+
+template <typename T>
+__global__ void k_axis_copy(const void* in_, void* out_, long long n, long long st, long long lines,
+                            int dir) {
+  typedef typename Vec2<T>::type T2;
+  const T2* in = reinterpret_cast<const T2*>(in_);
+  T2* out = reinterpret_cast<T2*>(out_);
+  long long total = n * lines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    // Iterate in strided-array order so both sides stay mostly coalesced.
+    long long inner = i % st;
+    long long t = i / st;
+    long long k = t % n;
+    long long outer = t / n;
+    long long strided = outer * n * st + k * st + inner;
+    long long contig = (outer * st + inner) * n + k;
+    if (dir == 0) out[contig] = in[strided];
+    else out[strided] = in[contig];
+  }
+}
+
+template <typename T>
+__global__ void k_fill(T* out, long long n, double value) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = (T)value;
+}
+
+template <typename T>
+__global__ void k_scale(const T* in, T* out, long long n, double scale) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = in[i] * (T)scale;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+
+static inline unsigned grid_for(long long total, int block) {
+  long long g = (total + block - 1) / block;
+  if (g > 1048576) g = 1048576;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+template <typename T> static size_t elt2() { return 2 * sizeof(T); }
+
+#define GOFFT_DISPATCH_MAXP(KERNEL, T, ARGS, BYTES, THREADS, GRID, STREAM)                      \
+  do {                                                                                         \
+    (void)maxp;                                                                                \
+    hipLaunchKernelGGL((KERNEL<T, 16>), dim3(GRID), dim3(THREADS), BYTES, STREAM, ARGS);       \
+  } while (0)
+
+hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s) {
+  long long groups = (a.ncols + a.f.C - 1) / a.f.C;
+  if (prec == 0) {
+    size_t bytes = (size_t)a.f.C * a.f.R * elt2<double>();
+    GOFFT_DISPATCH_MAXP(k_pass, double, a, bytes, threads, groups, s);
+  } else {
+    size_t bytes = (size_t)a.f.C * a.f.R * elt2<float>();
+    GOFFT_DISPATCH_MAXP(k_pass, float, a, bytes, threads, groups, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads, hipStream_t s) {
+  long long groups = (a.nlines + a.f.C - 1) / a.f.C;
+  if (prec == 0) {
+    size_t bytes = (size_t)a.f.C * a.f.R * elt2<double>();
+    GOFFT_DISPATCH_MAXP(k_real_fwd, double, a, bytes, threads, groups, s);
+  } else {
+    size_t bytes = (size_t)a.f.C * a.f.R * elt2<float>();
+    GOFFT_DISPATCH_MAXP(k_real_fwd, float, a, bytes, threads, groups, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads, hipStream_t s) {
+  long long groups = (a.nlines + a.f.C - 1) / a.f.C;
+  if (prec == 0) {
+    size_t bytes = (size_t)a.f.C * a.f.R * elt2<double>();
+    GOFFT_DISPATCH_MAXP(k_real_inv, double, a, bytes, threads, groups, s);
+  } else {
+    size_t bytes = (size_t)a.f.C * a.f.R * elt2<float>();
+    GOFFT_DISPATCH_MAXP(k_real_inv, float, a, bytes, threads, groups, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipStream_t s) {
+  long long groups = (a.nlines + a.f.C - 1) / a.f.C;
+  if (prec == 0) {
+    size_t bytes = (size_t)a.f.C * a.f.R * elt2<double>();
+    GOFFT_DISPATCH_MAXP(k_conv, double, a, bytes, threads, groups, s);
+  } else {
+    size_t bytes = (size_t)a.f.C * a.f.R * elt2<float>();
+    GOFFT_DISPATCH_MAXP(k_conv, float, a, bytes, threads, groups, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_cmul_table(int prec, const void* in, void* out, const void* tab, long long n,
+                             long long nlines, long long is, long long os, int conj_in, int conj_out,
+                             double scale, const long long* ioff, const long long* ooff,
+                             hipStream_t s) {
+  unsigned g = grid_for(n * nlines, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_cmul_table<double>, dim3(g), dim3(256), 0, s, in, out, tab, n, nlines, is, os,
+                       conj_in, conj_out, scale, ioff, ooff);
+  else
+    hipLaunchKernelGGL(k_cmul_table<float>, dim3(g), dim3(256), 0, s, in, out, tab, n, nlines, is, os,
+                       conj_in, conj_out, scale, ioff, ooff);
+  return hipGetLastError();
+}
+
+hipError_t launch_rader_gather(int prec, const void* in, void* out, const int* perm, long long q,
+                               long long cl, long long nlines, long long is, long long os,
+                               int conj_in, const long long* ioff, hipStream_t s) {
+  unsigned g = grid_for(cl * nlines, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_rader_gather<double>, dim3(g), dim3(256), 0, s, in, out, perm, q, cl, nlines,
+                       is, os, conj_in, ioff);
+  else
+    hipLaunchKernelGGL(k_rader_gather<float>, dim3(g), dim3(256), 0, s, in, out, perm, q, cl, nlines,
+                       is, os, conj_in, ioff);
+  return hipGetLastError();
+}
+
+hipError_t launch_rader_scatter(int prec, const void* x, const void* conv, void* out, const int* iperm,
+                                long long n, long long off, long long nlines, long long xs,
+                                long long cs, long long os, int conj_out, double scale,
+                                const long long* xoff, const long long* ooff, hipStream_t s) {
+  if (nlines <= 0) return hipSuccess;
+  if (prec == 0)
+    hipLaunchKernelGGL(k_rader_scatter<double>, dim3((unsigned)nlines), dim3(256), 0, s, x, conv, out,
+                       iperm, n, off, xs, cs, os, conj_out, scale, xoff, ooff);
+  else
+    hipLaunchKernelGGL(k_rader_scatter<float>, dim3((unsigned)nlines), dim3(256), 0, s, x, conv, out,
+                       iperm, n, off, xs, cs, os, conj_out, scale, xoff, ooff);
+  return hipGetLastError();
+}
+
+hipError_t launch_chirp_in(int prec, const void* in, void* out, const void* w, long long n,
+                           long long m, long long nlines, long long is, long long os, int conj_in,
+                           const long long* ioff, hipStream_t s) {
+  unsigned g = grid_for(m * nlines, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_chirp_in<double>, dim3(g), dim3(256), 0, s, in, out, w, n, m, nlines, is, os,
+                       conj_in, ioff);
+  else
+    hipLaunchKernelGGL(k_chirp_in<float>, dim3(g), dim3(256), 0, s, in, out, w, n, m, nlines, is, os,
+                       conj_in, ioff);
+  return hipGetLastError();
+}
+
+hipError_t launch_untangle(int prec, const void* Z, void* out, const void* utw, long long m,
+                           long long nlines, long long zs, long long os, int mode,
+                           double psd_scale, hipStream_t s) {
+  unsigned g = grid_for((m + 1) * nlines, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_untangle<double>, dim3(g), dim3(256), 0, s, Z, out, utw, m, nlines, zs, os,
+                       mode, psd_scale);
+  else
+    hipLaunchKernelGGL(k_untangle<float>, dim3(g), dim3(256), 0, s, Z, out, utw, m, nlines, zs, os,
+                       mode, psd_scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_herm_pack(int prec, const void* X, void* Z, const void* utw, long long m,
+                            long long nlines, long long xs, long long bins, long long zs,
+                            hipStream_t s) {
+  unsigned g = grid_for(m * nlines, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_herm_pack<double>, dim3(g), dim3(256), 0, s, X, Z, utw, m, nlines, xs, bins,
+                       zs);
+  else
+    hipLaunchKernelGGL(k_herm_pack<float>, dim3(g), dim3(256), 0, s, X, Z, utw, m, nlines, xs, bins,
+                       zs);
+  return hipGetLastError();
+}
+
+
+#define GOFFT_LAUNCH_EW(KERNEL, TOTAL, ...)                                                    \
+  do {                                                                                       \
+    unsigned g_ = grid_for(TOTAL, 256);                                                      \
+    if (prec == 0)                                                                           \
+      hipLaunchKernelGGL(KERNEL<double>, dim3(g_), dim3(256), 0, s, __VA_ARGS__);            \
+    else                                                                                     \
+      hipLaunchKernelGGL(KERNEL<float>, dim3(g_), dim3(256), 0, s, __VA_ARGS__);             \
+  } while (0)
+
+hipError_t launch_promote(int prec, const void* in, void* out, long long n, long long nlines,
+                          long long is, long long os, hipStream_t s) {
+  unsigned g = grid_for(n * nlines, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_promote<double>, dim3(g), dim3(256), 0, s, (const double*)in, out, n, nlines,
+                       is, os);
+  else
+    hipLaunchKernelGGL(k_promote<float>, dim3(g), dim3(256), 0, s, (const float*)in, out, n, nlines,
+                       is, os);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_lines(int prec, const void* in, void* out, long long n, long long nlines,
+                             long long is, long long os, hipStream_t s) {
+  GOFFT_LAUNCH_EW(k_copy_lines, n * nlines, in, out, n, nlines, is, os);
+  return hipGetLastError();
+}
+
+hipError_t launch_herm_full(int prec, const void* in, void* out, long long n, long long nlines,
+                            long long is, long long bins, long long os, hipStream_t s) {
+  GOFFT_LAUNCH_EW(k_herm_full, n * nlines, in, out, n, nlines, is, bins, os);
+  return hipGetLastError();
+}
+
+hipError_t launch_take_real(int prec, const void* in, void* out, long long n, long long nlines,
+                            long long is, long long os, double scale, hipStream_t s) {
+  unsigned g = grid_for(n * nlines, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_take_real<double>, dim3(g), dim3(256), 0, s, in, (double*)out, n, nlines, is,
+                       os, scale);
+  else
+    hipLaunchKernelGGL(k_take_real<float>, dim3(g), dim3(256), 0, s, in, (float*)out, n, nlines, is,
+                       os, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_power(int prec, const void* in, void* out, long long bins, long long n,
+                        long long nlines, long long is, long long os, double scale, hipStream_t s) {
+  unsigned g = grid_for(bins * nlines, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_power<double>, dim3(g), dim3(256), 0, s, in, (double*)out, bins, n, nlines, is,
+                       os, scale);
+  else
+    hipLaunchKernelGGL(k_power<float>, dim3(g), dim3(256), 0, s, in, (float*)out, bins, n, nlines, is,
+                       os, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_frames(int prec, const void* x, const void* w, void* out, long long seg,
+                         long long step, long long frames, hipStream_t s) {
+  unsigned g = grid_for(seg * frames, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_frames<double>, dim3(g), dim3(256), 0, s, (const double*)x, (const double*)w,
+                       (double*)out, seg, step, frames);
+  else
+    hipLaunchKernelGGL(k_frames<float>, dim3(g), dim3(256), 0, s, (const float*)x, (const float*)w,
+                       (float*)out, seg, step, frames);
+  return hipGetLastError();
+}
+
+hipError_t launch_axis_copy(int prec, const void* in, void* out, long long n, long long st,
+                            long long lines, int dir, hipStream_t s) {
+  GOFFT_LAUNCH_EW(k_axis_copy, n * lines, in, out, n, st, lines, dir);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(int prec, void* out, long long n, double value, hipStream_t s) {
+  unsigned g = grid_for(n, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, s, (double*)out, n, value);
+  else
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(256), 0, s, (float*)out, n, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale(int prec, const void* in, void* out, long long n, double scale,
+                        hipStream_t s) {
+  unsigned g = grid_for(n, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_scale<double>, dim3(g), dim3(256), 0, s, (const double*)in, (double*)out, n,
+                       scale);
+  else
+    hipLaunchKernelGGL(k_scale<float>, dim3(g), dim3(256), 0, s, (const float*)in, (float*)out, n,
+                       scale);
+  return hipGetLastError();
+}
+
+}  // namespace gofft

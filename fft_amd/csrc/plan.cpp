@@ -1,0 +1,1625 @@
+// Host side of libgofft_hip.so: plan construction (routing, radix schedules,
+// twiddle/chirp/kernel tables), per-stream workspace, execution, and the C ABI
+// declared in include/gofft.h.
+//
+// Routing follows the reference's NewPlan (plan.go:42-73): n <= 1 trivial,
+// 13-smooth lengths by Stockham FFT kernels, primes >= 700 by Rader's
+// algorithm, every other length by Bluestein's chirp-z. All tables are generated
+// here in long double from exact integer phases and rounded once to the working
+// precision.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <functional>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gofft.h"
+#include "kernels.h"
+
+using namespace gofft;
+typedef long long i64;
+typedef std::complex<long double> cld;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Errors.
+
+thread_local std::string t_err;
+
+int fail(int code, const std::string& msg) {
+  t_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(GOFFT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define RC_TRY(expr)            \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != GOFFT_OK) return rc_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device limits.
+
+struct DevInfo {
+  int id = -1;
+  int lds_bytes = 65536;  // usable dynamic LDS per workgroup
+  int cus = 0;
+};
+std::mutex g_dev_mu;
+std::map<int, DevInfo> g_devs;
+
+int current_device(DevInfo* out) {
+  int id = 0;
+  HIP_TRY(hipGetDevice(&id));
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  auto it = g_devs.find(id);
+  if (it == g_devs.end()) {
+    DevInfo d;
+    d.id = id;
+    int lds = 0, cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, id));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, id));
+    // Leave room for the kernels' static LDS (column tables, reductions).
+    d.lds_bytes = lds - 2048;
+    if (const char* e = std::getenv("GOFFT_LDS_BYTES")) {
+      int v = std::atoi(e);
+      if (v > 0 && v < d.lds_bytes) d.lds_bytes = v;
+    }
+    d.cus = cus;
+    it = g_devs.emplace(id, d).first;
+  }
+  *out = it->second;
+  return GOFFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Integer helpers (restated from the reference's plan.go / rader.go; the
+// product keeps its own copy so it never depends on the test oracle).
+
+bool factors_are_small(i64 n) {  // plan.go:134-141
+  for (int p : {2, 3, 5, 7, 11, 13})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+bool is_prime(i64 n) {  // rader.go:232-245
+  if (n < 2) return false;
+  if (n % 2 == 0) return n == 2;
+  for (i64 d = 3; d * d <= n; d += 2)
+    if (n % d == 0) return false;
+  return true;
+}
+
+i64 mod_pow(i64 b, i64 e, i64 m) {  // rader.go:217-228
+  i64 r = 1;
+  b %= m;
+  while (e > 0) {
+    if (e & 1) r = (i64)((__int128)r * b % m);
+    e >>= 1;
+    b = (i64)((__int128)b * b % m);
+  }
+  return r;
+}
+
+i64 primitive_root(i64 p) {  // rader.go:178-214
+  i64 phi = p - 1, m = phi;
+  std::vector<i64> fs;
+  for (i64 d = 2; d * d <= m; d++)
+    if (m % d == 0) {
+      fs.push_back(d);
+      while (m % d == 0) m /= d;
+    }
+  if (m > 1) fs.push_back(m);
+  if (fs.empty()) return 1;
+  for (i64 g = 2;; g++) {
+    bool gen = true;
+    for (i64 f : fs)
+      if (mod_pow(g, phi / f, p) == 1) {
+        gen = false;
+        break;
+      }
+    if (gen) return g;
+  }
+}
+
+const i64 kRaderThreshold = 700;  // plan.go:83
+
+int route(i64 n) {  // plan.go:42-73
+  if (n <= 1) return 0;
+  if ((n & (n - 1)) == 0) return 1;
+  if (factors_are_small(n)) return 2;
+  if (n >= kRaderThreshold && is_prime(n)) return 3;
+  return 4;
+}
+
+// ---------------------------------------------------------------------------
+// Twiddles: W_M^p = exp(-2 pi i p / M), exact integer phase reduced mod M,
+// evaluated in long double.
+
+cld root(i64 p, i64 M) {
+  p %= M;
+  if (p < 0) p += M;
+  if (p == 0) return cld(1, 0);
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  long double a = twopi * (long double)p / (long double)M;
+  return cld(cosl(a), -sinl(a));
+}
+
+// Host long-double FFT (forward, unnormalised) used only at plan time to
+// transform convolution kernels. Recursive mixed radix with a direct DFT for
+// prime factors.
+void host_fft(std::vector<cld>& x) {
+  const i64 n = (i64)x.size();
+  if (n <= 1) return;
+  i64 p = 0;
+  for (i64 d : {4, 2, 3, 5, 7, 11, 13})
+    if (n % d == 0) {
+      p = d;
+      break;
+    }
+  if (p == 0) {
+    for (i64 d = 17; d * d <= n; d += 2)
+      if (n % d == 0) {
+        p = d;
+        break;
+      }
+    if (p == 0) p = n;
+  }
+  const i64 m = n / p;
+  if (m == 1) {
+    std::vector<cld> y(n);
+    for (i64 k = 0; k < n; k++) {
+      cld s = 0;
+      for (i64 j = 0; j < n; j++) s += x[j] * root((j * k) % n, n);
+      y[k] = s;
+    }
+    x.swap(y);
+    return;
+  }
+  std::vector<std::vector<cld>> sub(p, std::vector<cld>(m));
+  for (i64 r = 0; r < p; r++)
+    for (i64 j = 0; j < m; j++) sub[r][j] = x[j * p + r];
+  for (i64 r = 0; r < p; r++) host_fft(sub[r]);
+  std::vector<cld> wp(p);
+  for (i64 q = 0; q < p; q++) wp[q] = root(q, p);
+  for (i64 k1 = 0; k1 < m; k1++) {
+    std::vector<cld> t(p);
+    for (i64 r = 0; r < p; r++) t[r] = sub[r][k1] * root(r * k1, n);
+    for (i64 k2 = 0; k2 < p; k2++) {
+      cld s = 0;
+      for (i64 r = 0; r < p; r++) s += t[r] * wp[(r * k2) % p];
+      x[k1 + m * k2] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Device buffers.
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() {}
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t b) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) return GOFFT_OK;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return fail(GOFFT_ERR_OOM, "hipMalloc of " + std::to_string(b) + " bytes failed");
+    }
+    bytes = b;
+    return GOFFT_OK;
+  }
+  int upload(const void* h, size_t b) {
+    RC_TRY(alloc(b));
+    if (b) HIP_TRY(hipMemcpy(p, h, b, hipMemcpyHostToDevice));
+    return GOFFT_OK;
+  }
+};
+
+size_t elt(int prec) { return prec == 0 ? 16 : 8; }  // complex element bytes
+size_t relt(int prec) { return prec == 0 ? 8 : 4; }  // real element bytes
+
+// Upload complex long-double values in the working precision.
+int upload_complex(DevBuf& b, const std::vector<cld>& v, int prec) {
+  if (prec == 0) {
+    std::vector<double> h(2 * v.size());
+    for (size_t i = 0; i < v.size(); i++) {
+      h[2 * i] = (double)v[i].real();
+      h[2 * i + 1] = (double)v[i].imag();
+    }
+    return b.upload(h.data(), h.size() * sizeof(double));
+  }
+  std::vector<float> h(2 * v.size());
+  for (size_t i = 0; i < v.size(); i++) {
+    h[2 * i] = (float)v[i].real();
+    h[2 * i + 1] = (float)v[i].imag();
+  }
+  return b.upload(h.data(), h.size() * sizeof(float));
+}
+
+int upload_real(DevBuf& b, const std::vector<long double>& v, int prec) {
+  if (prec == 0) {
+    std::vector<double> h(v.begin(), v.end());
+    return b.upload(h.data(), h.size() * sizeof(double));
+  }
+  std::vector<float> h(v.size());
+  for (size_t i = 0; i < v.size(); i++) h[i] = (float)v[i];
+  return b.upload(h.data(), h.size() * sizeof(float));
+}
+
+// ---------------------------------------------------------------------------
+// Sub-FFT schedules.
+
+std::vector<int> radix_schedule(int R) {
+  std::vector<int> r;
+  int x = R;
+  while (x % 16 == 0) {
+    r.push_back(16);
+    x /= 16;
+  }
+  if (x % 8 == 0) {
+    r.push_back(8);
+    x /= 8;
+  }
+  if (x % 4 == 0) {
+    r.push_back(4);
+    x /= 4;
+  }
+  if (x % 2 == 0) {
+    r.push_back(2);
+    x /= 2;
+  }
+  for (int p : {3, 5, 7, 11, 13})
+    while (x % p == 0) {
+      r.push_back(p);
+      x /= p;
+    }
+  if (x != 1) r.clear();
+  return r;
+}
+
+constexpr int kMaxP = 16;  // points per thread per stage (kernel template)
+
+struct SubPlan {
+  SubFFT f{};
+  int threads = 0;
+  std::vector<cld> tw;  // stage twiddles
+};
+
+int ilog2(int x) {
+  int l = 0;
+  while ((1 << l) < x) l++;
+  return l;
+}
+
+// Builds the LDS schedule for C columns of R points; false if impossible.
+bool make_sub(int R, int C, SubPlan& sp) {
+  if (R < 1 || C < 1 || C > kMaxCols || (C & (C - 1))) return false;
+  std::vector<int> rads;
+  if (R > 1) {
+    rads = radix_schedule(R);
+    if (rads.empty()) return false;
+  }
+  if ((int)rads.size() > kMaxStages) return false;
+  int T = 64;
+  while (T < C) T <<= 1;
+  while ((i64)T * kMaxP < (i64)C * R && T < 1024) T <<= 1;
+  bool ok = false;
+  for (; T <= 1024; T <<= 1) {
+    int tpc = T / C;
+    ok = true;
+    for (int rad : rads) {
+      int nb = R / rad;
+      int per = (nb + tpc - 1) / tpc;
+      int cap = kMaxP / rad > 0 ? kMaxP / rad : 1;
+      if (rad > kMaxP || per > cap) {
+        ok = false;
+        break;
+      }
+    }
+    if (ok) break;
+  }
+  if (!ok) return false;
+  sp = SubPlan();
+  sp.threads = T;
+  SubFFT& f = sp.f;
+  f.R = R;
+  f.C = C;
+  f.log2C = ilog2(C);
+  f.tpc = T / C;
+  f.log2tpc = ilog2(f.tpc);
+  f.nstages = (int)rads.size();
+  f.Rd = fastdiv_make((uint32_t)R);
+  int ns = 1, off = 0;
+  for (size_t i = 0; i < rads.size(); i++) {
+    int rad = rads[i];
+    Stage& s = f.st[i];
+    s.radix = rad;
+    s.ns = ns;
+    s.nb = R / rad;
+    s.tw_off = off;
+    s.nsd = fastdiv_make((uint32_t)ns);
+    if (ns > 1) {
+      for (int k = 0; k < ns; k++)
+        for (int j = 1; j < rad; j++) sp.tw.push_back(root((i64)j * k, (i64)ns * rad));
+      off += ns * (rad - 1);
+    }
+    ns *= rad;
+  }
+  return true;
+}
+
+// Largest power-of-two column count <= cap with C*R points within the LDS
+// budget `pts` and a feasible schedule.
+bool pick_cols(int R, int cap, i64 pts, SubPlan& sp, int* Cout) {
+  int C = 1;
+  while (C * 2 <= cap && (i64)C * 2 * R <= pts) C *= 2;
+  for (; C >= 1; C /= 2) {
+    if ((i64)C * R > pts) continue;
+    if (make_sub(R, C, sp)) {
+      *Cout = C;
+      return true;
+    }
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// Engines: a complex transform of length n over `batch` lines.
+
+struct LineIO {
+  const i64* off = nullptr;  // device offsets (elements) per line, or null
+  i64 stride = 0;            // elements between lines when off is null
+};
+
+struct Ctx {
+  hipStream_t s;
+  char* ws;  // workspace for this engine (ws_bytes() bytes)
+};
+
+class Engine {
+ public:
+  int prec = 0;
+  i64 n = 0, batch = 0;
+  virtual ~Engine() {}
+  virtual size_t ws_bytes() const = 0;
+  virtual int launches() const = 0;
+  // out = scale * conj?( FFT( conj?(in) ) ) per line.
+  virtual int run(const void* in, LineIO li, void* out, LineIO lo, int conj_in, int conj_out,
+                  double scale, const Ctx& c) const = 0;
+};
+
+std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, int* rc);
+
+// Trivial engine (n <= 1): out = scale * in (conjugations cancel for n = 1).
+class TrivialEngine : public Engine {
+ public:
+  DevBuf one;
+  int init() {
+    std::vector<cld> v(1, cld(1, 0));
+    return upload_complex(one, v, prec);
+  }
+  size_t ws_bytes() const override { return 0; }
+  int launches() const override { return n == 1 ? 1 : 0; }
+  int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
+          const Ctx& c) const override {
+    if (n != 1 || batch == 0) return GOFFT_OK;
+    HIP_TRY(launch_cmul_table(prec, in, out, one.p, 1, batch, li.stride, lo.stride, ci, co, scale, li.off,
+                              lo.off, c.s));
+    return GOFFT_OK;
+  }
+};
+
+// One pass of a global Stockham decomposition (or the single pass).
+struct Pass {
+  SubPlan sp;
+  i64 Ns = 1, R = 1;
+  ColMap im{}, om{};
+  i64 tw_M = 0;
+  int tw_shift = 0;
+  int tw_digit = 0;
+  size_t stw_off = 0;  // element offset into the stage-twiddle buffer
+  size_t ptw_hi = 0, ptw_lo = 0;  // double2 offsets into the pass-twiddle buffer
+};
+
+// Factor n into pass lengths each <= rmax (13-smooth n), balancing products.
+bool factor_passes(i64 n, i64 rmax, int P, std::vector<i64>& out) {
+  std::vector<i64> primes;
+  i64 x = n;
+  for (int p : {13, 11, 7, 5, 3, 2})
+    while (x % p == 0) {
+      primes.push_back(p);
+      x /= p;
+    }
+  if (x != 1) return false;
+  std::sort(primes.begin(), primes.end(), [](i64 a, i64 b) { return a > b; });
+  std::vector<i64> prod(P, 1);
+  for (i64 p : primes) {
+    int best = -1;
+    for (int i = 0; i < P; i++)
+      if (prod[i] * p <= rmax && (best < 0 || prod[i] < prod[best])) best = i;
+    if (best < 0) return false;
+    prod[best] *= p;
+  }
+  std::sort(prod.begin(), prod.end(), [](i64 a, i64 b) { return a > b; });
+  while (!prod.empty() && prod.back() == 1) prod.pop_back();
+  out = prod;
+  return !out.empty();
+}
+
+// Stockham engine over lines that may be strided: a line is n elements at
+// element stride `st`; `inner` lines share an outer block (adjacent in memory),
+// For a plain 1-D batch st = 1 and inner = 1.
+class StockhamEngine : public Engine {
+ public:
+  i64 st = 1;        // element stride along the transform axis
+  i64 inner = 1;     // lines interleaved at distance 1 (the axis stride)
+  std::vector<Pass> passes;
+  DevBuf stw, ptw;
+  int P() const { return (int)passes.size(); }
+
+  int init(unsigned flags) {
+    (void)flags;
+    DevInfo dev;
+    RC_TRY(current_device(&dev));
+    const i64 pts = dev.lds_bytes / (i64)elt(prec);
+    const i64 lines = batch * inner;  // columns of a single pass
+    // Single pass: all n points of a line in one workgroup. Strided lines need
+    // several adjacent lines per workgroup for coalesced access.
+    const i64 target = std::max<i64>(n, std::min<i64>(pts, prec == 0 ? 4096 : 8192));
+    const int cwant = st == 1 ? 1 : (int)std::min<i64>(prec == 0 ? 8 : 16, st);
+    const int ccap = (int)std::min<i64>(kMaxCols, std::max<i64>(cwant, lines));
+    SubPlan sp;
+    int C = 0;
+    if (n <= pts && (i64)cwant * n <= std::max<i64>(pts, n) && pick_cols((int)n, ccap, target, sp, &C) &&
+        C >= cwant) {
+      Pass p;
+      p.sp = sp;
+      p.R = n;
+      p.Ns = 1;
+      passes.push_back(p);
+    } else {
+      // Multi-pass: pass lengths <= rmax with cmp columns per workgroup.
+      int cmp = prec == 0 ? 8 : 16;
+      i64 rmax = std::min<i64>(pts / cmp, 1024);
+      std::vector<i64> lens;
+      bool ok = false;
+      for (int Pn = 2; Pn <= 6 && !ok; Pn++) ok = factor_passes(n, rmax, Pn, lens);
+      if (!ok) return fail(GOFFT_ERR_UNSUPPORTED, "no pass decomposition for n=" + std::to_string(n));
+      i64 Ns = 1;
+      for (i64 R : lens) {
+        Pass p;
+        int Cp = 0;
+        if (!pick_cols((int)R, cmp, pts, p.sp, &Cp))
+          return fail(GOFFT_ERR_UNSUPPORTED, "no LDS schedule for pass length " + std::to_string(R));
+        p.R = R;
+        p.Ns = Ns;
+        passes.push_back(p);
+        Ns *= R;
+      }
+    }
+    // Column maps and tables.
+    std::vector<cld> stw_h;
+    std::vector<cld> ptw_h;
+    for (size_t i = 0; i < passes.size(); i++) {
+      Pass& p = passes[i];
+      const i64 R = p.R, Ns = p.Ns;
+      const i64 D2 = n / (R * Ns);
+      // digits: d0 = inner, d1 = j % Ns, d2 = j / Ns, d3 = outer line (batch)
+      p.im.D0 = inner;
+      p.im.D1 = Ns;
+      p.im.D2 = D2;
+      p.im.a0 = 1;
+      p.im.a1 = st;
+      p.im.a2 = Ns * st;
+      p.im.a3 = n * st;  // replaced by the caller's line stride for 1-D
+      p.im.g = (n / R) * st;
+      p.im.rowmajor = p.im.g == 1;
+      p.om = p.im;
+      p.om.a2 = Ns * R * st;
+      p.om.g = Ns * st;
+      p.om.rowmajor = p.om.g == 1;
+      p.stw_off = stw_h.size();
+      stw_h.insert(stw_h.end(), p.sp.tw.begin(), p.sp.tw.end());
+      if (Ns > 1) {
+        const i64 M = Ns * R;
+        int sh = (ilog2((int)std::min<i64>(M, 1 << 30)) + 1) / 2;
+        i64 S = 1ll << sh;
+        i64 nh = (M + S - 1) / S;
+        p.tw_M = M;
+        p.tw_shift = sh;
+        p.tw_digit = 1;
+        p.ptw_hi = ptw_h.size();
+        for (i64 h = 0; h < nh; h++) ptw_h.push_back(root(h * S, M));
+        p.ptw_lo = ptw_h.size();
+        for (i64 l = 0; l < S; l++) ptw_h.push_back(root(l, M));
+      }
+    }
+    if (stw_h.empty()) stw_h.push_back(cld(1, 0));
+    RC_TRY(upload_complex(stw, stw_h, prec));
+    if (!ptw_h.empty()) RC_TRY(upload_complex(ptw, ptw_h, 0));
+    return GOFFT_OK;
+  }
+
+  size_t ws_bytes() const override {
+    // ping-pong copies of the whole (strided) array
+    size_t one = (size_t)n * (size_t)inner * (size_t)batch * elt(prec);
+    if (P() <= 1) return 0;
+    return P() == 2 ? one : 2 * one;
+  }
+  int launches() const override { return P(); }
+
+  int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
+          const Ctx& c) const override {
+    if (batch == 0) return GOFFT_OK;
+    const size_t one = (size_t)n * (size_t)inner * (size_t)batch * elt(prec);
+    char* bufs[2] = {c.ws, c.ws + one};
+    for (int i = 0; i < P(); i++) {
+      const Pass& p = passes[i];
+      PassArgs a{};
+      a.in = i == 0 ? in : bufs[(i - 1) % 2];
+      a.out = i == P() - 1 ? out : bufs[i % 2];
+      a.stw = (const char*)stw.p + p.stw_off * elt(prec);
+      a.ptw_hi = p.tw_M ? (const double2*)ptw.p + p.ptw_hi : nullptr;
+      a.ptw_lo = p.tw_M ? (const double2*)ptw.p + p.ptw_lo : nullptr;
+      a.ncols = batch * inner * (n / p.R);
+      a.tw_M = p.tw_M;
+      a.tw_shift = p.tw_shift;
+      a.tw_digit = p.tw_digit;
+      a.conj_in = i == 0 ? ci : 0;
+      a.conj_out = i == P() - 1 ? co : 0;
+      a.scale = i == P() - 1 ? scale : 1.0;
+      a.im = p.im;
+      a.om = p.om;
+      if (i == 0) {
+        a.in_off = li.off;
+        if (inner == 1) a.im.a3 = li.stride;
+      }
+      if (i == P() - 1) {
+        a.out_off = lo.off;
+        if (inner == 1) a.om.a3 = lo.stride;
+      }
+      a.f = p.sp.f;
+      HIP_TRY(launch_pass(prec, kMaxP, a, p.sp.threads, c.s));
+    }
+    return GOFFT_OK;
+  }
+};
+
+// Rader's algorithm for prime n (rader.go:39-172): gather by the generator
+// permutation, cyclic convolution with the precomputed kernel spectrum,
+// scatter by the inverse permutation.
+class RaderEngine : public Engine {
+ public:
+  i64 q = 0, cl = 0;
+  bool cyclic = true;
+  DevBuf perm, iperm, spec;
+  // fused path
+  bool fused = false;
+  SubPlan sp;
+  DevBuf stw;
+  // multi-kernel path
+  std::unique_ptr<Engine> inner;
+
+  int init(unsigned flags) {
+    q = n - 1;
+    const i64 g = primitive_root(n);
+    if (factors_are_small(q)) {
+      cyclic = true;
+      cl = q;
+    } else {
+      cyclic = false;
+      cl = choose_conv_len(2 * q);
+    }
+    std::vector<int> ph(q), iph(q);
+    i64 x = 1;
+    for (i64 k = 0; k < q; k++) {
+      ph[k] = (int)x;
+      x = x * g % n;
+    }
+    for (i64 k = 0; k < q; k++) iph[k] = ph[(q - k) % q];
+    RC_TRY(perm.upload(ph.data(), ph.size() * sizeof(int)));
+    RC_TRY(iperm.upload(iph.data(), iph.size() * sizeof(int)));
+    // kernel ker[m] = W_n^{perm[(q - m) mod q]} replicated over 2q for the
+    // linear case (rader.go:101-125); spectrum scaled by 1/cl.
+    std::vector<cld> b(cl, cld(0, 0));
+    i64 lim = cyclic ? q : 2 * q;
+    for (i64 m = 0; m < lim; m++) b[m] = root(ph[(q - (m % q)) % q], n);
+    host_fft(b);
+    for (auto& v : b) v /= (long double)cl;
+    RC_TRY(upload_complex(spec, b, prec));
+    DevInfo dev;
+    RC_TRY(current_device(&dev));
+    const i64 pts = dev.lds_bytes / (i64)elt(prec);
+    int C = 0;
+    if (cl <= pts && pick_cols((int)cl, 16, std::min<i64>(pts, std::max<i64>(cl, prec == 0 ? 4096 : 8192)), sp, &C)) {
+      fused = true;
+      RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
+    } else {
+      int rc = 0;
+      inner = make_engine(cl, batch, prec, flags, &rc);
+      if (!inner) return rc;
+    }
+    return GOFFT_OK;
+  }
+
+  // Smallest 7-smooth length in [lo, ceil(1.4 lo)] that is cheapest for the
+  // GPU: fewest kernel passes first, then the reference's cost model
+  // (plan.go:192-259).
+  i64 choose_conv_len(i64 lo) {
+    DevInfo dev;
+    if (current_device(&dev) != GOFFT_OK) dev.lds_bytes = 65536;
+    const i64 pts = dev.lds_bytes / (i64)elt(prec);
+    i64 hi = lo + (lo * 4 + 9) / 10;
+    i64 best = -1;
+    double bestc = 0;
+    int bestfit = 0;
+    for (i64 m = lo; m <= hi; m++) {
+      i64 x = m;
+      double cost = 0;
+      while (x % 4 == 0) { cost += 2.0; x /= 4; }
+      while (x % 2 == 0) { cost += 2.0; x /= 2; }
+      while (x % 3 == 0) { cost += 2.0; x /= 3; }
+      while (x % 5 == 0) { cost += 2.8; x /= 5; }
+      while (x % 7 == 0) { cost += 2.6; x /= 7; }
+      if (x != 1) continue;
+      cost *= (double)m;
+      SubPlan tmp;
+      int fit = (m <= pts && make_sub((int)m, 1, tmp)) ? 1 : 0;
+      if (best < 0 || fit > bestfit || (fit == bestfit && cost < bestc)) {
+        best = m;
+        bestc = cost;
+        bestfit = fit;
+      }
+    }
+    return best;
+  }
+
+  size_t ws_bytes() const override {
+    if (fused) return 0;
+    return 2 * (size_t)cl * batch * elt(prec) + inner->ws_bytes();
+  }
+  int launches() const override { return fused ? 1 : 3 + 2 * inner->launches(); }
+
+  int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
+          const Ctx& c) const override {
+    if (batch == 0) return GOFFT_OK;
+    // Inverse transforms use conj(DFT(conj(x))): ci and co are both set.
+    if (ci != co) return fail(GOFFT_ERR_UNSUPPORTED, "Rader engine needs matching conjugations");
+    if (fused) {
+      ConvArgs a{};
+      a.in = in;
+      a.out = out;
+      a.stw = stw.p;
+      a.spec = spec.p;
+      a.perm = (const int*)perm.p;
+      a.iperm = (const int*)iperm.p;
+      a.in_off = li.off;
+      a.out_off = lo.off;
+      a.nlines = batch;
+      a.in_stride = li.stride;
+      a.out_stride = lo.stride;
+      a.n = (int)n;
+      a.kind = cyclic ? 0 : 1;
+      a.conj_io = ci;
+      a.scale = scale;
+      a.f = sp.f;
+      HIP_TRY(launch_conv(prec, kMaxP, a, sp.threads, c.s));
+      return GOFFT_OK;
+    }
+    char* A = c.ws;
+    char* B = c.ws + (size_t)cl * batch * elt(prec);
+    Ctx ic{c.s, B + (size_t)cl * batch * elt(prec)};
+    LineIO cont{nullptr, cl};
+    HIP_TRY(launch_rader_gather(prec, in, A, (const int*)perm.p, q, cl, batch, li.stride, cl, ci, li.off,
+                                c.s));
+    RC_TRY(inner->run(A, cont, B, cont, 0, 0, 1.0, ic));
+    HIP_TRY(launch_cmul_table(prec, B, A, spec.p, cl, batch, cl, cl, 0, 1, 1.0, nullptr, nullptr, c.s));
+    RC_TRY(inner->run(A, cont, B, cont, 0, 1, 1.0, ic));
+    HIP_TRY(launch_rader_scatter(prec, in, B, out, (const int*)iperm.p, n, cyclic ? 0 : q, batch,
+                                 li.stride, cl, lo.stride, co, scale, li.off, lo.off, c.s));
+    return GOFFT_OK;
+  }
+};
+
+// Bluestein's chirp-z (bluestein.go:17-95): x*w zero-padded to a power of two
+// m >= 2n-1, convolution with the conj-chirp kernel spectrum, times w.
+class BluesteinEngine : public Engine {
+ public:
+  i64 m = 0;
+  DevBuf chirp, spec;
+  bool fused = false;
+  SubPlan sp;
+  DevBuf stw;
+  std::unique_ptr<Engine> inner;
+
+  int init(unsigned flags) {
+    m = 1;
+    while (m < 2 * n - 1) m <<= 1;
+    std::vector<cld> w(n), b(m, cld(0, 0));
+    for (i64 j = 0; j < n; j++) {
+      i64 jj = (i64)((__int128)j * j % (2 * n));  // bit-exact chirp phase index
+      w[j] = root(jj, 2 * n);                      // exp(-pi i jj / n)
+    }
+    b[0] = std::conj(w[0]);
+    for (i64 j = 1; j < n; j++) {
+      b[j] = std::conj(w[j]);
+      b[m - j] = std::conj(w[j]);
+    }
+    host_fft(b);
+    for (auto& v : b) v /= (long double)m;
+    RC_TRY(upload_complex(chirp, w, prec));
+    RC_TRY(upload_complex(spec, b, prec));
+    DevInfo dev;
+    RC_TRY(current_device(&dev));
+    const i64 pts = dev.lds_bytes / (i64)elt(prec);
+    int C = 0;
+    if (m <= pts && pick_cols((int)m, 16, std::min<i64>(pts, std::max<i64>(m, prec == 0 ? 4096 : 8192)), sp, &C)) {
+      fused = true;
+      RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
+    } else {
+      int rc = 0;
+      inner = make_engine(m, batch, prec, flags, &rc);
+      if (!inner) return rc;
+    }
+    return GOFFT_OK;
+  }
+
+  size_t ws_bytes() const override {
+    if (fused) return 0;
+    return 2 * (size_t)m * batch * elt(prec) + inner->ws_bytes();
+  }
+  int launches() const override { return fused ? 1 : 3 + 2 * inner->launches(); }
+
+  int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
+          const Ctx& c) const override {
+    if (batch == 0) return GOFFT_OK;
+    if (ci != co) return fail(GOFFT_ERR_UNSUPPORTED, "Bluestein engine needs matching conjugations");
+    if (fused) {
+      ConvArgs a{};
+      a.in = in;
+      a.out = out;
+      a.stw = stw.p;
+      a.spec = spec.p;
+      a.chirp = chirp.p;
+      a.in_off = li.off;
+      a.out_off = lo.off;
+      a.nlines = batch;
+      a.in_stride = li.stride;
+      a.out_stride = lo.stride;
+      a.n = (int)n;
+      a.kind = 2;
+      a.conj_io = ci;
+      a.scale = scale;
+      a.f = sp.f;
+      HIP_TRY(launch_conv(prec, kMaxP, a, sp.threads, c.s));
+      return GOFFT_OK;
+    }
+    char* A = c.ws;
+    char* B = c.ws + (size_t)m * batch * elt(prec);
+    Ctx ic{c.s, B + (size_t)m * batch * elt(prec)};
+    LineIO cont{nullptr, m};
+    HIP_TRY(launch_chirp_in(prec, in, A, chirp.p, n, m, batch, li.stride, m, ci, li.off, c.s));
+    RC_TRY(inner->run(A, cont, B, cont, 0, 0, 1.0, ic));
+    HIP_TRY(launch_cmul_table(prec, B, A, spec.p, m, batch, m, m, 0, 1, 1.0, nullptr, nullptr, c.s));
+    RC_TRY(inner->run(A, cont, B, cont, 0, 1, 1.0, ic));
+    HIP_TRY(launch_cmul_table(prec, B, out, chirp.p, n, batch, m, lo.stride, 0, co, scale, nullptr, lo.off,
+                              c.s));
+    return GOFFT_OK;
+  }
+};
+
+std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, int* rc) {
+  *rc = GOFFT_OK;
+  int r = route(n);
+  if (r == 0) {
+    auto e = std::make_unique<TrivialEngine>();
+    e->prec = prec;
+    e->n = n;
+    e->batch = batch;
+    *rc = e->init();
+    return *rc == GOFFT_OK ? std::move(e) : nullptr;
+  }
+  if (r == 1 || r == 2) {
+    auto e = std::make_unique<StockhamEngine>();
+    e->prec = prec;
+    e->n = n;
+    e->batch = batch;
+    *rc = e->init(flags);
+    return *rc == GOFFT_OK ? std::move(e) : nullptr;
+  }
+  if (r == 3) {
+    auto e = std::make_unique<RaderEngine>();
+    e->prec = prec;
+    e->n = n;
+    e->batch = batch;
+    *rc = e->init(flags);
+    return *rc == GOFFT_OK ? std::move(e) : nullptr;
+  }
+  auto e = std::make_unique<BluesteinEngine>();
+  e->prec = prec;
+  e->n = n;
+  e->batch = batch;
+  *rc = e->init(flags);
+  return *rc == GOFFT_OK ? std::move(e) : nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Real-input engine (realplan.go): even n packs pairs into an n/2-point complex
+// transform plus an untangle; odd n promotes to a full complex transform.
+
+class RealEngine {
+ public:
+  int prec = 0;
+  i64 n = 0, batch = 0, m = 0;
+  bool even = false, single = false;
+  SubPlan sp;
+  DevBuf stw, utw;
+  std::unique_ptr<Engine> half;  // m-point (even, multi) or n-point (odd) complex engine
+
+  int init(unsigned flags) {
+    even = n >= 2 && n % 2 == 0;
+    m = n / 2;
+    if (even) {
+      std::vector<cld> u(m + 1);
+      for (i64 k = 0; k <= m; k++) u[k] = root(k, n);
+      RC_TRY(upload_complex(utw, u, prec));
+      DevInfo dev;
+      RC_TRY(current_device(&dev));
+      const i64 pts = dev.lds_bytes / (i64)elt(prec);
+      i64 target = std::min<i64>(pts, prec == 0 ? 4096 : 8192);
+      int C = 0;
+      if (m <= target && factors_are_small(m) &&
+          pick_cols((int)m, (int)std::min<i64>(kMaxCols, std::max<i64>(1, batch)), std::max<i64>(m, target), sp, &C)) {
+        single = true;
+        RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
+        return GOFFT_OK;
+      }
+      int rc = 0;
+      half = make_engine(m, batch, prec, flags, &rc);
+      return half ? GOFFT_OK : rc;
+    }
+    int rc = 0;
+    half = make_engine(n, batch, prec, flags, &rc);
+    return half ? GOFFT_OK : rc;
+  }
+
+  size_t ws_bytes() const {
+    if (single) return 0;
+    size_t hw = half ? half->ws_bytes() : 0;
+    if (even) return (size_t)m * batch * elt(prec) + hw;
+    return 2 * (size_t)n * batch * elt(prec) + hw;
+  }
+  int launches() const {
+    if (single) return 1;
+    return half->launches() + (even ? 1 : 2);
+  }
+
+  // mode 0: complex bins (stride ostride per line), mode 1: PSD with psd_scale.
+  int forward(const void* in, i64 istride, void* out, i64 ostride, int mode, double psd_scale,
+              const void* window, const Ctx& c) const {
+    if (batch == 0 || n == 0) return GOFFT_OK;
+    if (single) {
+      RealFwdArgs a{};
+      a.in = in;
+      a.out = out;
+      a.stw = stw.p;
+      a.utw = utw.p;
+      a.window = window;
+      a.nlines = batch;
+      a.in_stride = istride;
+      a.out_stride = ostride;
+      a.mode = mode;
+      a.psd_scale = psd_scale;
+      a.f = sp.f;
+      HIP_TRY(launch_real_fwd(prec, kMaxP, a, sp.threads, c.s));
+      return GOFFT_OK;
+    }
+    if (window) return fail(GOFFT_ERR_UNSUPPORTED, "window needs the single-pass real kernel");
+    if (even) {
+      if (istride % 2) return fail(GOFFT_ERR_UNSUPPORTED, "odd real line stride");
+      char* Z = c.ws;
+      Ctx ic{c.s, Z + (size_t)m * batch * elt(prec)};
+      RC_TRY(half->run(in, LineIO{nullptr, istride / 2}, Z, LineIO{nullptr, m}, 0, 0, 1.0, ic));
+      HIP_TRY(launch_untangle(prec, Z, out, utw.p, m, batch, m, ostride, mode, psd_scale, c.s));
+      return GOFFT_OK;
+    }
+    char* A = c.ws;
+    char* B = c.ws + (size_t)n * batch * elt(prec);
+    Ctx ic{c.s, B + (size_t)n * batch * elt(prec)};
+    HIP_TRY(launch_promote(prec, in, A, n, batch, istride, n, c.s));
+    RC_TRY(half->run(A, LineIO{nullptr, n}, B, LineIO{nullptr, n}, 0, 0, 1.0, ic));
+    if (mode == 0)
+      HIP_TRY(launch_copy_lines(prec, B, out, n / 2 + 1, batch, n, ostride, c.s));
+    else
+      HIP_TRY(launch_power(prec, B, out, n / 2 + 1, n, batch, n, ostride, psd_scale, c.s));
+    return GOFFT_OK;
+  }
+
+  // IRFFT: `bins` bins per input line (stride istride), n reals per output line.
+  int inverse(const void* in, i64 istride, i64 bins, void* out, i64 ostride, const Ctx& c) const {
+    if (batch == 0 || n <= 0) return GOFFT_OK;
+    if (single) {
+      RealInvArgs a{};
+      a.in = in;
+      a.out = out;
+      a.stw = stw.p;
+      a.utw = utw.p;
+      a.nlines = batch;
+      a.in_stride = istride;
+      a.in_bins = bins;
+      a.out_stride = ostride;
+      a.scale = 1.0 / (double)m;
+      a.f = sp.f;
+      HIP_TRY(launch_real_inv(prec, kMaxP, a, sp.threads, c.s));
+      return GOFFT_OK;
+    }
+    if (even) {
+      if (ostride % 2) return fail(GOFFT_ERR_UNSUPPORTED, "odd real line stride");
+      char* Z = c.ws;
+      Ctx ic{c.s, Z + (size_t)m * batch * elt(prec)};
+      HIP_TRY(launch_herm_pack(prec, in, Z, utw.p, m, batch, istride, bins, m, c.s));
+      RC_TRY(half->run(Z, LineIO{nullptr, m}, out, LineIO{nullptr, ostride / 2}, 1, 1, 1.0 / (double)m, ic));
+      return GOFFT_OK;
+    }
+    char* A = c.ws;
+    char* B = c.ws + (size_t)n * batch * elt(prec);
+    Ctx ic{c.s, B + (size_t)n * batch * elt(prec)};
+    HIP_TRY(launch_herm_full(prec, in, A, n, batch, istride, bins, n, c.s));
+    RC_TRY(half->run(A, LineIO{nullptr, n}, B, LineIO{nullptr, n}, 1, 1, 1.0 / (double)n, ic));
+    HIP_TRY(launch_take_real(prec, B, out, n, batch, n, ostride, 1.0, c.s));
+    return GOFFT_OK;
+  }
+};
+
+struct NDAxis {
+  i64 n = 1, st = 1, outer = 1;
+  std::unique_ptr<Engine> eng;  // strided Stockham, or contiguous engine
+  bool gather = false;          // non-smooth strided axis: copy lines out and back
+};
+
+// ---------------------------------------------------------------------------
+// Per-stream workspace.
+
+struct WorkspacePool {
+  std::mutex mu;
+  std::map<hipStream_t, std::unique_ptr<DevBuf>> bufs;
+  int get(hipStream_t s, size_t bytes, char** out) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto& b = bufs[s];
+    if (!b) b = std::make_unique<DevBuf>();
+    if (b->bytes < bytes) RC_TRY(b->alloc(bytes));
+    *out = (char*)b->p;
+    return GOFFT_OK;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Public plan object.
+
+enum PlanKind { K_C2C = 0, K_MANY, K_R2C, K_ND_C2C, K_ND_R2C, K_PSD, K_SPEC };
+
+struct gofft_plan {
+  PlanKind kind = K_C2C;
+  int prec = 0;
+  unsigned flags = 0;
+  i64 n = 0, batch = 0;
+  // C2C
+  std::unique_ptr<Engine> eng;
+  // MANY
+  struct Group {
+    i64 len = 0;
+    std::unique_ptr<Engine> eng;
+    DevBuf off;
+  };
+  std::vector<std::unique_ptr<Group>> groups;
+  i64 total_elems = 0;
+  // R2C / PSD / SPEC
+  std::unique_ptr<RealEngine> real;
+  double d = 1.0;
+  // ND
+  std::vector<i64> shape;
+  std::vector<NDAxis> axes;
+  i64 nd_total = 1;
+  std::unique_ptr<RealEngine> nd_row;   // R2C rows
+  i64 rows = 0, cols = 0, rcols = 0;
+  // SPEC
+  i64 seg = 0, step = 0, frames = 0, sig_len = 0;
+  DevBuf window;
+  bool spec_gather = false;
+  size_t ws_need = 0;
+  mutable WorkspacePool pool;
+
+  int ws(hipStream_t s, char** out) const {
+    if (ws_need == 0) {
+      *out = nullptr;
+      return GOFFT_OK;
+    }
+    return pool.get(s, ws_need, out);
+  }
+};
+
+namespace {
+
+int new_plan(gofft_plan** out, std::unique_ptr<gofft_plan>& p) {
+  *out = p.release();
+  return GOFFT_OK;
+}
+
+int check_dtype(int dtype) {
+  if (dtype != GOFFT_C128 && dtype != GOFFT_C64)
+    return fail(GOFFT_ERR_INVALID_ARG, "dtype must be GOFFT_C128 or GOFFT_C64");
+  return GOFFT_OK;
+}
+
+unsigned default_flags(int dtype, unsigned flags) {
+  (void)dtype;
+  return flags;
+}
+
+// One axis of an N-D transform on a row-major array: `outer` blocks of n
+// elements at stride st, each block holding st interleaved lines.
+int build_axis(int prec, unsigned flags, i64 n, i64 st, i64 outer, NDAxis& a, size_t* ws_need) {
+  a.n = n;
+  a.st = st;
+  a.outer = outer;
+  if (n == 1) return GOFFT_OK;
+  int rc = GOFFT_OK;
+  if (st == 1) {
+    a.eng = make_engine(n, outer, prec, flags, &rc);
+  } else if (factors_are_small(n)) {
+    auto e = std::make_unique<StockhamEngine>();
+    e->prec = prec;
+    e->n = n;
+    e->batch = outer;
+    e->st = st;
+    e->inner = st;
+    rc = e->init(flags);
+    if (rc == GOFFT_OK) a.eng = std::move(e);
+  } else {
+    a.gather = true;
+    a.eng = make_engine(n, outer * st, prec, flags, &rc);
+  }
+  if (!a.eng) return rc;
+  size_t need = a.eng->ws_bytes();
+  if (a.gather) need += 2 * (size_t)n * st * outer * elt(prec);
+  *ws_need = std::max(*ws_need, need);
+  return GOFFT_OK;
+}
+
+int build_nd_axes(int prec, unsigned flags, const std::vector<i64>& shape, i64 batch,
+                  std::vector<NDAxis>& axes, size_t* ws_need) {
+  const int rank = (int)shape.size();
+  i64 total = 1;
+  for (i64 s : shape) total *= s;
+  for (int ax = 0; ax < rank; ax++) {
+    i64 st = 1;
+    for (int k = ax + 1; k < rank; k++) st *= shape[k];
+    NDAxis a;
+    RC_TRY(build_axis(prec, flags, shape[ax], st, total / (shape[ax] * st) * batch, a, ws_need));
+    axes.push_back(std::move(a));
+  }
+  return GOFFT_OK;
+}
+
+// Runs the axes of an N-D complex transform: src -> dst on the first
+// non-trivial axis, then in place on dst. `total` is the element count of the
+// whole (batched) array. An inverse conjugates on the first load and the last
+// store only: conj(F_all(conj x)) composes per axis.
+int run_nd_axes(const std::vector<NDAxis>& axes, int prec, i64 total, const void* src, void* dst,
+                int inverse, double final_scale, hipStream_t s, char* ws) {
+  const void* cur = src;
+  int last = -1;
+  for (int i = 0; i < (int)axes.size(); i++)
+    if (axes[i].n > 1) last = i;
+  if (last < 0) {
+    if (total > 0 && src != dst) HIP_TRY(launch_copy_lines(prec, src, dst, total, 1, total, total, s));
+    if (total > 0 && final_scale != 1.0) HIP_TRY(launch_scale(prec, dst, dst, 2 * total, final_scale, s));
+    return GOFFT_OK;
+  }
+  int first = 1;
+  for (int i = 0; i < (int)axes.size(); i++) {
+    const NDAxis& a = axes[i];
+    if (a.n == 1) continue;
+    int ci = inverse && first ? 1 : 0;
+    int co = inverse && i == last ? 1 : 0;
+    double sc = i == last ? final_scale : 1.0;
+    if (a.st == 1) {
+      Ctx c{s, ws};
+      RC_TRY(a.eng->run(cur, LineIO{nullptr, a.n}, dst, LineIO{nullptr, a.n}, ci, co, sc, c));
+    } else if (!a.gather) {
+      Ctx c{s, ws};
+      RC_TRY(a.eng->run(cur, LineIO{nullptr, 0}, dst, LineIO{nullptr, 0}, ci, co, sc, c));
+    } else {
+      const i64 elems = a.n * a.st * a.outer;
+      char* A = ws;
+      char* B = ws + (size_t)elems * elt(prec);
+      Ctx c{s, B + (size_t)elems * elt(prec)};
+      i64 lines = elems / a.n;
+      HIP_TRY(launch_axis_copy(prec, cur, A, a.n, a.st, lines, 0, s));
+      RC_TRY(a.eng->run(A, LineIO{nullptr, a.n}, B, LineIO{nullptr, a.n}, ci, co, sc, c));
+      HIP_TRY(launch_axis_copy(prec, B, dst, a.n, a.st, lines, 1, s));
+    }
+    cur = dst;
+    first = 0;
+  }
+  return GOFFT_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI.
+
+extern "C" {
+
+int gofft_init(void) {
+  DevInfo d;
+  return current_device(&d);
+}
+
+const char* gofft_last_error(void) { return t_err.c_str(); }
+
+const char* gofft_version(void) { return "gofft-hip 0.1 (gfx950)"; }
+
+int gofft_route(int64_t n) { return route(n); }
+
+int gofft_plan_c2c(gofft_plan** plan, int64_t n, int64_t batch, int dtype, unsigned flags) {
+  if (!plan) return fail(GOFFT_ERR_INVALID_ARG, "null plan pointer");
+  *plan = nullptr;
+  RC_TRY(check_dtype(dtype));
+  if (n < 0 || batch < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative length or batch");
+  RC_TRY(gofft_init());
+  auto p = std::make_unique<gofft_plan>();
+  p->kind = K_C2C;
+  p->prec = dtype;
+  p->flags = default_flags(dtype, flags);
+  p->n = n;
+  p->batch = batch;
+  int rc = GOFFT_OK;
+  p->eng = make_engine(n, batch, dtype, p->flags, &rc);
+  if (!p->eng) return rc;
+  p->ws_need = p->eng->ws_bytes();
+  return new_plan(plan, p);
+}
+
+int gofft_plan_c2c_many(gofft_plan** plan, const int64_t* lengths, int64_t count, int dtype,
+                        unsigned flags) {
+  if (!plan) return fail(GOFFT_ERR_INVALID_ARG, "null plan pointer");
+  *plan = nullptr;
+  RC_TRY(check_dtype(dtype));
+  if (count < 0 || (count > 0 && !lengths)) return fail(GOFFT_ERR_INVALID_ARG, "bad length list");
+  RC_TRY(gofft_init());
+  auto p = std::make_unique<gofft_plan>();
+  p->kind = K_MANY;
+  p->prec = dtype;
+  p->flags = default_flags(dtype, flags);
+  p->batch = count;
+  std::map<i64, std::vector<i64>> offs;
+  i64 off = 0;
+  for (i64 i = 0; i < count; i++) {
+    if (lengths[i] < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative length");
+    offs[lengths[i]].push_back(off);
+    off += lengths[i];
+  }
+  p->total_elems = off;
+  // Largest lengths first so the long transforms start early on the stream.
+  std::vector<i64> lens;
+  for (auto& kv : offs) lens.push_back(kv.first);
+  std::sort(lens.begin(), lens.end(), [](i64 a, i64 b) { return a > b; });
+  for (i64 len : lens) {
+    if (len == 0) continue;
+    auto g = std::make_unique<gofft_plan::Group>();
+    g->len = len;
+    const auto& o = offs[len];
+    int rc = GOFFT_OK;
+    g->eng = make_engine(len, (i64)o.size(), dtype, p->flags, &rc);
+    if (!g->eng) return rc;
+    RC_TRY(g->off.upload(o.data(), o.size() * sizeof(i64)));
+    p->ws_need = std::max(p->ws_need, g->eng->ws_bytes());
+    p->groups.push_back(std::move(g));
+  }
+  return new_plan(plan, p);
+}
+
+int gofft_plan_r2c(gofft_plan** plan, int64_t n, int64_t batch, int dtype, unsigned flags) {
+  if (!plan) return fail(GOFFT_ERR_INVALID_ARG, "null plan pointer");
+  *plan = nullptr;
+  RC_TRY(check_dtype(dtype));
+  if (n < 0 || batch < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative length or batch");
+  RC_TRY(gofft_init());
+  auto p = std::make_unique<gofft_plan>();
+  p->kind = K_R2C;
+  p->prec = dtype;
+  p->flags = default_flags(dtype, flags);
+  p->n = n;
+  p->batch = batch;
+  p->real = std::make_unique<RealEngine>();
+  p->real->prec = dtype;
+  p->real->n = n;
+  p->real->batch = batch;
+  if (n > 0) RC_TRY(p->real->init(p->flags));
+  p->ws_need = n > 0 ? p->real->ws_bytes() : 0;
+  return new_plan(plan, p);
+}
+
+int gofft_plan_psd(gofft_plan** plan, int64_t n, int64_t batch, double d, int dtype, unsigned flags) {
+  int rc = gofft_plan_r2c(plan, n, batch, dtype, flags);
+  if (rc != GOFFT_OK) return rc;
+  (*plan)->kind = K_PSD;
+  (*plan)->d = d;
+  return GOFFT_OK;
+}
+
+int gofft_plan_nd(gofft_plan** plan, const int64_t* shape, int rank, int kind, int64_t batch,
+                  int dtype, unsigned flags) {
+  if (!plan) return fail(GOFFT_ERR_INVALID_ARG, "null plan pointer");
+  *plan = nullptr;
+  RC_TRY(check_dtype(dtype));
+  if (rank < 0 || (rank > 0 && !shape) || batch < 0) return fail(GOFFT_ERR_INVALID_ARG, "bad shape");
+  for (int i = 0; i < rank; i++)
+    if (shape[i] <= 0) return fail(GOFFT_ERR_SHAPE, "fft: shape lengths must be positive");
+  RC_TRY(gofft_init());
+  auto p = std::make_unique<gofft_plan>();
+  p->prec = dtype;
+  p->flags = default_flags(dtype, flags);
+  p->batch = batch;
+  p->shape.assign(shape, shape + rank);
+  p->nd_total = 1;
+  for (int i = 0; i < rank; i++) p->nd_total *= shape[i];
+  if (kind == GOFFT_ND_C2C) {
+    p->kind = K_ND_C2C;
+    p->n = p->nd_total;
+    RC_TRY(build_nd_axes(dtype, p->flags, p->shape, batch, p->axes, &p->ws_need));
+    return new_plan(plan, p);
+  }
+  if (kind != GOFFT_ND_R2C || rank != 2)
+    return fail(GOFFT_ERR_UNSUPPORTED, "R2C plans are two-dimensional (RFFT2/IRFFT2)");
+  p->kind = K_ND_R2C;
+  p->rows = shape[0];
+  p->cols = shape[1];
+  p->rcols = p->cols / 2 + 1;
+  p->n = p->rows * p->cols;
+  p->nd_row = std::make_unique<RealEngine>();
+  p->nd_row->prec = dtype;
+  p->nd_row->n = p->cols;
+  p->nd_row->batch = p->rows * batch;
+  RC_TRY(p->nd_row->init(p->flags));
+  // Column transform over the rows x rcols complex half-spectrum (the only
+  // complex axis of RFFT2, rfft2.go:55-71).
+  NDAxis col;
+  size_t need = p->nd_row->ws_bytes();
+  RC_TRY(build_axis(dtype, p->flags, p->rows, p->rcols, batch, col, &need));
+  p->axes.push_back(std::move(col));
+  // IRFFT2 works on a padded copy of the spectrum.
+  p->ws_need = need + (size_t)p->rows * p->rcols * batch * elt(dtype);
+  return new_plan(plan, p);
+}
+
+int gofft_plan_spectrogram(gofft_plan** plan, int64_t length, int64_t segment, int64_t overlap,
+                           const double* window, int64_t window_len, double d, int dtype,
+                           unsigned flags) {
+  if (!plan) return fail(GOFFT_ERR_INVALID_ARG, "null plan pointer");
+  *plan = nullptr;
+  RC_TRY(check_dtype(dtype));
+  // Argument checks in the reference's order and wording (freq.go:108-116).
+  if (segment <= 0) return fail(GOFFT_ERR_INVALID_ARG, "fft: segment length must be positive");
+  if (window_len != segment || !window)
+    return fail(GOFFT_ERR_INVALID_ARG, "fft: window length must equal segment length");
+  if (overlap < 0 || overlap >= segment)
+    return fail(GOFFT_ERR_INVALID_ARG, "fft: overlap must satisfy 0 <= overlap < segment");
+  if (length < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative signal length");
+  RC_TRY(gofft_init());
+  auto p = std::make_unique<gofft_plan>();
+  p->kind = K_SPEC;
+  p->prec = dtype;
+  p->flags = default_flags(dtype, flags);
+  p->seg = segment;
+  p->step = segment - overlap;
+  p->sig_len = length;
+  p->d = d;
+  p->frames = 0;
+  for (i64 start = 0; start + segment <= length; start += p->step) p->frames++;
+  p->n = segment;
+  p->batch = p->frames;
+  std::vector<long double> w(window, window + segment);
+  RC_TRY(upload_real(p->window, w, dtype));
+  p->real = std::make_unique<RealEngine>();
+  p->real->prec = dtype;
+  p->real->n = segment;
+  p->real->batch = p->frames;
+  if (p->frames > 0) RC_TRY(p->real->init(p->flags));
+  // The fused kernel reads frames in place; otherwise frames are gathered.
+  p->spec_gather = !p->real->single;
+  p->ws_need = p->frames > 0 ? p->real->ws_bytes() : 0;
+  if (p->spec_gather && p->frames > 0) p->ws_need += (size_t)p->frames * segment * relt(dtype);
+  return new_plan(plan, p);
+}
+
+int gofft_plan_destroy(gofft_plan* plan) {
+  delete plan;
+  return GOFFT_OK;
+}
+
+int64_t gofft_plan_len(const gofft_plan* p) { return p ? p->n : -1; }
+int64_t gofft_plan_frames(const gofft_plan* p) { return p ? p->frames : -1; }
+int64_t gofft_plan_workspace_bytes(const gofft_plan* p) { return p ? (int64_t)p->ws_need : -1; }
+
+int64_t gofft_plan_in_elems(const gofft_plan* p, int direction) {
+  if (!p) return -1;
+  switch (p->kind) {
+    case K_C2C: return p->n * p->batch;
+    case K_MANY: return p->total_elems;
+    case K_R2C: return direction == 0 ? p->n * p->batch : (p->n / 2 + 1) * p->batch;
+    case K_PSD: return p->n * p->batch;
+    case K_ND_C2C: return p->nd_total * p->batch;
+    case K_ND_R2C: return direction == 0 ? p->rows * p->cols * p->batch : p->rows * p->rcols * p->batch;
+    case K_SPEC: return p->sig_len;
+  }
+  return -1;
+}
+
+int64_t gofft_plan_out_elems(const gofft_plan* p, int direction) {
+  if (!p) return -1;
+  switch (p->kind) {
+    case K_C2C: return p->n * p->batch;
+    case K_MANY: return p->total_elems;
+    case K_R2C: return direction == 0 ? (p->n / 2 + 1) * p->batch : p->n * p->batch;
+    case K_PSD: return (p->n / 2 + 1) * p->batch;
+    case K_ND_C2C: return p->nd_total * p->batch;
+    case K_ND_R2C: return direction == 0 ? p->rows * p->rcols * p->batch : p->rows * p->cols * p->batch;
+    case K_SPEC: return p->frames * (p->seg / 2 + 1);
+  }
+  return -1;
+}
+
+int gofft_plan_num_launches(const gofft_plan* p, int direction) {
+  if (!p) return -1;
+  (void)direction;
+  switch (p->kind) {
+    case K_C2C: return p->eng->launches();
+    case K_MANY: {
+      int k = 0;
+      for (auto& g : p->groups) k += g->eng->launches();
+      return k;
+    }
+    case K_R2C:
+    case K_PSD: return p->n > 0 ? p->real->launches() : 0;
+    case K_ND_C2C: {
+      int k = 0;
+      for (auto& a : p->axes) k += a.eng ? a.eng->launches() + (a.gather ? 2 : 0) : 0;
+      return k;
+    }
+    case K_ND_R2C: {
+      int k = p->nd_row->launches();
+      for (auto& a : p->axes) k += a.eng ? a.eng->launches() + (a.gather ? 2 : 0) : 0;
+      return k;
+    }
+    case K_SPEC: return p->frames > 0 ? p->real->launches() + (p->spec_gather ? 1 : 0) : 0;
+  }
+  return -1;
+}
+
+int gofft_exec_c2c(const gofft_plan* p, const void* src, void* dst, int direction, void* stream) {
+  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  if (direction != GOFFT_FORWARD && direction != GOFFT_INVERSE)
+    return fail(GOFFT_ERR_INVALID_ARG, "direction must be GOFFT_FORWARD or GOFFT_INVERSE");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = nullptr;
+  RC_TRY(p->ws(s, &ws));
+  const int inv = direction == GOFFT_INVERSE;
+  if (p->kind == K_C2C) {
+    if (p->n == 0 || p->batch == 0) return GOFFT_OK;
+    Ctx c{s, ws};
+    return p->eng->run(src, LineIO{nullptr, p->n}, dst, LineIO{nullptr, p->n}, inv, inv,
+                       inv ? 1.0 / (double)p->n : 1.0, c);
+  }
+  if (p->kind == K_MANY) {
+    for (const auto& g : p->groups) {
+      Ctx c{s, ws};
+      const i64* o = (const i64*)g->off.p;
+      RC_TRY(g->eng->run(src, LineIO{o, 0}, dst, LineIO{o, 0}, inv, inv, inv ? 1.0 / (double)g->len : 1.0, c));
+    }
+    return GOFFT_OK;
+  }
+  if (p->kind == K_ND_C2C) {
+    if (p->nd_total * p->batch == 0) return GOFFT_OK;
+    return run_nd_axes(p->axes, p->prec, p->nd_total * p->batch, src, dst, inv,
+                       inv ? 1.0 / (double)p->nd_total : 1.0, s, ws);
+  }
+  return fail(GOFFT_ERR_INVALID_ARG, "plan is not a complex-to-complex plan");
+}
+
+int gofft_exec_r2c(const gofft_plan* p, const void* src, void* dst, void* stream) {
+  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = nullptr;
+  RC_TRY(p->ws(s, &ws));
+  if (p->n == 0) return GOFFT_OK;
+  return p->real->forward(src, p->n, dst, p->n / 2 + 1, 0, 1.0, nullptr, Ctx{s, ws});
+}
+
+int gofft_exec_c2r(const gofft_plan* p, const void* src, int64_t bins, void* dst, void* stream) {
+  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  if (bins < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative bin count");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = nullptr;
+  RC_TRY(p->ws(s, &ws));
+  if (p->n <= 0) return GOFFT_OK;
+  i64 keep = std::min<i64>(bins, p->n / 2 + 1);
+  return p->real->inverse(src, bins, keep, dst, p->n, Ctx{s, ws});
+}
+
+int gofft_exec_psd(const gofft_plan* p, const void* src, void* dst, void* stream) {
+  if (!p || p->kind != K_PSD) return fail(GOFFT_ERR_INVALID_ARG, "plan is not a psd plan");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = nullptr;
+  RC_TRY(p->ws(s, &ws));
+  if (p->n == 0) return GOFFT_OK;
+  return p->real->forward(src, p->n, dst, p->n / 2 + 1, 1, p->d / (double)p->n, nullptr, Ctx{s, ws});
+}
+
+int gofft_exec_spectrogram(const gofft_plan* p, const void* src, void* dst, void* stream) {
+  if (!p || p->kind != K_SPEC) return fail(GOFFT_ERR_INVALID_ARG, "plan is not a spectrogram plan");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = nullptr;
+  RC_TRY(p->ws(s, &ws));
+  if (p->frames == 0) return GOFFT_OK;
+  const i64 bins = p->seg / 2 + 1;
+  const double sc = p->d / (double)p->seg;
+  if (!p->spec_gather)
+    return p->real->forward(src, p->step, dst, bins, 1, sc, p->window.p, Ctx{s, ws});
+  char* F = ws;
+  char* rest = ws + (size_t)p->frames * p->seg * relt(p->prec);
+  HIP_TRY(launch_frames(p->prec, src, p->window.p, F, p->seg, p->step, p->frames, s));
+  return p->real->forward(F, p->seg, dst, bins, 1, sc, nullptr, Ctx{s, rest});
+}
+
+int gofft_exec_nd_r2c(const gofft_plan* p, const void* src, int64_t bins, void* dst, int direction,
+                      void* stream) {
+  if (!p || p->kind != K_ND_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an nd r2c plan");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = nullptr;
+  RC_TRY(p->ws(s, &ws));
+  const i64 B = p->batch;
+  if (B == 0) return GOFFT_OK;
+  const i64 half = p->rows * p->rcols;
+  const size_t pad_bytes = (size_t)half * B * elt(p->prec);
+  const i64 htotal = half * B;
+  if (direction == GOFFT_FORWARD) {
+    // Rows: real transform of each row into dst (rows x rcols), then the
+    // column FFT in place (rfft2.go:22-74).
+    RC_TRY(p->nd_row->forward(src, p->cols, dst, p->rcols, 0, 1.0, nullptr, Ctx{s, ws}));
+    return run_nd_axes(p->axes, p->prec, htotal, dst, dst, 0, 1.0, s, ws);
+  }
+  // IRFFT2 (rfft2.go:87-138): copy zero-padded to rows x rcols per array,
+  // column inverse, row IRFFT. `bins` complex values are supplied per array.
+  if (bins < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative bin count");
+  char* H = ws;
+  char* rest = ws + pad_bytes;
+  i64 keep = std::min<i64>(bins, half);
+  if (keep < half) HIP_TRY(launch_fill(p->prec, H, 2 * htotal, 0.0, s));
+  if (keep > 0) HIP_TRY(launch_copy_lines(p->prec, src, H, keep, B, bins, half, s));
+  RC_TRY(run_nd_axes(p->axes, p->prec, htotal, H, H, 1, 1.0 / (double)p->rows, s, rest));
+  return p->nd_row->inverse(H, p->rcols, p->rcols, dst, p->cols, Ctx{s, rest});
+}
+
+int gofft_vjp_c2c(const gofft_plan* p, const void* gy, void* gx, int direction, void* stream) {
+  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  // y = F x  =>  gx = F^H gy = conj(F conj(gy)) = n * IFFT(gy).
+  // y = IFFT x = F^H x / n  =>  gx = F gy / n.
+  if (p->kind == K_C2C || p->kind == K_MANY || p->kind == K_ND_C2C) {
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = nullptr;
+    RC_TRY(p->ws(s, &ws));
+    if (p->kind == K_C2C) {
+      if (p->n == 0 || p->batch == 0) return GOFFT_OK;
+      Ctx c{s, ws};
+      int adj = direction == GOFFT_FORWARD;
+      return p->eng->run(gy, LineIO{nullptr, p->n}, gx, LineIO{nullptr, p->n}, adj, adj,
+                         adj ? 1.0 : 1.0 / (double)p->n, c);
+    }
+    if (p->kind == K_MANY) {
+      int adj = direction == GOFFT_FORWARD;
+      for (const auto& g : p->groups) {
+        Ctx c{s, ws};
+        const i64* o = (const i64*)g->off.p;
+        RC_TRY(g->eng->run(gy, LineIO{o, 0}, gx, LineIO{o, 0}, adj, adj, adj ? 1.0 : 1.0 / (double)g->len, c));
+      }
+      return GOFFT_OK;
+    }
+    int adj = direction == GOFFT_FORWARD;
+    return run_nd_axes(p->axes, p->prec, p->nd_total * p->batch, gy, gx, adj,
+                       adj ? 1.0 : 1.0 / (double)p->nd_total, s, ws);
+  }
+  return fail(GOFFT_ERR_INVALID_ARG, "plan is not a complex-to-complex plan");
+}
+
+int gofft_vjp_r2c(const gofft_plan* p, const void* gy, void* gx, void* stream) {
+  (void)p; (void)gy; (void)gx; (void)stream;
+  return fail(GOFFT_ERR_UNSUPPORTED, "gofft_vjp_r2c: not built yet");
+}
+
+int gofft_vjp_c2r(const gofft_plan* p, const void* gx, void* gX, void* stream) {
+  (void)p; (void)gx; (void)gX; (void)stream;
+  return fail(GOFFT_ERR_UNSUPPORTED, "gofft_vjp_c2r: not built yet");
+}
+
+int gofft_vjp_nd_r2c(const gofft_plan* p, const void* g_out, void* g_in, int direction, void* stream) {
+  (void)p; (void)g_out; (void)g_in; (void)direction; (void)stream;
+  return fail(GOFFT_ERR_UNSUPPORTED, "gofft_vjp_nd_r2c: not built yet");
+}
+
+// ---------------------------------------------------------------------------
+// Host convenience calls: stage through device memory.
+
+static int host_roundtrip(size_t in_bytes, size_t out_bytes, const void* src, void* dst,
+                          const std::function<int(const void*, void*)>& fn) {
+  DevBuf din, dout;
+  RC_TRY(din.alloc(in_bytes ? in_bytes : 16));
+  RC_TRY(dout.alloc(out_bytes ? out_bytes : 16));
+  if (in_bytes) HIP_TRY(hipMemcpy(din.p, src, in_bytes, hipMemcpyHostToDevice));
+  RC_TRY(fn(din.p, dout.p));
+  HIP_TRY(hipDeviceSynchronize());
+  if (out_bytes) HIP_TRY(hipMemcpy(dst, dout.p, out_bytes, hipMemcpyDeviceToHost));
+  return GOFFT_OK;
+}
+
+int gofft_exec_c2c_host(const gofft_plan* p, const void* src, void* dst, int direction) {
+  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  size_t b = (size_t)gofft_plan_in_elems(p, direction) * elt(p->prec);
+  return host_roundtrip(b, b, src, dst, [&](const void* i, void* o) {
+    return gofft_exec_c2c(p, i, o, direction, nullptr);
+  });
+}
+
+int gofft_exec_r2c_host(const gofft_plan* p, const void* src, void* dst) {
+  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  size_t ib = (size_t)p->n * p->batch * relt(p->prec);
+  size_t ob = (size_t)(p->n / 2 + 1) * p->batch * elt(p->prec);
+  if (p->n == 0) ob = 0;
+  return host_roundtrip(ib, ob, src, dst, [&](const void* i, void* o) { return gofft_exec_r2c(p, i, o, nullptr); });
+}
+
+int gofft_exec_c2r_host(const gofft_plan* p, const void* src, int64_t bins, void* dst) {
+  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  size_t ib = (size_t)bins * p->batch * elt(p->prec);
+  size_t ob = (size_t)p->n * p->batch * relt(p->prec);
+  return host_roundtrip(ib, ob, src, dst,
+                        [&](const void* i, void* o) { return gofft_exec_c2r(p, i, bins, o, nullptr); });
+}
+
+int gofft_sync(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return GOFFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,174 @@
+/* gofft.h — C ABI of libgofft_hip.so, the gfx950 (MI355X) backend for the
+ * go-fft transform path.
+ *
+ * The reference (go-fft, pure Go) has no FFI; its drop-in boundary is its public
+ * Go API. Each entry point below names the reference function(s) it replaces
+ * (file:line in the reference tree). A Go package binds these over cgo; the
+ * Python test harness binds them over ctypes.
+ *
+ * Conventions
+ *  - Complex data is interleaved (re, im): complex128 = 2 x double,
+ *    complex64 = 2 x float. Real data is double or float.
+ *  - `gofft_exec_*` take DEVICE pointers and a hipStream_t (as void*, may be
+ *    NULL for the default stream); they enqueue work and return without
+ *    synchronising. `gofft_*_host` take HOST pointers, stage through device
+ *    memory and return after the result is in host memory.
+ *  - Forward transforms are unnormalised; inverses divide by n (by the product
+ *    of the axis lengths for N-D), as the reference does (doc.go:21-26).
+ *  - dst may alias src for every transform (plan.go:88-90); src is never
+ *    modified unless it aliases dst.
+ *  - Every call returns GOFFT_OK (0) or a negative status; gofft_last_error()
+ *    returns a thread-local message for the last failure on this thread.
+ *  - Plans are immutable after creation and safe to execute concurrently from
+ *    several threads provided each concurrent call uses its own stream; the
+ *    plan keeps one workspace per stream (plan.go:12-14, realplan.go:10-11).
+ */
+#ifndef GOFFT_H
+#define GOFFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define GOFFT_OK 0
+#define GOFFT_ERR_INVALID_ARG (-1)
+#define GOFFT_ERR_SHAPE (-2)
+#define GOFFT_ERR_OOM (-3)
+#define GOFFT_ERR_HIP (-4)
+#define GOFFT_ERR_UNSUPPORTED (-5)
+
+/* element types */
+#define GOFFT_C128 0 /* complex128 / float64: the plain drop-in */
+#define GOFFT_C64 1  /* complex64 / float32: the reduced-precision batched form */
+
+/* directions */
+#define GOFFT_FORWARD 0
+#define GOFFT_INVERSE 1
+
+/* plan flags */
+#define GOFFT_FLAG_NONE 0u
+/* Bluestein lengths use the reference's recurrence twiddles for the
+ * power-of-two convolution (internal/kernels/kernels.go:38-59) so the result
+ * reproduces the reference's rounding; set by default for GOFFT_C128. */
+#define GOFFT_FLAG_REFERENCE_TWIDDLES 1u
+/* Force accurate (directly evaluated) twiddles everywhere. */
+#define GOFFT_FLAG_ACCURATE_TWIDDLES 2u
+
+/* N-D kinds */
+#define GOFFT_ND_C2C 0
+#define GOFFT_ND_R2C 1 /* RFFT2/IRFFT2 semantics on the last axis */
+
+typedef struct gofft_plan gofft_plan;
+
+/* Library set-up and diagnostics. gofft_init selects the current HIP device
+ * and reads its limits; it is called implicitly by every plan constructor. */
+int gofft_init(void);
+const char* gofft_last_error(void);
+const char* gofft_version(void);
+/* Engine a length routes to (plan.go:42-73): 0 trivial, 1 power of two,
+ * 2 mixed-radix, 3 Rader, 4 Bluestein. */
+int gofft_route(int64_t n);
+
+/* --- plan construction ------------------------------------------------- */
+
+/* NewPlan (plan.go:42-73) for `batch` independent contiguous lines of length n.
+ * Replaces Plan.FFT / Plan.IFFT (plan.go:91-109) and FFT/IFFT (fft.go:14-40). */
+int gofft_plan_c2c(gofft_plan** plan, int64_t n, int64_t batch, int dtype, unsigned flags);
+
+/* Many transforms of uneven lengths in one plan: line i has length lengths[i]
+ * and starts at element offset sum(lengths[0..i-1]) of both src and dst. No
+ * reference counterpart (the reference loops over Plan.FFT per length). */
+int gofft_plan_c2c_many(gofft_plan** plan, const int64_t* lengths, int64_t count, int dtype,
+                        unsigned flags);
+
+/* NewRealPlan (realplan.go:37-58) for `batch` lines of n real samples; line i
+ * of the spectrum holds n/2+1 bins. Replaces RealPlan.RFFT / RealPlan.IRFFT
+ * (realplan.go:77-192) and RFFT/IRFFT (rfft.go:19-45). */
+int gofft_plan_r2c(gofft_plan** plan, int64_t n, int64_t batch, int dtype, unsigned flags);
+
+/* FFTN/IFFTN/FFT2/IFFT2 (ndfft.go:22-45) for kind GOFFT_ND_C2C, and
+ * RFFT2/IRFFT2 (rfft2.go:22-138) for kind GOFFT_ND_R2C (rank 2 only), over
+ * `batch` contiguous arrays of the given row-major shape. rank 0 is the
+ * identity. A non-positive axis returns GOFFT_ERR_SHAPE with the reference's
+ * panic text "fft: shape lengths must be positive" (ndfft.go:54). */
+int gofft_plan_nd(gofft_plan** plan, const int64_t* shape, int rank, int kind, int64_t batch,
+                  int dtype, unsigned flags);
+
+/* PSD (freq.go:69-90) of `batch` real lines of length n with sample spacing d;
+ * each output line holds n/2+1 densities. */
+int gofft_plan_psd(gofft_plan** plan, int64_t n, int64_t batch, double d, int dtype,
+                   unsigned flags);
+
+/* Spectrogram (freq.go:107-127) of a real signal of `length` samples: frames of
+ * `segment` samples every `segment - overlap`, multiplied by `window` (host
+ * array of `segment` doubles, e.g. Hann, window.go:20-22), one-sided PSD each.
+ * Errors carry the reference's panic texts (freq.go:109,112,115). */
+int gofft_plan_spectrogram(gofft_plan** plan, int64_t length, int64_t segment, int64_t overlap,
+                           const double* window, int64_t window_len, double d, int dtype,
+                           unsigned flags);
+
+int gofft_plan_destroy(gofft_plan* plan);
+
+/* Plan queries: transform length, frames (spectrogram), element counts of one
+ * input and one output array, and the workspace the plan holds per stream. */
+int64_t gofft_plan_len(const gofft_plan* plan);
+int64_t gofft_plan_frames(const gofft_plan* plan);
+int64_t gofft_plan_in_elems(const gofft_plan* plan, int direction);
+int64_t gofft_plan_out_elems(const gofft_plan* plan, int direction);
+int64_t gofft_plan_workspace_bytes(const gofft_plan* plan);
+/* Number of kernel launches one forward execution enqueues. */
+int gofft_plan_num_launches(const gofft_plan* plan, int direction);
+
+/* --- execution on device memory -------------------------------------------- */
+
+/* c2c / c2c_many / nd-C2C plans. */
+int gofft_exec_c2c(const gofft_plan* plan, const void* src, void* dst, int direction,
+                   void* stream);
+/* r2c plans: forward real -> n/2+1 bins per line. */
+int gofft_exec_r2c(const gofft_plan* plan, const void* src, void* dst, void* stream);
+/* r2c plans: inverse; each input line holds `bins` bins (missing bins read as
+ * zero, the imaginary parts of DC and Nyquist are ignored; realplan.go:221-250). */
+int gofft_exec_c2r(const gofft_plan* plan, const void* src, int64_t bins, void* dst, void* stream);
+/* nd-R2C plans: RFFT2 (direction 0) / IRFFT2 (direction 1); for the inverse
+ * `bins` is the number of complex bins supplied per array (rfft2.go:99-106). */
+int gofft_exec_nd_r2c(const gofft_plan* plan, const void* src, int64_t bins, void* dst,
+                      int direction, void* stream);
+/* psd and spectrogram plans: real in, real densities out. */
+int gofft_exec_psd(const gofft_plan* plan, const void* src, void* dst, void* stream);
+int gofft_exec_spectrogram(const gofft_plan* plan, const void* src, void* dst, void* stream);
+
+/* --- vector-Jacobian products (no reference counterpart) ------------------ */
+
+/* For y = FFT(x) (complex, unnormalised) and an upstream gradient gy = dL/dconj(y)
+ * in the convention where dL = Re<gy, dy>, the gradient w.r.t. x is
+ * gx = n * IFFT(gy) = conj(FFT(conj(gy))). direction selects which transform's
+ * VJP: GOFFT_FORWARD for y = FFT(x), GOFFT_INVERSE for y = IFFT(x). */
+int gofft_vjp_c2c(const gofft_plan* plan, const void* gy, void* gx, int direction, void* stream);
+/* For y = RFFT(x) (n/2+1 bins per line, real x): gx = real part of the adjoint,
+ * i.e. n * IRFFT(w * gy) with w = 1 at DC and Nyquist and 1/2 elsewhere. */
+int gofft_vjp_r2c(const gofft_plan* plan, const void* gy, void* gx, void* stream);
+/* For x = IRFFT(X) (real output): gX = RFFT(gx) / n with interior bins doubled. */
+int gofft_vjp_c2r(const gofft_plan* plan, const void* gx, void* gX, void* stream);
+/* nd-R2C plans: VJP of RFFT2 (direction 0) and IRFFT2 (direction 1). */
+int gofft_vjp_nd_r2c(const gofft_plan* plan, const void* g_out, void* g_in, int direction,
+                     void* stream);
+
+/* --- host convenience (reference slice API) -------------------------------- */
+
+/* Plan.FFT / Plan.IFFT on host slices (plan.go:91-109): stages through device
+ * memory, runs, copies back. n*batch complex elements in and out. */
+int gofft_exec_c2c_host(const gofft_plan* plan, const void* src, void* dst, int direction);
+int gofft_exec_r2c_host(const gofft_plan* plan, const void* src, void* dst);
+int gofft_exec_c2r_host(const gofft_plan* plan, const void* src, int64_t bins, void* dst);
+
+/* Synchronise the device (for callers without their own stream handling). */
+int gofft_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOFFT_H */
