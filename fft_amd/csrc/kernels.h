@@ -42,6 +42,8 @@ struct Stage {
   int ns;
   int nb;      // R / radix butterflies per column
   int tw_off;  // offset of this stage's twiddles in the stage table
+  int kmul;    // twiddles of butterfly k start at tw_off + k*kmul + cdig*cmul
+  int cmul;    // (cdig: the column's twiddle digit; normally kmul = radix-1, cmul = 0)
   FastDiv nsd;
 };
 
@@ -133,7 +135,7 @@ struct ConvArgs {
   long long in_stride, out_stride;
   int n;                    // transform length
   int kind;                 // 0 Rader cyclic, 1 Rader linear (read conv[k+q]), 2 Bluestein
-  int conj_io;              // inverse via conj(F(conj(x)))
+  int conj_in, conj_out;    // conjugate the input / the output (inverse = both)
   double scale;             // post scale (1/n for IFFT)
   SubFFT f;
 };
@@ -158,7 +160,7 @@ hipError_t launch_rader_gather(int prec, const void* in, void* out, const int* p
 // Rader scatter: out[l*os + iperm[k]] = x0 + conv[l*cs + k + off], out[l*os] = sum(x).
 hipError_t launch_rader_scatter(int prec, const void* x, const void* conv, void* out, const int* iperm,
                                 long long n, long long off, long long nlines, long long xs,
-                                long long cs, long long os, int conj_out, double scale,
+                                long long cs, long long os, int conj_x, int conj_out, double scale,
                                 const long long* xoff, const long long* ooff, hipStream_t s);
 // Bluestein chirp in: out[l*os + j] = j < n ? in[l*is + j] * w[j] : 0, j < m.
 hipError_t launch_chirp_in(int prec, const void* in, void* out, const void* w, long long n,

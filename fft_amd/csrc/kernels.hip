@@ -198,7 +198,7 @@ template <int RAD, typename T2> __device__ __forceinline__ void dft(T2* v) {
 
 template <typename T2, int RAD, int MAXP>
 __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage& s, const T2* stw,
-                                          int c, int lt) {
+                                          int c, int lt, int cdig) {
   constexpr int NB = MAXP / RAD > 0 ? MAXP / RAD : 1;
   T2* col = lds + c * f.R;
   const int nb = s.nb, ns = s.ns, tpc = f.tpc;
@@ -209,10 +209,10 @@ __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage&
     if (j < nb) {
 #pragma unroll
       for (int i = 0; i < RAD; ++i) v[m][i] = col[j + i * nb];
-      if (ns > 1) {
+      if (ns > 1 || s.cmul) {
         int q = (int)fastdiv_q((uint32_t)j, s.nsd);
         int k = j - q * ns;
-        const T2* tw = stw + s.tw_off + k * (RAD - 1);
+        const T2* tw = stw + s.tw_off + k * s.kmul + cdig * s.cmul;
 #pragma unroll
         for (int i = 1; i < RAD; ++i) v[m][i] = cmul(v[m][i], tw[i - 1]);
       }
@@ -234,24 +234,25 @@ __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage&
   __syncthreads();
 }
 
-// Runs the whole R-point sub-FFT (forward) on every column in LDS.
+// Runs the whole R-point sub-FFT (forward) on every column in LDS. cdig is the
+// calling thread's column twiddle digit (only read by stages with cmul != 0).
 template <typename T2, int MAXP>
-__device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw) {
+__device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw, int cdig = 0) {
   const int tid = threadIdx.x;
   const int c = tid >> f.log2tpc;
   const int lt = tid & (f.tpc - 1);
   for (int si = 0; si < f.nstages; ++si) {
     const Stage& s = f.st[si];
     switch (s.radix) {
-      case 2: lds_stage<T2, 2, MAXP>(lds, f, s, stw, c, lt); break;
-      case 3: lds_stage<T2, 3, MAXP>(lds, f, s, stw, c, lt); break;
-      case 4: lds_stage<T2, 4, MAXP>(lds, f, s, stw, c, lt); break;
-      case 5: lds_stage<T2, 5, MAXP>(lds, f, s, stw, c, lt); break;
-      case 7: lds_stage<T2, 7, MAXP>(lds, f, s, stw, c, lt); break;
-      case 8: lds_stage<T2, 8, MAXP>(lds, f, s, stw, c, lt); break;
-      case 11: lds_stage<T2, 11, MAXP>(lds, f, s, stw, c, lt); break;
-      case 13: lds_stage<T2, 13, MAXP>(lds, f, s, stw, c, lt); break;
-      default: lds_stage<T2, 16, MAXP>(lds, f, s, stw, c, lt); break;
+      case 2: lds_stage<T2, 2, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 3: lds_stage<T2, 3, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 4: lds_stage<T2, 4, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 5: lds_stage<T2, 5, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 7: lds_stage<T2, 7, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 8: lds_stage<T2, 8, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 11: lds_stage<T2, 11, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 13: lds_stage<T2, 13, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      default: lds_stage<T2, 16, MAXP>(lds, f, s, stw, c, lt, cdig); break;
     }
   }
 }
@@ -337,7 +338,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
     lds[c * R + r] = v;
   }
   __syncthreads();
-  lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
+  lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw), (int)s_tk[tid >> f.log2tpc]);
   T2* out = reinterpret_cast<T2*>(a.out);
   const T sc = (T)a.scale;
   const bool do_scale = a.scale != 1.0;
@@ -521,12 +522,12 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
       if (a.kind == 2) {
         if (r < n) {
           v = x[r];
-          if (a.conj_io) v.y = -v.y;
+          if (a.conj_in) v.y = -v.y;
           v = cmul(v, chirp[r]);
         }
       } else if (r < q) {
         v = x[a.perm[r]];
-        if (a.conj_io) v.y = -v.y;
+        if (a.conj_in) v.y = -v.y;
       }
     }
     lds[c * R + r] = v;
@@ -562,22 +563,22 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   if (a.kind == 2) {
     for (int k = lt; k < n && live; k += f.tpc) {
       T2 v = cmul(cconj(lds[c * R + k]), chirp[k]);
-      if (a.conj_io) v.y = -v.y;
+      if (a.conj_out) v.y = -v.y;
       out[k] = cscale(v, sc);
     }
   } else {
     const T2* x = in + (live ? (a.in_off ? a.in_off[l] : l * a.in_stride) : 0);
     T2 x0 = live ? x[0] : mkc<T2>(0, 0);
-    if (a.conj_io) x0.y = -x0.y;
+    if (a.conj_in) x0.y = -x0.y;
     const int off = a.kind == 1 ? q : 0;
     for (int k = lt; k < q && live; k += f.tpc) {
       T2 v = cadd(x0, cconj(lds[c * R + k + off]));
-      if (a.conj_io) v.y = -v.y;
+      if (a.conj_out) v.y = -v.y;
       out[a.iperm[k]] = cscale(v, sc);
     }
     if (lt == 0 && live) {
       T2 v = cadd(x0, colsum);
-      if (a.conj_io) v.y = -v.y;
+      if (a.conj_out) v.y = -v.y;
       out[0] = cscale(v, sc);
     }
   }
@@ -636,7 +637,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_rader_scatter(const void* x_, const void* conv_, void* out_,
                                                        const int* iperm, long long n, long long off,
                                                        long long xs, long long cs, long long os,
-                                                       int conj_out, double scale,
+                                                       int conj_x, int conj_out, double scale,
                                                        const long long* xoff, const long long* ooff) {
   typedef typename Vec2<T>::type T2;
   __shared__ T2 red[256];
@@ -654,10 +655,10 @@ __global__ void __launch_bounds__(256) k_rader_scatter(const void* x_, const voi
     if ((int)threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  // For an inverse (conj_out) the convolution ran on conj(x), so x0 and the DC
-  // sum enter conjugated and the whole output is conjugated back.
+  // With conj_x the convolution ran on conj(x), so x0 and the DC sum enter
+  // conjugated; conj_out conjugates every output.
   T2 sum = red[0];
-  if (conj_out) {
+  if (conj_x) {
     x0.y = -x0.y;
     sum.y = -sum.y;
   }
@@ -978,15 +979,15 @@ hipError_t launch_rader_gather(int prec, const void* in, void* out, const int* p
 
 hipError_t launch_rader_scatter(int prec, const void* x, const void* conv, void* out, const int* iperm,
                                 long long n, long long off, long long nlines, long long xs,
-                                long long cs, long long os, int conj_out, double scale,
+                                long long cs, long long os, int conj_x, int conj_out, double scale,
                                 const long long* xoff, const long long* ooff, hipStream_t s) {
   if (nlines <= 0) return hipSuccess;
   if (prec == 0)
     hipLaunchKernelGGL(k_rader_scatter<double>, dim3((unsigned)nlines), dim3(256), 0, s, x, conv, out,
-                       iperm, n, off, xs, cs, os, conj_out, scale, xoff, ooff);
+                       iperm, n, off, xs, cs, os, conj_x, conj_out, scale, xoff, ooff);
   else
     hipLaunchKernelGGL(k_rader_scatter<float>, dim3((unsigned)nlines), dim3(256), 0, s, x, conv, out,
-                       iperm, n, off, xs, cs, os, conj_out, scale, xoff, ooff);
+                       iperm, n, off, xs, cs, os, conj_x, conj_out, scale, xoff, ooff);
   return hipGetLastError();
 }
 

@@ -364,6 +364,8 @@ bool make_sub(int R, int C, SubPlan& sp) {
     s.ns = ns;
     s.nb = R / rad;
     s.tw_off = off;
+    s.kmul = rad - 1;
+    s.cmul = 0;
     s.nsd = fastdiv_make((uint32_t)ns);
     if (ns > 1) {
       for (int k = 0; k < ns; k++)
@@ -373,6 +375,165 @@ bool make_sub(int R, int C, SubPlan& sp) {
     ns *= rad;
   }
   return true;
+}
+
+// Radix-2-only schedule for one pass of the reference-twiddle transform:
+// stage s combines sub-transforms of 2^s points within a column; its twiddle
+// for local butterfly k of column digit c is the reference's layer table
+// w_L[c + k*Ns] with L = Ns * 2^(s+1), stored at offset L/2 - 1 of one table
+// holding every layer L = 2, 4, ..., n.
+bool make_sub_r2(int R, int C, i64 Ns, SubPlan& sp) {
+  if (R < 2 || (R & (R - 1)) || C < 1 || C > kMaxCols || (C & (C - 1))) return false;
+  const int q = ilog2(R);
+  if (q > kMaxStages) return false;
+  int T = 64;
+  while (T < C) T <<= 1;
+  while ((i64)T * (kMaxP / 2) < (i64)C * R / 2 && T < 1024) T <<= 1;
+  if ((i64)T * (kMaxP / 2) < (i64)C * R / 2) return false;
+  sp = SubPlan();
+  sp.threads = T;
+  SubFFT& f = sp.f;
+  f.R = R;
+  f.C = C;
+  f.log2C = ilog2(C);
+  f.tpc = T / C;
+  f.log2tpc = ilog2(f.tpc);
+  f.nstages = q;
+  f.Rd = fastdiv_make((uint32_t)R);
+  for (int st = 0; st < q; st++) {
+    Stage& g = f.st[st];
+    g.radix = 2;
+    g.ns = 1 << st;
+    g.nb = R / 2;
+    g.tw_off = (int)(Ns * ((i64)1 << st) - 1);
+    g.kmul = (int)Ns;
+    g.cmul = 1;
+    g.nsd = fastdiv_make((uint32_t)g.ns);
+  }
+  return true;
+}
+
+// Go's math.Sin / math.Cos (Go standard library math/sin.go, the Cephes
+// sin.c algorithm of S. L. Moshier) for |x| < 2^29. The reference's Bluestein
+// twiddle recurrence (internal/kernels/kernels.go:44-55) amplifies any
+// one-ulp difference in its starting pair, so reproducing the reference's
+// rounding needs Go's values, not the C library's.
+#pragma clang fp contract(off)
+void go_sincos(double x, double* s_out, double* c_out) {
+  static const double S[6] = {1.58962301576546568060e-10, -2.50507477628578072866e-8,
+                              2.75573136213857245213e-6,  -1.98412698295895385996e-4,
+                              8.33333333332211858878e-3,  -1.66666666666666307295e-1};
+  static const double Cc[6] = {-1.13585365213876817300e-11, 2.08757008419747316778e-9,
+                               -2.75573141792967388112e-7,  2.48015872888517045348e-5,
+                               -1.38888888888730564116e-3,  4.16666666666665929218e-2};
+  const double PI4A = 7.85398125648498535156e-1, PI4B = 3.77489470793079817668e-8,
+               PI4C = 2.69515142907905952645e-15;
+  const double fourOverPi = 1.27323954473516268615107010698011489627567716592365;
+  bool ss = false, cs = false;
+  if (x < 0) {
+    x = -x;
+    ss = true;
+  }
+  uint64_t j = (uint64_t)(x * fourOverPi);
+  double y = (double)j;
+  if (j & 1) {
+    j++;
+    y++;
+  }
+  j &= 7;
+  if (j > 3) {
+    j -= 4;
+    ss = !ss;
+    cs = !cs;
+  }
+  if (j > 1) cs = !cs;
+  double z = ((x - y * PI4A) - y * PI4B) - y * PI4C;
+  double zz = z * z;
+  double c = 1.0 - 0.5 * zz + zz * zz * ((((((Cc[0] * zz) + Cc[1]) * zz + Cc[2]) * zz + Cc[3]) * zz + Cc[4]) * zz + Cc[5]);
+  double sn = z + z * zz * ((((((S[0] * zz) + S[1]) * zz + S[2]) * zz + S[3]) * zz + S[4]) * zz + S[5]);
+  if (j == 1 || j == 2) std::swap(sn, c);
+  *s_out = ss ? -sn : sn;
+  *c_out = cs ? -c : c;
+}
+
+struct dcx {
+  double re, im;
+};
+static inline dcx dmul(dcx a, dcx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+
+// The per-layer twiddles of the reference's Radix2 (kernels.go:38-59):
+// w_L[k] = w_L[k-1] * wlen, wlen = (cos, sin)(-2*pi/L), for L = 2 .. n; the
+// table for L sits at offset L/2 - 1.
+std::vector<dcx> recurrence_table(i64 n) {
+  std::vector<dcx> t;
+  t.reserve(n);
+  const double kPi = 3.14159265358979323846264338327950288;
+  for (i64 L = 2; L <= n; L <<= 1) {
+    double ang = -1.0 * 2 * kPi / (double)L;
+    double sn, c;
+    go_sincos(ang, &sn, &c);
+    dcx wlen{c, sn}, w{1, 0};
+    for (i64 k = 0; k < L / 2; k++) {
+      t.push_back(w);
+      w = dmul(w, wlen);
+    }
+  }
+  return t;
+}
+
+// Reference Bluestein set-up (bluestein.go:28-63) in float64 with separately
+// rounded products: chirps from Go's sin/cos and kernel spectra by
+// BitReverse + Radix2 with recurrence twiddles.
+void reference_bluestein_tables(i64 n, i64 m, std::vector<dcx>& wF, std::vector<dcx>& wI,
+                                std::vector<dcx>& bF, std::vector<dcx>& bI) {
+  const double kPi = 3.14159265358979323846264338327950288;
+  wF.resize(n);
+  wI.resize(n);
+  for (i64 j = 0; j < n; j++) {
+    i64 jj = (i64)((__int128)j * j % (2 * n));
+    double ang = kPi * (double)jj / (double)n;
+    double s1, c1, s2, c2;
+    go_sincos(-ang, &s1, &c1);
+    go_sincos(ang, &s2, &c2);
+    wF[j] = {c1, s1};
+    wI[j] = {c2, s2};
+  }
+  std::vector<dcx> tab = recurrence_table(m);
+  auto spectrum = [&](const std::vector<dcx>& w, std::vector<dcx>& b) {
+    b.assign(m, dcx{0, 0});
+    b[0] = {w[0].re, -w[0].im};
+    for (i64 j = 1; j < n; j++) {
+      dcx c{w[j].re, -w[j].im};
+      b[j] = c;
+      b[m - j] = c;
+    }
+    for (i64 i = 1, j = 0; i < m; i++) {  // kernels.go:20-32
+      i64 bit = m >> 1;
+      for (; j & bit; bit >>= 1) j ^= bit;
+      j ^= bit;
+      if (i < j) std::swap(b[i], b[j]);
+    }
+    for (i64 L = 2; L <= m; L <<= 1) {  // kernels.go:38-59
+      const dcx* wl = tab.data() + (L / 2 - 1);
+      i64 half = L >> 1;
+      for (i64 st = 0; st < m; st += L)
+        for (i64 k = 0; k < half; k++) {
+          dcx u = b[st + k];
+          dcx v = dmul(b[st + k + half], wl[k]);
+          b[st + k] = {u.re + v.re, u.im + v.im};
+          b[st + k + half] = {u.re - v.re, u.im - v.im};
+        }
+    }
+  };
+  spectrum(wF, bF);
+  spectrum(wI, bI);
+}
+#pragma clang fp contract(on)
+
+int upload_dcx(DevBuf& b, const std::vector<dcx>& v, int prec, double scale = 1.0) {
+  std::vector<cld> t(v.size());
+  for (size_t i = 0; i < v.size(); i++) t[i] = cld(v[i].re * scale, v[i].im * scale);
+  return upload_complex(b, t, prec);
 }
 
 // Largest power-of-two column count <= cap with C*R points within the LDS
@@ -483,6 +644,59 @@ class StockhamEngine : public Engine {
   std::vector<Pass> passes;
   DevBuf stw, ptw;
   int P() const { return (int)passes.size(); }
+
+  // Power-of-two transform with the reference's recurrence twiddles
+  // (kernels.Radix2): radix-2 layers only, grouped into passes of up to 2^8
+  // layers; no pass twiddles (each layer reads its own table).
+  int init_recurrence() {
+    DevInfo dev;
+    RC_TRY(current_device(&dev));
+    if (n < 2 || (n & (n - 1)) || st != 1 || inner != 1)
+      return fail(GOFFT_ERR_UNSUPPORTED, "recurrence transform needs a contiguous power of two");
+    const i64 pts = dev.lds_bytes / (i64)elt(prec);
+    const int q = ilog2((int)std::min<i64>(n, 1 << 30));
+    std::vector<int> qs;
+    if (n <= std::min<i64>(pts, 4096)) {
+      qs.push_back(q);
+    } else {
+      int P = (q + 7) / 8;
+      for (int i = 0; i < P; i++) qs.push_back(q / P + (i < q % P ? 1 : 0));
+    }
+    i64 Ns = 1;
+    for (int qp : qs) {
+      Pass p;
+      p.R = (i64)1 << qp;
+      p.Ns = Ns;
+      int C = qs.size() == 1 ? 1 : 8;
+      if (qs.size() == 1) {
+        while (C * 2 <= kMaxCols && C * 2 <= batch && (i64)C * 2 * p.R <= std::min<i64>(pts, 4096)) C *= 2;
+      }
+      if (!make_sub_r2((int)p.R, C, Ns, p.sp))
+        return fail(GOFFT_ERR_UNSUPPORTED, "no radix-2 schedule for pass length " + std::to_string(p.R));
+      passes.push_back(p);
+      Ns *= p.R;
+    }
+    for (auto& p : passes) {
+      const i64 R = p.R, Nsp = p.Ns;
+      p.im.D0 = 1;
+      p.im.D1 = Nsp;
+      p.im.D2 = n / (R * Nsp);
+      p.im.a0 = 1;
+      p.im.a1 = 1;
+      p.im.a2 = Nsp;
+      p.im.a3 = n;
+      p.im.g = n / R;
+      p.im.rowmajor = p.im.g == 1;
+      p.om = p.im;
+      p.om.a2 = Nsp * R;
+      p.om.g = Nsp;
+      p.om.rowmajor = p.om.g == 1;
+      p.stw_off = 0;
+      p.tw_M = 0;
+      p.tw_digit = 1;
+    }
+    return upload_dcx(stw, recurrence_table(n), prec);
+  }
 
   int init(unsigned flags) {
     (void)flags;
@@ -710,8 +924,6 @@ class RaderEngine : public Engine {
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
           const Ctx& c) const override {
     if (batch == 0) return GOFFT_OK;
-    // Inverse transforms use conj(DFT(conj(x))): ci and co are both set.
-    if (ci != co) return fail(GOFFT_ERR_UNSUPPORTED, "Rader engine needs matching conjugations");
     if (fused) {
       ConvArgs a{};
       a.in = in;
@@ -727,7 +939,8 @@ class RaderEngine : public Engine {
       a.out_stride = lo.stride;
       a.n = (int)n;
       a.kind = cyclic ? 0 : 1;
-      a.conj_io = ci;
+      a.conj_in = ci;
+      a.conj_out = co;
       a.scale = scale;
       a.f = sp.f;
       HIP_TRY(launch_conv(prec, kMaxP, a, sp.threads, c.s));
@@ -743,7 +956,7 @@ class RaderEngine : public Engine {
     HIP_TRY(launch_cmul_table(prec, B, A, spec.p, cl, batch, cl, cl, 0, 1, 1.0, nullptr, nullptr, c.s));
     RC_TRY(inner->run(A, cont, B, cont, 0, 1, 1.0, ic));
     HIP_TRY(launch_rader_scatter(prec, in, B, out, (const int*)iperm.p, n, cyclic ? 0 : q, batch,
-                                 li.stride, cl, lo.stride, co, scale, li.off, lo.off, c.s));
+                                 li.stride, cl, lo.stride, ci, co, scale, li.off, lo.off, c.s));
     return GOFFT_OK;
   }
 };
@@ -753,7 +966,11 @@ class RaderEngine : public Engine {
 class BluesteinEngine : public Engine {
  public:
   i64 m = 0;
-  DevBuf chirp, spec;
+  // Reference-twiddle mode (plain complex128): the convolution transforms
+  // use the reference's recurrence twiddles and its forward/inverse kernel
+  // spectra, so results round the way the reference's do.
+  bool faithful = false;
+  DevBuf chirp, spec, chirpI, specI;
   bool fused = false;
   SubPlan sp;
   DevBuf stw;
@@ -762,25 +979,56 @@ class BluesteinEngine : public Engine {
   int init(unsigned flags) {
     m = 1;
     while (m < 2 * n - 1) m <<= 1;
-    std::vector<cld> w(n), b(m, cld(0, 0));
-    for (i64 j = 0; j < n; j++) {
-      i64 jj = (i64)((__int128)j * j % (2 * n));  // bit-exact chirp phase index
-      w[j] = root(jj, 2 * n);                      // exp(-pi i jj / n)
+    faithful = prec == 0 && (flags & GOFFT_FLAG_REFERENCE_TWIDDLES) && !(flags & GOFFT_FLAG_ACCURATE_TWIDDLES);
+    if (faithful) {
+      std::vector<dcx> wF, wI, bF, bI;
+      reference_bluestein_tables(n, m, wF, wI, bF, bI);
+      const double invm = 1.0 / (double)m;  // exact: m is a power of two
+      RC_TRY(upload_dcx(chirp, wF, prec));
+      RC_TRY(upload_dcx(chirpI, wI, prec));
+      RC_TRY(upload_dcx(spec, bF, prec, invm));
+      RC_TRY(upload_dcx(specI, bI, prec, invm));
+    } else {
+      std::vector<cld> w(n), b(m, cld(0, 0));
+      for (i64 j = 0; j < n; j++) {
+        i64 jj = (i64)((__int128)j * j % (2 * n));  // bit-exact chirp phase index
+        w[j] = root(jj, 2 * n);                      // exp(-pi i jj / n)
+      }
+      b[0] = std::conj(w[0]);
+      for (i64 j = 1; j < n; j++) {
+        b[j] = std::conj(w[j]);
+        b[m - j] = std::conj(w[j]);
+      }
+      host_fft(b);
+      for (auto& v : b) v /= (long double)m;
+      RC_TRY(upload_complex(chirp, w, prec));
+      RC_TRY(upload_complex(spec, b, prec));
     }
-    b[0] = std::conj(w[0]);
-    for (i64 j = 1; j < n; j++) {
-      b[j] = std::conj(w[j]);
-      b[m - j] = std::conj(w[j]);
-    }
-    host_fft(b);
-    for (auto& v : b) v /= (long double)m;
-    RC_TRY(upload_complex(chirp, w, prec));
-    RC_TRY(upload_complex(spec, b, prec));
     DevInfo dev;
     RC_TRY(current_device(&dev));
     const i64 pts = dev.lds_bytes / (i64)elt(prec);
+    const i64 target = std::min<i64>(pts, std::max<i64>(m, prec == 0 ? 4096 : 8192));
     int C = 0;
-    if (m <= pts && pick_cols((int)m, 16, std::min<i64>(pts, std::max<i64>(m, prec == 0 ? 4096 : 8192)), sp, &C)) {
+    if (faithful) {
+      if (m <= std::min<i64>(pts, 4096)) {
+        C = 1;
+        while (C * 2 <= 16 && (i64)C * 2 * m <= 4096) C *= 2;
+        while (C >= 1 && !make_sub_r2((int)m, C, 1, sp)) C /= 2;
+        if (C >= 1) {
+          fused = true;
+          RC_TRY(upload_dcx(stw, recurrence_table(m), prec));
+          return GOFFT_OK;
+        }
+      }
+      auto e = std::make_unique<StockhamEngine>();
+      e->prec = prec;
+      e->n = m;
+      e->batch = batch;
+      RC_TRY(e->init_recurrence());
+      inner = std::move(e);
+      return GOFFT_OK;
+    }
+    if (m <= pts && pick_cols((int)m, 16, target, sp, &C)) {
       fused = true;
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
@@ -800,14 +1048,20 @@ class BluesteinEngine : public Engine {
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
           const Ctx& c) const override {
     if (batch == 0) return GOFFT_OK;
-    if (ci != co) return fail(GOFFT_ERR_UNSUPPORTED, "Bluestein engine needs matching conjugations");
+    // An inverse (ci and co) in reference mode runs the reference's inverse
+    // (bluestein.go:67-95 with the inverse chirp and spectrum) instead of
+    // conjugating a forward transform.
+    const bool ref_inverse = faithful && ci && co;
+    const void* w = ref_inverse ? chirpI.p : chirp.p;
+    const void* b = ref_inverse ? specI.p : spec.p;
+    if (ref_inverse) ci = co = 0;
     if (fused) {
       ConvArgs a{};
       a.in = in;
       a.out = out;
       a.stw = stw.p;
-      a.spec = spec.p;
-      a.chirp = chirp.p;
+      a.spec = b;
+      a.chirp = w;
       a.in_off = li.off;
       a.out_off = lo.off;
       a.nlines = batch;
@@ -815,7 +1069,8 @@ class BluesteinEngine : public Engine {
       a.out_stride = lo.stride;
       a.n = (int)n;
       a.kind = 2;
-      a.conj_io = ci;
+      a.conj_in = ci;
+      a.conj_out = co;
       a.scale = scale;
       a.f = sp.f;
       HIP_TRY(launch_conv(prec, kMaxP, a, sp.threads, c.s));
@@ -825,11 +1080,11 @@ class BluesteinEngine : public Engine {
     char* B = c.ws + (size_t)m * batch * elt(prec);
     Ctx ic{c.s, B + (size_t)m * batch * elt(prec)};
     LineIO cont{nullptr, m};
-    HIP_TRY(launch_chirp_in(prec, in, A, chirp.p, n, m, batch, li.stride, m, ci, li.off, c.s));
+    HIP_TRY(launch_chirp_in(prec, in, A, w, n, m, batch, li.stride, m, ci, li.off, c.s));
     RC_TRY(inner->run(A, cont, B, cont, 0, 0, 1.0, ic));
-    HIP_TRY(launch_cmul_table(prec, B, A, spec.p, m, batch, m, m, 0, 1, 1.0, nullptr, nullptr, c.s));
+    HIP_TRY(launch_cmul_table(prec, B, A, b, m, batch, m, m, 0, 1, 1.0, nullptr, nullptr, c.s));
     RC_TRY(inner->run(A, cont, B, cont, 0, 1, 1.0, ic));
-    HIP_TRY(launch_cmul_table(prec, B, out, chirp.p, n, batch, m, lo.stride, 0, co, scale, nullptr, lo.off,
+    HIP_TRY(launch_cmul_table(prec, B, out, w, n, batch, m, lo.stride, 0, co, scale, nullptr, lo.off,
                               c.s));
     return GOFFT_OK;
   }
@@ -1080,8 +1335,10 @@ int check_dtype(int dtype) {
   return GOFFT_OK;
 }
 
+// The plain complex128 path reproduces the reference's Bluestein rounding
+// unless accurate twiddles are requested.
 unsigned default_flags(int dtype, unsigned flags) {
-  (void)dtype;
+  if (dtype == GOFFT_C128 && !(flags & GOFFT_FLAG_ACCURATE_TWIDDLES)) flags |= GOFFT_FLAG_REFERENCE_TWIDDLES;
   return flags;
 }
 

@@ -70,14 +70,28 @@ def test_c2c_fp32_vs_fp64_oracle(dev, oracle, n):
     assert rel_l2(goti.reshape(batch, n), oracle.fft_batch(x, inverse=True)) <= BOUND32
 
 
-def test_bluestein_20014_fp64_accuracy(dev, oracle):
-    # The Bluestein length of the mixed batch. Accurate twiddles reach the
-    # exact DFT to ~1e-15 (the reference's recurrence leaves ~1e-12).
+@pytest.mark.parametrize("n", [17, 641, 2000, 20014, 2 * 9973, 40000])
+def test_bluestein_reference_parity(dev, oracle, n):
+    # Plain complex128 Bluestein reproduces the reference's recurrence-twiddle
+    # rounding (~1e-12 away from the exact DFT at n = 20014), so it matches
+    # the oracle to the 1e-14*log2(n) bar in both directions.
+    D, torch = dev
+    x = cgauss(2 * n, seed=n).reshape(2, n)
+    for inv in (False, True):
+        got = _run(D, torch, n, 2, x.ravel(), torch.complex128, inverse=inv).reshape(2, n)
+        want = oracle.fft_batch(x, inverse=inv)
+        assert rel_l2(got, want) <= bound64(n), (n, inv, rel_l2(got, want))
+
+
+def test_bluestein_accurate_flag(dev, oracle):
+    # With accurate twiddles the same length reaches the exact DFT to ~1e-15.
+    from fft_amd import _lib as L
     D, torch = dev
     n = 20014
     x = cgauss(n, seed=5)
-    got = _run(D, torch, n, 1, x, torch.complex128)
-    assert rel_l2(got, np.fft.fft(x)) <= bound64(n)
+    got = _run(D, torch, n, 1, x, torch.complex128, flags=L.FLAG_ACCURATE_TWIDDLES)
+    assert rel_l2(got, np.fft.fft(x)) <= bound64(n) / 10
+    assert rel_l2(oracle.fft(x), np.fft.fft(x)) > 3e-13  # the reference's own error
 
 
 def test_config1_roundtrip(dev, oracle):
@@ -131,7 +145,7 @@ def test_mixed_length_batch(dev, oracle):
         worst = {}
         for n in lengths:
             seg = x[off: off + n]
-            want = np.fft.fft(seg) if n == 20014 else oracle.fft(seg)
+            want = oracle.fft(seg)
             e = rel_l2(y[off: off + n], want)
             worst[n] = max(worst.get(n, 0), e)
             off += n
@@ -150,4 +164,5 @@ def test_vjp_c2c_adjoint_identity(dev):
         gx = p.vjp(torch.from_numpy(gy).to("cuda")).cpu().numpy()
         lhs = np.vdot(gy, fx)
         rhs = np.vdot(gx, x)
-        assert abs(lhs - rhs) <= 1e-12 * abs(lhs), n
+        # 20014 reproduces the reference's ~1e-12 Bluestein rounding
+        assert abs(lhs - rhs) <= 1e-11 * abs(lhs), n
