@@ -61,6 +61,7 @@ struct ColMap {
 // Sub-FFT description shared by all kernels.
 struct SubFFT {
   int R;         // points per column
+  int Rp;        // LDS column pitch (R + R/16 + 1, see kernels.hip)
   int C;         // columns per workgroup (power of two)
   int log2C;
   int tpc;       // threads per column (power of two)
@@ -86,6 +87,26 @@ struct PassArgs {
   const long long* out_off;
   ColMap im, om;
   SubFFT f;
+  // --- convolution hooks (Rader / Bluestein as three fused passes) -------
+  // in_hook 1: element i of a line loads x[i] * w[i] (i < nvalid, else 0)
+  // in_hook 2: element i loads x[perm[i]]          (i < nvalid, else 0)
+  // out_hook 1: output p = conj(v) * w[p] for p < nvalid (others dropped)
+  // out_hook 2: output p -> x0 + conj(v) at iperm[p - conv_off] when
+  //             0 <= p - conv_off < nvalid, plus X[0] = x0 + dc[line] at p = 0
+  int in_hook, out_hook;
+  long long nvalid, conv_off;
+  const void* hook_w;
+  const int* hook_perm;
+  const void* rader_x;             // out_hook 2: the transform input (for x0)
+  int x0_conj;                     // out_hook 2: x0 enters conjugated (inverse)
+  const long long* rader_x_off;
+  long long rader_x_stride;
+  void* dc;                        // Rader: per-line A[0], written by the mid pass
+  // mid pass: after the first sub-FFT, multiply by mid_spec at the forward
+  // output position (mm map), conjugate, run the second sub-FFT f2.
+  const void* mid_spec;
+  ColMap mm;
+  SubFFT f2;
 };
 
 // Real-input forward (RFFT / PSD / spectrogram), single pass: each column is

@@ -2,10 +2,12 @@
 //
 // Layout conventions: complex data is interleaved (re, im) in the working
 // precision (double2 for the plain complex128 path, float2 for the complex64
-// path). A workgroup owns C columns of R points; LDS holds them as
-// lds[c * R + r]. Every LDS stage reads all its butterfly inputs into
-// registers, synchronises, then writes the outputs (in-place Stockham), so LDS
-// needs exactly C*R points.
+// path). A workgroup owns C columns of R points; LDS holds point r of column c
+// at lds[c * Rp + lp(r)], with an odd column pitch Rp = R + R/16 + 1 and one
+// pad slot after every 16 points, so that neither walking across columns
+// (load/store phases) nor the stride-radix writes of the first Stockham stages
+// hit one LDS bank. Every LDS stage reads all its butterfly inputs into
+// registers, synchronises, then writes the outputs (in-place Stockham).
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -196,11 +198,13 @@ template <int RAD, typename T2> __device__ __forceinline__ void dft(T2* v) {
 // One LDS stage over all columns of the workgroup. Thread (c, lt) owns the
 // butterflies j = lt, lt + tpc, ... of column c.
 
+__device__ __forceinline__ int lp(int p) { return p + (p >> 4); }
+
 template <typename T2, int RAD, int MAXP>
 __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage& s, const T2* stw,
                                           int c, int lt, int cdig) {
   constexpr int NB = MAXP / RAD > 0 ? MAXP / RAD : 1;
-  T2* col = lds + c * f.R;
+  T2* col = lds + c * f.Rp;
   const int nb = s.nb, ns = s.ns, tpc = f.tpc;
   T2 v[NB][RAD];
 #pragma unroll
@@ -208,7 +212,7 @@ __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage&
     int j = lt + m * tpc;
     if (j < nb) {
 #pragma unroll
-      for (int i = 0; i < RAD; ++i) v[m][i] = col[j + i * nb];
+      for (int i = 0; i < RAD; ++i) v[m][i] = col[lp(j + i * nb)];
       if (ns > 1 || s.cmul) {
         int q = (int)fastdiv_q((uint32_t)j, s.nsd);
         int k = j - q * ns;
@@ -226,9 +230,9 @@ __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage&
     if (j < nb) {
       int q = (int)fastdiv_q((uint32_t)j, s.nsd);
       int k = j - q * ns;
-      T2* o = col + q * ns * RAD + k;
+      const int o = q * ns * RAD + k;
 #pragma unroll
-      for (int i = 0; i < RAD; ++i) o[i * ns] = v[m][i];
+      for (int i = 0; i < RAD; ++i) col[lp(o + i * ns)] = v[m][i];
     }
   }
   __syncthreads();
@@ -257,8 +261,11 @@ __device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw, int cdig = 0) {
   }
 }
 
-__device__ __forceinline__ long long col_offset(const ColMap& m, long long g, long long* dtw,
-                                                int tw_digit, const long long* off) {
+// Column g's digits: returns the in-line part sum(d_i * a_i, i < 3); *line
+// gets the line part (offset table entry or d3 * a3), *d3 the line index.
+__device__ __forceinline__ long long col_digits(const ColMap& m, long long g, long long* dtw,
+                                               int tw_digit, const long long* off, long long* line,
+                                               long long* d3out) {
   long long d0 = g % m.D0;
   long long t = g / m.D0;
   long long d1 = t % m.D1;
@@ -266,7 +273,9 @@ __device__ __forceinline__ long long col_offset(const ColMap& m, long long g, lo
   long long d2 = t % m.D2;
   long long d3 = t / m.D2;
   if (dtw) *dtw = tw_digit == 0 ? d0 : d1;
-  return d0 * m.a0 + d1 * m.a1 + d2 * m.a2 + (off ? off[d3] : d3 * m.a3);
+  if (line) *line = off ? off[d3] : d3 * m.a3;
+  if (d3out) *d3out = d3;
+  return d0 * m.a0 + d1 * m.a1 + d2 * m.a2;
 }
 
 __device__ __forceinline__ double2 pass_twiddle(const double2* hi, const double2* lo, long long e,
@@ -295,68 +304,160 @@ template <typename T, int MAXP>
 __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
-  __shared__ long long s_ib[kMaxCols], s_ob[kMaxCols], s_tk[kMaxCols];
+  __shared__ long long s_ib[kMaxCols], s_il[kMaxCols], s_ob[kMaxCols], s_ol[kMaxCols], s_tk[kMaxCols];
+  __shared__ long long s_mk[kMaxCols], s_ln[kMaxCols];
+  __shared__ T2 s_x0[kMaxCols];
   const SubFFT& f = a.f;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int C = f.C, R = f.R;
+  const int C = f.C, R = f.R, Rp = f.Rp;
   const long long g0 = (long long)blockIdx.x * C;
   if (tid < C) {
     long long g = g0 + tid;
     if (g < a.ncols) {
-      long long k = 0;
-      s_ib[tid] = col_offset(a.im, g, &k, a.tw_digit, a.in_off);
-      s_ob[tid] = col_offset(a.om, g, nullptr, 0, a.out_off);
+      long long k = 0, lb = 0, d3 = 0;
+      s_il[tid] = col_digits(a.im, g, &k, a.tw_digit, a.in_off, &lb, &d3);
+      s_ib[tid] = lb;
+      s_ol[tid] = col_digits(a.om, g, nullptr, 0, a.out_off, &lb, nullptr);
+      s_ob[tid] = lb;
       s_tk[tid] = k;
+      s_ln[tid] = d3;
+      if (a.mid_spec) s_mk[tid] = col_digits(a.mm, g, nullptr, 0, nullptr, &lb, nullptr);
+      if (a.out_hook == 2) {
+        const T2* xr = reinterpret_cast<const T2*>(a.rader_x);
+        T2 x0 = xr[a.rader_x_off ? a.rader_x_off[d3] : d3 * a.rader_x_stride];
+        if (a.x0_conj) x0.y = -x0.y;
+        s_x0[tid] = x0;
+      }
     } else {
+      // Tail columns past ncols: marked invalid (base -1); every other
+      // per-column value is defined so that no path can index with garbage.
       s_ib[tid] = -1;
       s_ob[tid] = -1;
+      s_il[tid] = 0;
+      s_ol[tid] = 0;
       s_tk[tid] = 0;
+      s_mk[tid] = 0;
+      s_ln[tid] = 0;
+      s_x0[tid] = mkc<T2>(0, 0);
     }
   }
   __syncthreads();
   const T2* in = reinterpret_cast<const T2*>(a.in);
-  const int total = C * R;
-  for (int fi = tid; fi < total; fi += nthr) {
-    int c, r;
-    if (a.im.rowmajor) {
-      c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-      r = fi - c * R;
-    } else {
-      c = fi & (C - 1);
-      r = fi >> f.log2C;
+  const int total = C * R;  // <= MAXP * blockDim.x (guaranteed by the planner)
+  // Issue every global load of this thread before using any of them, so that
+  // MAXP loads per lane are in flight at once.
+  T2 buf[MAXP];
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int fi = tid + u * nthr;
+    buf[u] = mkc<T2>(0, 0);
+    if (fi < total) {
+      int c, r;
+      if (a.im.rowmajor) {
+        c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+        r = fi - c * R;
+      } else {
+        c = fi & (C - 1);
+        r = fi >> f.log2C;
+      }
+      const long long base = s_ib[c];
+      if (base >= 0) {
+        const long long i = s_il[c] + (long long)r * a.im.g;
+        if (a.in_hook == 0) {
+          buf[u] = in[base + i];
+        } else if (i < a.nvalid) {
+          buf[u] = in[base + (a.in_hook == 2 ? (long long)a.hook_perm[i] : i)];
+        }
+      }
     }
-    long long base = s_ib[c];
-    T2 v = mkc<T2>(0, 0);
-    if (base >= 0) {
-      v = in[base + (long long)r * a.im.g];
+  }
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int fi = tid + u * nthr;
+    if (fi < total) {
+      int c, r;
+      if (a.im.rowmajor) {
+        c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+        r = fi - c * R;
+      } else {
+        c = fi & (C - 1);
+        r = fi >> f.log2C;
+      }
+      T2 v = buf[u];
       if (a.tw_M) {
-        long long e = (long long)r * s_tk[c];
+        const long long e = (long long)r * s_tk[c];
         if (e) v = cmul(v, from_d2<T2>(pass_twiddle(a.ptw_hi, a.ptw_lo, e, a.tw_shift)));
       }
       if (a.conj_in) v.y = -v.y;
+      if (a.in_hook == 1 && s_ib[c] >= 0) {
+        const long long i = s_il[c] + (long long)r * a.im.g;
+        if (i < a.nvalid) v = cmul(v, reinterpret_cast<const T2*>(a.hook_w)[i]);
+      }
+      lds[c * Rp + lp(r)] = v;
     }
-    lds[c * R + r] = v;
   }
   __syncthreads();
   lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw), (int)s_tk[tid >> f.log2tpc]);
+  if (a.mid_spec) {
+    // forward output position k of point r of column c: s_mk[c] + r * mm.g
+    const T2* spec = reinterpret_cast<const T2*>(a.mid_spec);
+    const int c = tid >> f.log2tpc;
+    const int lt = tid & (f.tpc - 1);
+    if (s_ob[c] >= 0) {
+      for (int r = lt; r < R; r += f.tpc) {
+        const long long k = s_mk[c] + (long long)r * a.mm.g;
+        T2 v = lds[c * Rp + lp(r)];
+        if (a.dc && k == 0) reinterpret_cast<T2*>(a.dc)[s_ln[c]] = v;  // Rader: A[0]
+        lds[c * Rp + lp(r)] = cconj(cmul(v, spec[k]));
+      }
+    }
+    __syncthreads();
+    lds_fft<T2, MAXP>(lds, a.f2, reinterpret_cast<const T2*>(a.stw), 0);
+  }
   T2* out = reinterpret_cast<T2*>(a.out);
   const T sc = (T)a.scale;
   const bool do_scale = a.scale != 1.0;
-  for (int fi = tid; fi < total; fi += nthr) {
-    int c, r;
-    if (a.om.rowmajor) {
-      c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-      r = fi - c * R;
-    } else {
-      c = fi & (C - 1);
-      r = fi >> f.log2C;
-    }
-    long long base = s_ob[c];
-    if (base >= 0) {
-      T2 v = lds[c * R + r];
-      if (a.conj_out) v.y = -v.y;
-      if (do_scale) v = cscale(v, sc);
-      out[base + (long long)r * a.om.g] = v;
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int fi = tid + u * nthr;
+    if (fi < total) {
+      int c, r;
+      if (a.om.rowmajor) {
+        c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+        r = fi - c * R;
+      } else {
+        c = fi & (C - 1);
+        r = fi >> f.log2C;
+      }
+      const long long base = s_ob[c];
+      if (base >= 0) {
+        T2 v = lds[c * Rp + lp(r)];
+        const long long p = s_ol[c] + (long long)r * a.om.g;
+        if (a.out_hook == 0) {
+          if (a.conj_out) v.y = -v.y;
+          if (do_scale) v = cscale(v, sc);
+          out[base + p] = v;
+        } else if (a.out_hook == 1) {
+          if (p < a.nvalid) {
+            v = cmul(cconj(v), reinterpret_cast<const T2*>(a.hook_w)[p]);
+            if (a.conj_out) v.y = -v.y;
+            out[base + p] = cscale(v, sc);
+          }
+        } else {
+          const T2 x0 = s_x0[c];
+          const long long idx = p - a.conv_off;
+          if (idx >= 0 && idx < a.nvalid) {
+            T2 y = cadd(x0, cconj(v));
+            if (a.conj_out) y.y = -y.y;
+            out[base + a.hook_perm[idx]] = cscale(y, sc);
+          }
+          if (p == 0) {
+            T2 y = cadd(x0, reinterpret_cast<const T2*>(a.dc)[s_ln[c]]);
+            if (a.conj_out) y.y = -y.y;
+            out[base] = cscale(y, sc);
+          }
+        }
+      }
     }
   }
 }
@@ -370,22 +471,36 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
   const SubFFT& f = a.f;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int C = f.C, R = f.R;
+  const int C = f.C, R = f.R, Rp = f.Rp;
   const long long l0 = (long long)blockIdx.x * C;
   const T* in = reinterpret_cast<const T*>(a.in);
   const T* win = reinterpret_cast<const T*>(a.window);
   const int total = C * R;
-  for (int fi = tid; fi < total; fi += nthr) {
-    int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-    int r = fi - c * R;
-    long long l = l0 + c;
-    T2 v = mkc<T2>(0, 0);
-    if (l < a.nlines) {
-      const T* p = in + l * a.in_stride + 2 * (long long)r;
-      v = mkc<T2>(p[0], p[1]);
-      if (win) v = mkc<T2>(v.x * win[2 * r], v.y * win[2 * r + 1]);
+  T2 buf[MAXP];
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int fi = tid + u * nthr;
+    buf[u] = mkc<T2>(0, 0);
+    if (fi < total) {
+      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+      const int r = fi - c * R;
+      const long long l = l0 + c;
+      if (l < a.nlines) {
+        const T* p = in + l * a.in_stride + 2 * (long long)r;
+        buf[u] = mkc<T2>(p[0], p[1]);
+      }
     }
-    lds[c * R + r] = v;
+  }
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int fi = tid + u * nthr;
+    if (fi < total) {
+      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+      const int r = fi - c * R;
+      T2 v = buf[u];
+      if (win) v = mkc<T2>(v.x * win[2 * r], v.y * win[2 * r + 1]);
+      lds[c * Rp + lp(r)] = v;
+    }
   }
   __syncthreads();
   lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
@@ -397,13 +512,13 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
     int k = fi - c * nb;
     long long l = l0 + c;
     if (l >= a.nlines) continue;
-    const T2* Z = lds + c * R;
+    const T2* Z = lds + c * Rp;
     T2 X;
     if (k == 0 || k == R) {
       T2 z0 = Z[0];
       X = mkc<T2>(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0);
     } else {
-      T2 zk = Z[k], zmk = Z[R - k];
+      T2 zk = Z[lp(k)], zmk = Z[lp(R - k)];
       T xer = (zk.x + zmk.x) * (T)0.5;
       T xei = (zk.y - zmk.y) * (T)0.5;
       T xor_ = (zk.y + zmk.y) * (T)0.5;
@@ -432,18 +547,31 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
   const SubFFT& f = a.f;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int C = f.C, R = f.R;
+  const int C = f.C, R = f.R, Rp = f.Rp;
   const long long l0 = (long long)blockIdx.x * C;
   const T2* in = reinterpret_cast<const T2*>(a.in);
   const T2* utw = reinterpret_cast<const T2*>(a.utw);
   const int total = C * R;
-  for (int fi = tid; fi < total; fi += nthr) {
-    int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-    int r = fi - c * R;
-    long long l = l0 + c;
-    T2 v = mkc<T2>(0, 0);
-    if (l < a.nlines && r < a.in_bins) v = in[l * a.in_stride + r];
-    lds[c * R + r] = v;
+  T2 buf[MAXP];
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int fi = tid + u * nthr;
+    buf[u] = mkc<T2>(0, 0);
+    if (fi < total) {
+      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+      const int r = fi - c * R;
+      const long long l = l0 + c;
+      if (l < a.nlines && r < a.in_bins) buf[u] = in[l * a.in_stride + r];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int fi = tid + u * nthr;
+    if (fi < total) {
+      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+      const int r = fi - c * R;
+      lds[c * Rp + lp(r)] = buf[u];
+    }
   }
   __syncthreads();
   // Pairs (k, R-k) for k = 0 .. R/2, each owned by one thread; k = 0 pairs DC
@@ -454,7 +582,7 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
     int c = fi / half;
     int k = fi - c * half;
     long long l = l0 + c;
-    T2* X = lds + c * R;
+    T2* X = lds + c * Rp;
     if (k == 0) {
       T x0 = X[0].x;
       T xm = 0;
@@ -464,7 +592,7 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
       continue;
     }
     int mk_ = R - k;
-    T2 xk = X[k], xmk = X[mk_];
+    T2 xk = X[lp(k)], xmk = X[lp(mk_)];
     // Z[k] = xe + i*xo with xe = (Xk + conj Xmk)/2, d = (Xk - conj Xmk)/2,
     // xo = conj(W^k) d; Z[m-k] symmetric.
     T2 w = utw[k];
@@ -475,9 +603,9 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
       T2 zk = mkc<T2>(xer - xoi, xei + xor_);
       if (k != mk_) {
         T2 zmk = mkc<T2>(xer + xoi, -(xei - xor_));
-        X[mk_] = cconj(zmk);
+        X[lp(mk_)] = cconj(zmk);
       }
-      X[k] = cconj(zk);
+      X[lp(k)] = cconj(zk);
     }
   }
   __syncthreads();
@@ -489,10 +617,9 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
     int r = fi - c * R;
     long long l = l0 + c;
     if (l >= a.nlines) continue;
-    T2 z = lds[c * R + r];
-    T* p = out + l * a.out_stride + 2 * (long long)r;
-    p[0] = z.x * sc;
-    p[1] = -z.y * sc;
+    T2 z = lds[c * Rp + lp(r)];
+    // out_stride is even (n = 2R), so each sample pair is one aligned T2
+    *reinterpret_cast<T2*>(out + l * a.out_stride + 2 * (long long)r) = mkc<T2>(z.x * sc, -z.y * sc);
   }
 }
 
@@ -505,32 +632,43 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   __shared__ T2 s_red[16];  // one partial per wave
   const SubFFT& f = a.f;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int C = f.C, R = f.R;
+  const int C = f.C, R = f.R, Rp = f.Rp;
   const long long l0 = (long long)blockIdx.x * C;
   const T2* in = reinterpret_cast<const T2*>(a.in);
   const T2* chirp = reinterpret_cast<const T2*>(a.chirp);
   const int n = a.n;
   const int q = n - 1;
   const int total = C * R;
-  for (int fi = tid; fi < total; fi += nthr) {
-    int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-    int r = fi - c * R;
-    long long l = l0 + c;
-    T2 v = mkc<T2>(0, 0);
-    if (l < a.nlines) {
-      const T2* x = in + (a.in_off ? a.in_off[l] : l * a.in_stride);
-      if (a.kind == 2) {
-        if (r < n) {
-          v = x[r];
-          if (a.conj_in) v.y = -v.y;
-          v = cmul(v, chirp[r]);
+  T2 buf[MAXP];
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int fi = tid + u * nthr;
+    buf[u] = mkc<T2>(0, 0);
+    if (fi < total) {
+      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+      const int r = fi - c * R;
+      const long long l = l0 + c;
+      if (l < a.nlines) {
+        const T2* x = in + (a.in_off ? a.in_off[l] : l * a.in_stride);
+        if (a.kind == 2) {
+          if (r < n) buf[u] = x[r];
+        } else if (r < q) {
+          buf[u] = x[a.perm[r]];
         }
-      } else if (r < q) {
-        v = x[a.perm[r]];
-        if (a.conj_in) v.y = -v.y;
       }
     }
-    lds[c * R + r] = v;
+  }
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int fi = tid + u * nthr;
+    if (fi < total) {
+      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+      const int r = fi - c * R;
+      T2 v = buf[u];
+      if (a.conj_in) v.y = -v.y;
+      if (a.kind == 2 && r < n) v = cmul(v, chirp[r]);
+      lds[c * Rp + lp(r)] = v;
+    }
   }
   __syncthreads();
   const int c = tid >> f.log2tpc;
@@ -539,7 +677,7 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   if (a.kind != 2) {
     // Rader DC: sum of x[1..n-1] (all gathered samples) plus x[0].
     T2 acc = mkc<T2>(0, 0);
-    for (int r = lt; r < q; r += f.tpc) acc = cadd(acc, lds[c * R + r]);
+    for (int r = lt; r < q; r += f.tpc) acc = cadd(acc, lds[c * Rp + lp(r)]);
     acc = group_sum(acc, f.tpc < 64 ? f.tpc : 64);
     if (f.tpc > 64) {
       if ((tid & 63) == 0) s_red[tid >> 6] = acc;
@@ -553,7 +691,7 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   const T2* stw = reinterpret_cast<const T2*>(a.stw);
   lds_fft<T2, MAXP>(lds, f, stw);
   const T2* spec = reinterpret_cast<const T2*>(a.spec);
-  for (int r = lt; r < R; r += f.tpc) lds[c * R + r] = cconj(cmul(lds[c * R + r], spec[r]));
+  for (int r = lt; r < R; r += f.tpc) lds[c * Rp + lp(r)] = cconj(cmul(lds[c * Rp + lp(r)], spec[r]));
   __syncthreads();
   lds_fft<T2, MAXP>(lds, f, stw);
   const long long l = l0 + c;
@@ -562,7 +700,7 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   const T sc = (T)a.scale;
   if (a.kind == 2) {
     for (int k = lt; k < n && live; k += f.tpc) {
-      T2 v = cmul(cconj(lds[c * R + k]), chirp[k]);
+      T2 v = cmul(cconj(lds[c * Rp + lp(k)]), chirp[k]);
       if (a.conj_out) v.y = -v.y;
       out[k] = cscale(v, sc);
     }
@@ -572,7 +710,7 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
     if (a.conj_in) x0.y = -x0.y;
     const int off = a.kind == 1 ? q : 0;
     for (int k = lt; k < q && live; k += f.tpc) {
-      T2 v = cadd(x0, cconj(lds[c * R + k + off]));
+      T2 v = cadd(x0, cconj(lds[c * Rp + lp(k + off)]));
       if (a.conj_out) v.y = -v.y;
       out[a.iperm[k]] = cscale(v, sc);
     }
@@ -905,10 +1043,10 @@ template <typename T> static size_t elt2() { return 2 * sizeof(T); }
 hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s) {
   long long groups = (a.ncols + a.f.C - 1) / a.f.C;
   if (prec == 0) {
-    size_t bytes = (size_t)a.f.C * a.f.R * elt2<double>();
+    size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
     GOFFT_DISPATCH_MAXP(k_pass, double, a, bytes, threads, groups, s);
   } else {
-    size_t bytes = (size_t)a.f.C * a.f.R * elt2<float>();
+    size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
     GOFFT_DISPATCH_MAXP(k_pass, float, a, bytes, threads, groups, s);
   }
   return hipGetLastError();
@@ -917,10 +1055,10 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
 hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads, hipStream_t s) {
   long long groups = (a.nlines + a.f.C - 1) / a.f.C;
   if (prec == 0) {
-    size_t bytes = (size_t)a.f.C * a.f.R * elt2<double>();
+    size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
     GOFFT_DISPATCH_MAXP(k_real_fwd, double, a, bytes, threads, groups, s);
   } else {
-    size_t bytes = (size_t)a.f.C * a.f.R * elt2<float>();
+    size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
     GOFFT_DISPATCH_MAXP(k_real_fwd, float, a, bytes, threads, groups, s);
   }
   return hipGetLastError();
@@ -929,10 +1067,10 @@ hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads
 hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads, hipStream_t s) {
   long long groups = (a.nlines + a.f.C - 1) / a.f.C;
   if (prec == 0) {
-    size_t bytes = (size_t)a.f.C * a.f.R * elt2<double>();
+    size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
     GOFFT_DISPATCH_MAXP(k_real_inv, double, a, bytes, threads, groups, s);
   } else {
-    size_t bytes = (size_t)a.f.C * a.f.R * elt2<float>();
+    size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
     GOFFT_DISPATCH_MAXP(k_real_inv, float, a, bytes, threads, groups, s);
   }
   return hipGetLastError();
@@ -941,10 +1079,10 @@ hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads
 hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipStream_t s) {
   long long groups = (a.nlines + a.f.C - 1) / a.f.C;
   if (prec == 0) {
-    size_t bytes = (size_t)a.f.C * a.f.R * elt2<double>();
+    size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
     GOFFT_DISPATCH_MAXP(k_conv, double, a, bytes, threads, groups, s);
   } else {
-    size_t bytes = (size_t)a.f.C * a.f.R * elt2<float>();
+    size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
     GOFFT_DISPATCH_MAXP(k_conv, float, a, bytes, threads, groups, s);
   }
   return hipGetLastError();

@@ -306,6 +306,14 @@ std::vector<int> radix_schedule(int R) {
 
 constexpr int kMaxP = 16;  // points per thread per stage (kernel template)
 
+// LDS column pitch for R points (kernels.hip: one pad slot per 16 points plus
+// one, odd so adjacent columns fall in different banks).
+i64 pitch(i64 R) { return R + (R >> 4) + 1; }
+
+// Preferred LDS footprint (points) of a batched single-pass workgroup: about
+// 64 KB, so that two or more workgroups share a CU's 160 KB.
+i64 soft_pts(int prec) { return prec == 0 ? 4352 : 8704; }
+
 struct SubPlan {
   SubFFT f{};
   int threads = 0;
@@ -333,7 +341,8 @@ bool make_sub(int R, int C, SubPlan& sp) {
   bool ok = false;
   for (; T <= 1024; T <<= 1) {
     int tpc = T / C;
-    ok = true;
+    // every thread holds at most kMaxP points in the load/store phases
+    ok = (i64)T * kMaxP >= (i64)C * R;
     for (int rad : rads) {
       int nb = R / rad;
       int per = (nb + tpc - 1) / tpc;
@@ -350,6 +359,7 @@ bool make_sub(int R, int C, SubPlan& sp) {
   sp.threads = T;
   SubFFT& f = sp.f;
   f.R = R;
+  f.Rp = (int)pitch(R);
   f.C = C;
   f.log2C = ilog2(C);
   f.tpc = T / C;
@@ -394,6 +404,7 @@ bool make_sub_r2(int R, int C, i64 Ns, SubPlan& sp) {
   sp.threads = T;
   SubFFT& f = sp.f;
   f.R = R;
+  f.Rp = (int)pitch(R);
   f.C = C;
   f.log2C = ilog2(C);
   f.tpc = T / C;
@@ -540,9 +551,9 @@ int upload_dcx(DevBuf& b, const std::vector<dcx>& v, int prec, double scale = 1.
 // budget `pts` and a feasible schedule.
 bool pick_cols(int R, int cap, i64 pts, SubPlan& sp, int* Cout) {
   int C = 1;
-  while (C * 2 <= cap && (i64)C * 2 * R <= pts) C *= 2;
+  while (C * 2 <= cap && (i64)C * 2 * pitch(R) <= pts) C *= 2;
   for (; C >= 1; C /= 2) {
-    if ((i64)C * R > pts) continue;
+    if ((i64)C * pitch(R) > pts) continue;
     if (make_sub(R, C, sp)) {
       *Cout = C;
       return true;
@@ -571,10 +582,38 @@ class Engine {
   virtual ~Engine() {}
   virtual size_t ws_bytes() const = 0;
   virtual int launches() const = 0;
-  // out = scale * conj?( FFT( conj?(in) ) ) per line.
+  // out = scale * conj?( FFT( conj?(in) ) ) per line, for the first `nl`
+  // lines (all `batch` lines when nl < 0).
   virtual int run(const void* in, LineIO li, void* out, LineIO lo, int conj_in, int conj_out,
-                  double scale, const Ctx& c) const = 0;
+                  double scale, const Ctx& c, i64 nl = -1) const = 0;
 };
+
+// Lines of a multi-kernel transform are processed in chunks whose
+// intermediate buffers stay within this many bytes each, so that passes after
+// the first read from the 256 MB Infinity Cache rather than HBM.
+i64 chunk_bytes() {
+  static i64 v = [] {
+    const char* e = std::getenv("GOFFT_CHUNK_BYTES");
+    i64 x = e ? std::atoll(e) : 0;
+    return x > 0 ? x : (i64)64 << 20;
+  }();
+  return v;
+}
+
+i64 chunk_lines(i64 per_line_bytes, i64 batch) {
+  i64 cb = std::max<i64>(1, chunk_bytes() / std::max<i64>(1, per_line_bytes));
+  return std::max<i64>(1, std::min<i64>(cb, batch));
+}
+
+// Advance a line address by b0 lines: offset tables index from b0, strided
+// lines move the base pointer.
+const void* line_ptr(const void* p, const LineIO& li, i64 b0, size_t eb) {
+  return li.off ? p : (const void*)((const char*)p + (size_t)(b0 * li.stride) * eb);
+}
+void* line_ptr(void* p, const LineIO& li, i64 b0, size_t eb) {
+  return li.off ? p : (void*)((char*)p + (size_t)(b0 * li.stride) * eb);
+}
+LineIO line_sub(const LineIO& li, i64 b0) { return LineIO{li.off ? li.off + b0 : nullptr, li.stride}; }
 
 std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, int* rc);
 
@@ -589,9 +628,10 @@ class TrivialEngine : public Engine {
   size_t ws_bytes() const override { return 0; }
   int launches() const override { return n == 1 ? 1 : 0; }
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
-          const Ctx& c) const override {
-    if (n != 1 || batch == 0) return GOFFT_OK;
-    HIP_TRY(launch_cmul_table(prec, in, out, one.p, 1, batch, li.stride, lo.stride, ci, co, scale, li.off,
+          const Ctx& c, i64 nl) const override {
+    const i64 lines = nl < 0 ? batch : nl;
+    if (n != 1 || lines == 0) return GOFFT_OK;
+    HIP_TRY(launch_cmul_table(prec, in, out, one.p, 1, lines, li.stride, lo.stride, ci, co, scale, li.off,
                               lo.off, c.s));
     return GOFFT_OK;
   }
@@ -648,20 +688,28 @@ class StockhamEngine : public Engine {
   // Power-of-two transform with the reference's recurrence twiddles
   // (kernels.Radix2): radix-2 layers only, grouped into passes of up to 2^8
   // layers; no pass twiddles (each layer reads its own table).
-  int init_recurrence() {
-    DevInfo dev;
-    RC_TRY(current_device(&dev));
-    if (n < 2 || (n & (n - 1)) || st != 1 || inner != 1)
-      return fail(GOFFT_ERR_UNSUPPORTED, "recurrence transform needs a contiguous power of two");
-    const i64 pts = dev.lds_bytes / (i64)elt(prec);
+  // log2 pass lengths of a recurrence transform: one pass when it fits the
+  // preferred LDS budget, else balanced passes of at most 2^8 points.
+  static std::vector<int> recurrence_qs(i64 n, int prec, i64 pts) {
     const int q = ilog2((int)std::min<i64>(n, 1 << 30));
     std::vector<int> qs;
-    if (n <= std::min<i64>(pts, 4096)) {
+    if (pitch(n) <= std::min<i64>(pts, soft_pts(prec))) {
       qs.push_back(q);
     } else {
       int P = (q + 7) / 8;
       for (int i = 0; i < P; i++) qs.push_back(q / P + (i < q % P ? 1 : 0));
     }
+    return qs;
+  }
+
+  int init_recurrence(std::vector<int> qs = {}) {
+    DevInfo dev;
+    RC_TRY(current_device(&dev));
+    if (n < 2 || (n & (n - 1)) || st != 1 || inner != 1)
+      return fail(GOFFT_ERR_UNSUPPORTED, "recurrence transform needs a contiguous power of two");
+    const i64 pts = dev.lds_bytes / (i64)elt(prec);
+    if (qs.empty()) qs = recurrence_qs(n, prec, pts);
+    passes.clear();
     i64 Ns = 1;
     for (int qp : qs) {
       Pass p;
@@ -669,7 +717,8 @@ class StockhamEngine : public Engine {
       p.Ns = Ns;
       int C = qs.size() == 1 ? 1 : 8;
       if (qs.size() == 1) {
-        while (C * 2 <= kMaxCols && C * 2 <= batch && (i64)C * 2 * p.R <= std::min<i64>(pts, 4096)) C *= 2;
+        while (C * 2 <= kMaxCols && C * 2 <= batch && (i64)C * 2 * pitch(p.R) <= std::min<i64>(pts, soft_pts(prec)))
+          C *= 2;
       }
       if (!make_sub_r2((int)p.R, C, Ns, p.sp))
         return fail(GOFFT_ERR_UNSUPPORTED, "no radix-2 schedule for pass length " + std::to_string(p.R));
@@ -706,39 +755,62 @@ class StockhamEngine : public Engine {
     const i64 lines = batch * inner;  // columns of a single pass
     // Single pass: all n points of a line in one workgroup. Strided lines need
     // several adjacent lines per workgroup for coalesced access.
-    const i64 target = std::max<i64>(n, std::min<i64>(pts, prec == 0 ? 4096 : 8192));
     const int cwant = st == 1 ? 1 : (int)std::min<i64>(prec == 0 ? 8 : 16, st);
     const int ccap = (int)std::min<i64>(kMaxCols, std::max<i64>(cwant, lines));
+    const i64 budget = std::min<i64>(pts, std::max<i64>(pitch(n) * cwant, soft_pts(prec)));
     SubPlan sp;
     int C = 0;
-    if (n <= pts && (i64)cwant * n <= std::max<i64>(pts, n) && pick_cols((int)n, ccap, target, sp, &C) &&
-        C >= cwant) {
+    if (pitch(n) * cwant <= pts && pick_cols((int)n, ccap, budget, sp, &C) && C >= cwant) {
       Pass p;
       p.sp = sp;
       p.R = n;
       p.Ns = 1;
       passes.push_back(p);
     } else {
-      // Multi-pass: pass lengths <= rmax with cmp columns per workgroup.
-      int cmp = prec == 0 ? 8 : 16;
-      i64 rmax = std::min<i64>(pts / cmp, 1024);
       std::vector<i64> lens;
-      bool ok = false;
-      for (int Pn = 2; Pn <= 6 && !ok; Pn++) ok = factor_passes(n, rmax, Pn, lens);
-      if (!ok) return fail(GOFFT_ERR_UNSUPPORTED, "no pass decomposition for n=" + std::to_string(n));
-      i64 Ns = 1;
-      for (i64 R : lens) {
-        Pass p;
-        int Cp = 0;
-        if (!pick_cols((int)R, cmp, pts, p.sp, &Cp))
-          return fail(GOFFT_ERR_UNSUPPORTED, "no LDS schedule for pass length " + std::to_string(R));
-        p.R = R;
-        p.Ns = Ns;
-        passes.push_back(p);
-        Ns *= R;
-      }
+      RC_TRY(multi_lengths(lens));
+      return init_lengths(lens);
     }
-    // Column maps and tables.
+    return finish();
+  }
+
+  // Pass lengths of a multi-pass decomposition: each <= rmax so that cmp
+  // columns of it fit in LDS, as few passes as possible, balanced.
+  int multi_lengths(std::vector<i64>& lens) const {
+    DevInfo dev;
+    RC_TRY(current_device(&dev));
+    const i64 pts = dev.lds_bytes / (i64)elt(prec);
+    int cmp = prec == 0 ? 8 : 16;
+    i64 rmax = std::min<i64>(pts / cmp * 16 / 17, 1024);
+    bool ok = false;
+    for (int Pn = 2; Pn <= 6 && !ok; Pn++) ok = factor_passes(n, rmax, Pn, lens);
+    if (!ok) return fail(GOFFT_ERR_UNSUPPORTED, "no pass decomposition for n=" + std::to_string(n));
+    return GOFFT_OK;
+  }
+
+  // Multi-pass with the given pass lengths (product n), in order.
+  int init_lengths(const std::vector<i64>& lens) {
+    DevInfo dev;
+    RC_TRY(current_device(&dev));
+    const i64 pts = dev.lds_bytes / (i64)elt(prec);
+    const int cmp = prec == 0 ? 8 : 16;
+    passes.clear();
+    i64 Ns = 1;
+    for (i64 R : lens) {
+      Pass p;
+      int Cp = 0;
+      if (!pick_cols((int)R, cmp, pts, p.sp, &Cp))
+        return fail(GOFFT_ERR_UNSUPPORTED, "no LDS schedule for pass length " + std::to_string(R));
+      p.R = R;
+      p.Ns = Ns;
+      passes.push_back(p);
+      Ns *= R;
+    }
+    return finish();
+  }
+
+  // Column maps and tables for the chosen passes.
+  int finish() {
     std::vector<cld> stw_h;
     std::vector<cld> ptw_h;
     for (size_t i = 0; i < passes.size(); i++) {
@@ -781,18 +853,41 @@ class StockhamEngine : public Engine {
     return GOFFT_OK;
   }
 
+  // Outer units (d3 digit) per chunk of a multi-pass run.
+  i64 chunk() const { return chunk_lines((i64)n * inner * (i64)elt(prec), batch); }
+
   size_t ws_bytes() const override {
-    // ping-pong copies of the whole (strided) array
-    size_t one = (size_t)n * (size_t)inner * (size_t)batch * elt(prec);
+    // ping-pong buffers for one chunk of outer units
     if (P() <= 1) return 0;
+    size_t one = (size_t)n * (size_t)inner * (size_t)chunk() * elt(prec);
     return P() == 2 ? one : 2 * one;
   }
-  int launches() const override { return P(); }
+  int launches() const override {
+    if (P() <= 1) return P();
+    return (int)(P() * ((batch + chunk() - 1) / chunk()));
+  }
 
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
-          const Ctx& c) const override {
-    if (batch == 0) return GOFFT_OK;
-    const size_t one = (size_t)n * (size_t)inner * (size_t)batch * elt(prec);
+          const Ctx& c, i64 nl) const override {
+    const i64 lines = nl < 0 ? batch : nl;
+    if (lines == 0) return GOFFT_OK;
+    if (P() <= 1) return run_chunk(in, li, out, lo, ci, co, scale, c, lines);
+    // Strided (N-D) lines advance by whole outer blocks of n*st elements.
+    const i64 cb = chunk();
+    const size_t eb = elt(prec);
+    for (i64 b0 = 0; b0 < lines; b0 += cb) {
+      const i64 nb = std::min<i64>(cb, lines - b0);
+      LineIO lis = inner == 1 ? li : LineIO{nullptr, n * st};
+      LineIO los = inner == 1 ? lo : LineIO{nullptr, n * st};
+      RC_TRY(run_chunk(line_ptr(in, lis, b0, eb), line_sub(li, b0), line_ptr(out, los, b0, eb),
+                       line_sub(lo, b0), ci, co, scale, c, nb));
+    }
+    return GOFFT_OK;
+  }
+
+  int run_chunk(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
+                const Ctx& c, i64 lines) const {
+    const size_t one = (size_t)n * (size_t)inner * (size_t)chunk() * elt(prec);
     char* bufs[2] = {c.ws, c.ws + one};
     for (int i = 0; i < P(); i++) {
       const Pass& p = passes[i];
@@ -802,7 +897,7 @@ class StockhamEngine : public Engine {
       a.stw = (const char*)stw.p + p.stw_off * elt(prec);
       a.ptw_hi = p.tw_M ? (const double2*)ptw.p + p.ptw_hi : nullptr;
       a.ptw_lo = p.tw_M ? (const double2*)ptw.p + p.ptw_lo : nullptr;
-      a.ncols = batch * inner * (n / p.R);
+      a.ncols = lines * inner * (n / p.R);
       a.tw_M = p.tw_M;
       a.tw_shift = p.tw_shift;
       a.tw_digit = p.tw_digit;
@@ -826,6 +921,174 @@ class StockhamEngine : public Engine {
   }
 };
 
+// Cyclic convolution of length m over lines in 2P-1 kernels. The forward
+// transform runs passes with radices (R1..RP) and the inverse runs them
+// reversed (RP..R1): the forward's last pass writes column j's outputs to
+// j + r*Ns with Ns = m/RP, which are exactly the inputs of the inverse's first
+// pass column j. One kernel therefore runs both around the spectrum multiply.
+// The Bluestein chirp or the Rader gather rides on the first load, and the
+// chirp or the Rader scatter on the last store.
+struct ConvChain {
+  int prec = 0;
+  i64 m = 0, cb = 0;
+  StockhamEngine fwd, inv;
+
+  struct Hooks {
+    int in_hook = 0, out_hook = 0;
+    i64 nvalid = 0, conv_off = 0;
+    const void* w = nullptr;
+    const int* perm = nullptr;
+    const int* iperm = nullptr;
+    const void* spec = nullptr;
+  };
+
+  int init(i64 m_, i64 cb_, int prec_, bool recurrence) {
+    m = m_;
+    cb = cb_;
+    prec = prec_;
+    for (StockhamEngine* e : {&fwd, &inv}) {
+      e->prec = prec;
+      e->n = m;
+      e->batch = cb;
+    }
+    if (recurrence) {
+      DevInfo dev;
+      RC_TRY(current_device(&dev));
+      std::vector<int> qs = StockhamEngine::recurrence_qs(m, prec, dev.lds_bytes / (i64)elt(prec));
+      if (qs.size() < 2) {
+        int q = qs[0];
+        qs = {q - q / 2, q / 2};
+      }
+      RC_TRY(fwd.init_recurrence(qs));
+      std::reverse(qs.begin(), qs.end());
+      RC_TRY(inv.init_recurrence(qs));
+    } else {
+      std::vector<i64> lens;
+      RC_TRY(fwd.multi_lengths(lens));
+      RC_TRY(fwd.init_lengths(lens));
+      std::reverse(lens.begin(), lens.end());
+      RC_TRY(inv.init_lengths(lens));
+    }
+    const Pass& a = fwd.passes.back();
+    const Pass& b = inv.passes.front();
+    if (fwd.P() < 2 || a.R != b.R || a.sp.f.C != b.sp.f.C || a.sp.threads != b.sp.threads)
+      return fail(GOFFT_ERR_UNSUPPORTED, "convolution passes do not pair for m=" + std::to_string(m));
+    return GOFFT_OK;
+  }
+
+  size_t ws_bytes() const { return 2 * (size_t)m * cb * elt(prec) + (size_t)cb * elt(prec); }
+  int launches() const { return 2 * fwd.P() - 1; }
+
+  void fill(PassArgs& a, const StockhamEngine& e, const Pass& p, i64 nl) const {
+    a.stw = (const char*)e.stw.p + p.stw_off * elt(prec);
+    a.ptw_hi = p.tw_M ? (const double2*)e.ptw.p + p.ptw_hi : nullptr;
+    a.ptw_lo = p.tw_M ? (const double2*)e.ptw.p + p.ptw_lo : nullptr;
+    a.ncols = nl * (m / p.R);
+    a.tw_M = p.tw_M;
+    a.tw_shift = p.tw_shift;
+    a.tw_digit = p.tw_digit;
+    a.scale = 1.0;
+    a.im = p.im;
+    a.om = p.om;
+    a.f = p.sp.f;
+  }
+
+  // One chunk of nl <= cb lines.
+  int run(const Hooks& h, const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
+          char* ws, hipStream_t s, i64 nl) const {
+    char* bufs[2] = {ws, ws + (size_t)m * cb * elt(prec)};
+    void* dc = ws + 2 * (size_t)m * cb * elt(prec);
+    const int P = fwd.P();
+    const int K = 2 * P - 1;
+    for (int k = 0; k < K; k++) {
+      PassArgs a{};
+      int threads = 0;
+      if (k < P - 1) {
+        fill(a, fwd, fwd.passes[k], nl);
+        threads = fwd.passes[k].sp.threads;
+      } else if (k == P - 1) {
+        const Pass& pf = fwd.passes[P - 1];
+        const Pass& pi = inv.passes[0];
+        fill(a, fwd, pf, nl);
+        a.mm = pf.om;
+        a.om = pi.om;
+        a.mid_spec = h.spec;
+        a.f2 = pi.sp.f;
+        if (h.out_hook == 2) a.dc = dc;
+        threads = pf.sp.threads;
+      } else {
+        const Pass& pi = inv.passes[k - (P - 1)];
+        fill(a, inv, pi, nl);
+        threads = pi.sp.threads;
+      }
+      a.in = k == 0 ? in : bufs[(k - 1) % 2];
+      a.out = k == K - 1 ? out : bufs[k % 2];
+      if (k == 0) {
+        a.im.a3 = li.stride;
+        a.in_off = li.off;
+        a.in_hook = h.in_hook;
+        a.nvalid = h.nvalid;
+        a.hook_w = h.w;
+        a.hook_perm = h.perm;
+        a.conj_in = ci;
+      }
+      if (k == K - 1) {
+        a.om.a3 = lo.stride;
+        a.out_off = lo.off;
+        a.out_hook = h.out_hook;
+        a.nvalid = h.nvalid;
+        a.conv_off = h.conv_off;
+        a.hook_w = h.w;
+        a.hook_perm = h.iperm;
+        a.conj_out = co;
+        a.scale = scale;
+        if (h.out_hook == 2) {
+          a.rader_x = in;
+          a.rader_x_off = li.off;
+          a.rader_x_stride = li.stride;
+          a.x0_conj = ci;
+          a.dc = dc;
+        }
+      }
+      HIP_TRY(launch_pass(prec, kMaxP, a, threads, s));
+    }
+    return GOFFT_OK;
+  }
+};
+
+// Convolution length for Rader's linear path and for Bluestein outside the
+// reference-twiddle mode: a 7-smooth length in [lo, ceil(1.4 lo)] that a single
+// LDS pass can hold if one exists, else the cheapest by the reference's cost
+// model (plan.go:192-259).
+i64 choose_conv_len(i64 lo, int prec) {
+  DevInfo dev;
+  if (current_device(&dev) != GOFFT_OK) dev.lds_bytes = 65536;
+  const i64 pts = dev.lds_bytes / (i64)elt(prec);
+  i64 hi = lo + (lo * 4 + 9) / 10;
+  i64 best = -1;
+  double bestc = 0;
+  int bestfit = 0;
+  for (i64 m = lo; m <= hi; m++) {
+    i64 x = m;
+    double cost = 0;
+    while (x % 4 == 0) { cost += 2.0; x /= 4; }
+    while (x % 2 == 0) { cost += 2.0; x /= 2; }
+    while (x % 3 == 0) { cost += 2.0; x /= 3; }
+    while (x % 5 == 0) { cost += 2.8; x /= 5; }
+    while (x % 7 == 0) { cost += 2.6; x /= 7; }
+    if (x != 1) continue;
+    cost *= (double)m;
+    SubPlan tmp;
+    int fit = (pitch(m) <= pts && make_sub((int)m, 1, tmp)) ? 1 : 0;
+    if (best < 0 || fit > bestfit || (fit == bestfit && cost < bestc)) {
+      best = m;
+      bestc = cost;
+      bestfit = fit;
+    }
+  }
+  return best;
+}
+
 // Rader's algorithm for prime n (rader.go:39-172): gather by the generator
 // permutation, cyclic convolution with the precomputed kernel spectrum,
 // scatter by the inverse permutation.
@@ -839,9 +1102,10 @@ class RaderEngine : public Engine {
   SubPlan sp;
   DevBuf stw;
   // multi-kernel path
-  std::unique_ptr<Engine> inner;
+  ConvChain chain;
 
   int init(unsigned flags) {
+    (void)flags;
     q = n - 1;
     const i64 g = primitive_root(n);
     if (factors_are_small(q)) {
@@ -849,7 +1113,7 @@ class RaderEngine : public Engine {
       cl = q;
     } else {
       cyclic = false;
-      cl = choose_conv_len(2 * q);
+      cl = choose_conv_len(2 * q, prec);
     }
     std::vector<int> ph(q), iph(q);
     i64 x = 1;
@@ -872,58 +1136,27 @@ class RaderEngine : public Engine {
     RC_TRY(current_device(&dev));
     const i64 pts = dev.lds_bytes / (i64)elt(prec);
     int C = 0;
-    if (cl <= pts && pick_cols((int)cl, 16, std::min<i64>(pts, std::max<i64>(cl, prec == 0 ? 4096 : 8192)), sp, &C)) {
+    if (pitch(cl) <= pts &&
+        pick_cols((int)cl, 16, std::min<i64>(pts, std::max<i64>(pitch(cl), soft_pts(prec))), sp, &C)) {
       fused = true;
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
-      int rc = 0;
-      inner = make_engine(cl, batch, prec, flags, &rc);
-      if (!inner) return rc;
+      cb = chunk_lines(2 * cl * (i64)elt(prec), batch);
+      RC_TRY(chain.init(cl, cb, prec, false));
     }
     return GOFFT_OK;
   }
+  i64 cb = 1;  // lines per chunk (multi-kernel path)
 
-  // Smallest 7-smooth length in [lo, ceil(1.4 lo)] that is cheapest for the
-  // GPU: fewest kernel passes first, then the reference's cost model
-  // (plan.go:192-259).
-  i64 choose_conv_len(i64 lo) {
-    DevInfo dev;
-    if (current_device(&dev) != GOFFT_OK) dev.lds_bytes = 65536;
-    const i64 pts = dev.lds_bytes / (i64)elt(prec);
-    i64 hi = lo + (lo * 4 + 9) / 10;
-    i64 best = -1;
-    double bestc = 0;
-    int bestfit = 0;
-    for (i64 m = lo; m <= hi; m++) {
-      i64 x = m;
-      double cost = 0;
-      while (x % 4 == 0) { cost += 2.0; x /= 4; }
-      while (x % 2 == 0) { cost += 2.0; x /= 2; }
-      while (x % 3 == 0) { cost += 2.0; x /= 3; }
-      while (x % 5 == 0) { cost += 2.8; x /= 5; }
-      while (x % 7 == 0) { cost += 2.6; x /= 7; }
-      if (x != 1) continue;
-      cost *= (double)m;
-      SubPlan tmp;
-      int fit = (m <= pts && make_sub((int)m, 1, tmp)) ? 1 : 0;
-      if (best < 0 || fit > bestfit || (fit == bestfit && cost < bestc)) {
-        best = m;
-        bestc = cost;
-        bestfit = fit;
-      }
-    }
-    return best;
+  size_t ws_bytes() const override { return fused ? 0 : chain.ws_bytes(); }
+  int launches() const override {
+    return fused ? 1 : (int)(chain.launches() * ((batch + cb - 1) / cb));
   }
-
-  size_t ws_bytes() const override {
-    if (fused) return 0;
-    return 2 * (size_t)cl * batch * elt(prec) + inner->ws_bytes();
-  }
-  int launches() const override { return fused ? 1 : 3 + 2 * inner->launches(); }
 
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
-          const Ctx& c) const override {
-    if (batch == 0) return GOFFT_OK;
+          const Ctx& c, i64 nl) const override {
+    const i64 lines = nl < 0 ? batch : nl;
+    if (lines == 0) return GOFFT_OK;
     if (fused) {
       ConvArgs a{};
       a.in = in;
@@ -934,7 +1167,7 @@ class RaderEngine : public Engine {
       a.iperm = (const int*)iperm.p;
       a.in_off = li.off;
       a.out_off = lo.off;
-      a.nlines = batch;
+      a.nlines = lines;
       a.in_stride = li.stride;
       a.out_stride = lo.stride;
       a.n = (int)n;
@@ -946,17 +1179,20 @@ class RaderEngine : public Engine {
       HIP_TRY(launch_conv(prec, kMaxP, a, sp.threads, c.s));
       return GOFFT_OK;
     }
-    char* A = c.ws;
-    char* B = c.ws + (size_t)cl * batch * elt(prec);
-    Ctx ic{c.s, B + (size_t)cl * batch * elt(prec)};
-    LineIO cont{nullptr, cl};
-    HIP_TRY(launch_rader_gather(prec, in, A, (const int*)perm.p, q, cl, batch, li.stride, cl, ci, li.off,
-                                c.s));
-    RC_TRY(inner->run(A, cont, B, cont, 0, 0, 1.0, ic));
-    HIP_TRY(launch_cmul_table(prec, B, A, spec.p, cl, batch, cl, cl, 0, 1, 1.0, nullptr, nullptr, c.s));
-    RC_TRY(inner->run(A, cont, B, cont, 0, 1, 1.0, ic));
-    HIP_TRY(launch_rader_scatter(prec, in, B, out, (const int*)iperm.p, n, cyclic ? 0 : q, batch,
-                                 li.stride, cl, lo.stride, ci, co, scale, li.off, lo.off, c.s));
+    ConvChain::Hooks h;
+    h.in_hook = 2;
+    h.out_hook = 2;
+    h.nvalid = q;
+    h.conv_off = cyclic ? 0 : q;
+    h.perm = (const int*)perm.p;
+    h.iperm = (const int*)iperm.p;
+    h.spec = spec.p;
+    const size_t eb = elt(prec);
+    for (i64 b0 = 0; b0 < lines; b0 += cb) {
+      const i64 nb = std::min<i64>(cb, lines - b0);
+      RC_TRY(chain.run(h, line_ptr(in, li, b0, eb), line_sub(li, b0), line_ptr(out, lo, b0, eb),
+                       line_sub(lo, b0), ci, co, scale, c.ws, c.s, nb));
+    }
     return GOFFT_OK;
   }
 };
@@ -974,12 +1210,18 @@ class BluesteinEngine : public Engine {
   bool fused = false;
   SubPlan sp;
   DevBuf stw;
-  std::unique_ptr<Engine> inner;
+  ConvChain chain;
 
   int init(unsigned flags) {
-    m = 1;
-    while (m < 2 * n - 1) m <<= 1;
     faithful = prec == 0 && (flags & GOFFT_FLAG_REFERENCE_TWIDDLES) && !(flags & GOFFT_FLAG_ACCURATE_TWIDDLES);
+    // The reference pads to a power of two (bluestein.go:29-32); that length
+    // is kept in reference mode, otherwise any smooth m >= 2n-1 serves.
+    if (faithful) {
+      m = 1;
+      while (m < 2 * n - 1) m <<= 1;
+    } else {
+      m = choose_conv_len(2 * n - 1, prec);
+    }
     if (faithful) {
       std::vector<dcx> wF, wI, bF, bI;
       reference_bluestein_tables(n, m, wF, wI, bF, bI);
@@ -1007,12 +1249,12 @@ class BluesteinEngine : public Engine {
     DevInfo dev;
     RC_TRY(current_device(&dev));
     const i64 pts = dev.lds_bytes / (i64)elt(prec);
-    const i64 target = std::min<i64>(pts, std::max<i64>(m, prec == 0 ? 4096 : 8192));
+    const i64 target = std::min<i64>(pts, std::max<i64>(pitch(m), soft_pts(prec)));
     int C = 0;
     if (faithful) {
-      if (m <= std::min<i64>(pts, 4096)) {
+      if (pitch(m) <= std::min<i64>(pts, soft_pts(prec))) {
         C = 1;
-        while (C * 2 <= 16 && (i64)C * 2 * m <= 4096) C *= 2;
+        while (C * 2 <= 16 && (i64)C * 2 * pitch(m) <= std::min<i64>(pts, soft_pts(prec))) C *= 2;
         while (C >= 1 && !make_sub_r2((int)m, C, 1, sp)) C /= 2;
         if (C >= 1) {
           fused = true;
@@ -1020,34 +1262,29 @@ class BluesteinEngine : public Engine {
           return GOFFT_OK;
         }
       }
-      auto e = std::make_unique<StockhamEngine>();
-      e->prec = prec;
-      e->n = m;
-      e->batch = batch;
-      RC_TRY(e->init_recurrence());
-      inner = std::move(e);
-      return GOFFT_OK;
+      cb = chunk_lines(2 * m * (i64)elt(prec), batch);
+      return chain.init(m, cb, prec, true);
     }
-    if (m <= pts && pick_cols((int)m, 16, target, sp, &C)) {
+    if (pitch(m) <= pts && pick_cols((int)m, 16, target, sp, &C)) {
       fused = true;
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
-      int rc = 0;
-      inner = make_engine(m, batch, prec, flags, &rc);
-      if (!inner) return rc;
+      cb = chunk_lines(2 * m * (i64)elt(prec), batch);
+      RC_TRY(chain.init(m, cb, prec, false));
     }
     return GOFFT_OK;
   }
+  i64 cb = 1;  // lines per chunk (multi-kernel path)
 
-  size_t ws_bytes() const override {
-    if (fused) return 0;
-    return 2 * (size_t)m * batch * elt(prec) + inner->ws_bytes();
+  size_t ws_bytes() const override { return fused ? 0 : chain.ws_bytes(); }
+  int launches() const override {
+    return fused ? 1 : (int)(chain.launches() * ((batch + cb - 1) / cb));
   }
-  int launches() const override { return fused ? 1 : 3 + 2 * inner->launches(); }
 
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
-          const Ctx& c) const override {
-    if (batch == 0) return GOFFT_OK;
+          const Ctx& c, i64 nl) const override {
+    const i64 lines = nl < 0 ? batch : nl;
+    if (lines == 0) return GOFFT_OK;
     // An inverse (ci and co) in reference mode runs the reference's inverse
     // (bluestein.go:67-95 with the inverse chirp and spectrum) instead of
     // conjugating a forward transform.
@@ -1064,7 +1301,7 @@ class BluesteinEngine : public Engine {
       a.chirp = w;
       a.in_off = li.off;
       a.out_off = lo.off;
-      a.nlines = batch;
+      a.nlines = lines;
       a.in_stride = li.stride;
       a.out_stride = lo.stride;
       a.n = (int)n;
@@ -1076,16 +1313,18 @@ class BluesteinEngine : public Engine {
       HIP_TRY(launch_conv(prec, kMaxP, a, sp.threads, c.s));
       return GOFFT_OK;
     }
-    char* A = c.ws;
-    char* B = c.ws + (size_t)m * batch * elt(prec);
-    Ctx ic{c.s, B + (size_t)m * batch * elt(prec)};
-    LineIO cont{nullptr, m};
-    HIP_TRY(launch_chirp_in(prec, in, A, w, n, m, batch, li.stride, m, ci, li.off, c.s));
-    RC_TRY(inner->run(A, cont, B, cont, 0, 0, 1.0, ic));
-    HIP_TRY(launch_cmul_table(prec, B, A, b, m, batch, m, m, 0, 1, 1.0, nullptr, nullptr, c.s));
-    RC_TRY(inner->run(A, cont, B, cont, 0, 1, 1.0, ic));
-    HIP_TRY(launch_cmul_table(prec, B, out, w, n, batch, m, lo.stride, 0, co, scale, nullptr, lo.off,
-                              c.s));
+    ConvChain::Hooks h;
+    h.in_hook = 1;
+    h.out_hook = 1;
+    h.nvalid = n;
+    h.w = w;
+    h.spec = b;
+    const size_t eb = elt(prec);
+    for (i64 b0 = 0; b0 < lines; b0 += cb) {
+      const i64 nb = std::min<i64>(cb, lines - b0);
+      RC_TRY(chain.run(h, line_ptr(in, li, b0, eb), line_sub(li, b0), line_ptr(out, lo, b0, eb),
+                       line_sub(lo, b0), ci, co, scale, c.ws, c.s, nb));
+    }
     return GOFFT_OK;
   }
 };
@@ -1148,10 +1387,10 @@ class RealEngine {
       DevInfo dev;
       RC_TRY(current_device(&dev));
       const i64 pts = dev.lds_bytes / (i64)elt(prec);
-      i64 target = std::min<i64>(pts, prec == 0 ? 4096 : 8192);
+      const i64 budget = std::min<i64>(pts, std::max<i64>(pitch(m), soft_pts(prec)));
       int C = 0;
-      if (m <= target && factors_are_small(m) &&
-          pick_cols((int)m, (int)std::min<i64>(kMaxCols, std::max<i64>(1, batch)), std::max<i64>(m, target), sp, &C)) {
+      if (pitch(m) <= pts && factors_are_small(m) &&
+          pick_cols((int)m, (int)std::min<i64>(kMaxCols, std::max<i64>(1, batch)), budget, sp, &C)) {
         single = true;
         RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
         return GOFFT_OK;

@@ -48,7 +48,7 @@ class _Plan:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h and L._lib is not None:
+        if h and L is not None and L._lib is not None:
             L._lib.gofft_plan_destroy(h)
             self._h = ctypes.c_void_p()
 
