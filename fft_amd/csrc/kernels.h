@@ -56,7 +56,20 @@ struct ColMap {
   long long g;
   int rowmajor;  // 1: walk each column contiguously (g == 1); 0: walk across columns
   int pad;
+  // Division magic for D0..D2: q = umul64hi(n << 8, M) = floor(n / D) exactly
+  // for n, D < 2^28 (M = ceil(2^56 / D)); M = 0 selects plain division.
+  unsigned long long M0, M1, M2;
 };
+
+inline void colmap_magic(ColMap& m, long long max_cols) {
+  auto mk = [&](long long d) -> unsigned long long {
+    if (d <= 0 || d >= (1ll << 28) || max_cols >= (1ll << 28)) return 0;
+    return ((1ull << 56) + (unsigned long long)d - 1) / (unsigned long long)d;
+  };
+  m.M0 = mk(m.D0);
+  m.M1 = mk(m.D1);
+  m.M2 = mk(m.D2);
+}
 
 // Sub-FFT description shared by all kernels.
 struct SubFFT {

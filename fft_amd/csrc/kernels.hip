@@ -263,15 +263,24 @@ __device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw, int cdig = 0) {
 
 // Column g's digits: returns the in-line part sum(d_i * a_i, i < 3); *line
 // gets the line part (offset table entry or d3 * a3), *d3 the line index.
+__device__ __forceinline__ unsigned long long udiv(unsigned long long n, long long d,
+                                                   unsigned long long M) {
+  return M ? __umul64hi(n << 8, M) : n / (unsigned long long)d;
+}
+
 __device__ __forceinline__ long long col_digits(const ColMap& m, long long g, long long* dtw,
                                                int tw_digit, const long long* off, long long* line,
                                                long long* d3out) {
-  long long d0 = g % m.D0;
-  long long t = g / m.D0;
-  long long d1 = t % m.D1;
-  t /= m.D1;
-  long long d2 = t % m.D2;
-  long long d3 = t / m.D2;
+  unsigned long long t = (unsigned long long)g;
+  unsigned long long q = udiv(t, m.D0, m.M0);
+  const long long d0 = (long long)(t - q * (unsigned long long)m.D0);
+  t = q;
+  q = udiv(t, m.D1, m.M1);
+  const long long d1 = (long long)(t - q * (unsigned long long)m.D1);
+  t = q;
+  q = udiv(t, m.D2, m.M2);
+  const long long d2 = (long long)(t - q * (unsigned long long)m.D2);
+  const long long d3 = (long long)q;
   if (dtw) *dtw = tw_digit == 0 ? d0 : d1;
   if (line) *line = off ? off[d3] : d3 * m.a3;
   if (d3out) *d3out = d3;
@@ -282,6 +291,10 @@ __device__ __forceinline__ double2 pass_twiddle(const double2* hi, const double2
                                                 int shift) {
   double2 a = hi[e >> shift];
   double2 b = lo[e & ((1ll << shift) - 1)];
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ double2 dmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
@@ -371,6 +384,18 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
       }
     }
   }
+  // Pass twiddles W_M^(r*k): twiddled passes walk across columns, so this
+  // thread's column is fixed (c = tid mod C) and r steps by tpc; one base and
+  // one step lookup, then an FP64 running product.
+  double2 tw_w = make_double2(1.0, 0.0), tw_step = make_double2(1.0, 0.0);
+  if (a.tw_M) {
+    const long long k = s_tk[tid & (C - 1)];
+    const long long r0 = tid >> f.log2C;
+    if (k) {
+      tw_w = pass_twiddle(a.ptw_hi, a.ptw_lo, r0 * k, a.tw_shift);
+      tw_step = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)f.tpc * k, a.tw_shift);
+    }
+  }
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
     const int fi = tid + u * nthr;
@@ -385,8 +410,8 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
       }
       T2 v = buf[u];
       if (a.tw_M) {
-        const long long e = (long long)r * s_tk[c];
-        if (e) v = cmul(v, from_d2<T2>(pass_twiddle(a.ptw_hi, a.ptw_lo, e, a.tw_shift)));
+        v = cmul(v, from_d2<T2>(tw_w));
+        tw_w = dmul2(tw_w, tw_step);
       }
       if (a.conj_in) v.y = -v.y;
       if (a.in_hook == 1 && s_ib[c] >= 0) {

@@ -589,13 +589,14 @@ class Engine {
 };
 
 // Lines of a multi-kernel transform are processed in chunks whose
-// intermediate buffers stay within this many bytes each, so that passes after
-// the first read from the 256 MB Infinity Cache rather than HBM.
+// intermediate buffers stay within this many bytes each (GOFFT_CHUNK_BYTES).
+// Measured on MI355X, small chunks lose more to partly filled launches than
+// they gain from cache residency, so the default only bounds workspace size.
 i64 chunk_bytes() {
   static i64 v = [] {
     const char* e = std::getenv("GOFFT_CHUNK_BYTES");
     i64 x = e ? std::atoll(e) : 0;
-    return x > 0 ? x : (i64)64 << 20;
+    return x > 0 ? x : (i64)2 << 30;
   }();
   return v;
 }
@@ -740,6 +741,8 @@ class StockhamEngine : public Engine {
       p.om.a2 = Nsp * R;
       p.om.g = Nsp;
       p.om.rowmajor = p.om.g == 1;
+      colmap_magic(p.im, batch * (n / R));
+      colmap_magic(p.om, batch * (n / R));
       p.stw_off = 0;
       p.tw_M = 0;
       p.tw_digit = 1;
@@ -831,6 +834,8 @@ class StockhamEngine : public Engine {
       p.om.a2 = Ns * R * st;
       p.om.g = Ns * st;
       p.om.rowmajor = p.om.g == 1;
+      colmap_magic(p.im, batch * inner * (n / R));
+      colmap_magic(p.om, batch * inner * (n / R));
       p.stw_off = stw_h.size();
       stw_h.insert(stw_h.end(), p.sp.tw.begin(), p.sp.tw.end());
       if (Ns > 1) {
@@ -841,6 +846,7 @@ class StockhamEngine : public Engine {
         p.tw_M = M;
         p.tw_shift = sh;
         p.tw_digit = 1;
+        if (p.im.rowmajor) return fail(GOFFT_ERR_UNSUPPORTED, "twiddled pass must walk across columns");
         p.ptw_hi = ptw_h.size();
         for (i64 h = 0; h < nh; h++) ptw_h.push_back(root(h * S, M));
         p.ptw_lo = ptw_h.size();
@@ -979,6 +985,17 @@ struct ConvChain {
   size_t ws_bytes() const { return 2 * (size_t)m * cb * elt(prec) + (size_t)cb * elt(prec); }
   int launches() const { return 2 * fwd.P() - 1; }
 
+  // The mid kernel reads the spectrum at forward output position j + r*Ns
+  // for column j and point r; storing it as spec[j*R + r] makes a column's
+  // reads contiguous.
+  int upload_spec(const std::vector<cld>& b, DevBuf& out) const {
+    const i64 R = fwd.passes.back().R, Ns = m / R;
+    std::vector<cld> t(m);
+    for (i64 j = 0; j < Ns; j++)
+      for (i64 r = 0; r < R; r++) t[j * R + r] = b[j + r * Ns];
+    return upload_complex(out, t, prec);
+  }
+
   void fill(PassArgs& a, const StockhamEngine& e, const Pass& p, i64 nl) const {
     a.stw = (const char*)e.stw.p + p.stw_off * elt(prec);
     a.ptw_hi = p.tw_M ? (const double2*)e.ptw.p + p.ptw_hi : nullptr;
@@ -1010,7 +1027,11 @@ struct ConvChain {
         const Pass& pf = fwd.passes[P - 1];
         const Pass& pi = inv.passes[0];
         fill(a, fwd, pf, nl);
-        a.mm = pf.om;
+        a.mm = pf.om;  // column j, point r -> spectrum entry j*R + r (see upload_spec)
+        a.mm.a0 = 0;
+        a.mm.a1 = pf.R;
+        a.mm.a2 = 0;
+        a.mm.g = 1;
         a.om = pi.om;
         a.mid_spec = h.spec;
         a.f2 = pi.sp.f;
@@ -1057,29 +1078,40 @@ struct ConvChain {
 };
 
 // Convolution length for Rader's linear path and for Bluestein outside the
-// reference-twiddle mode: a 7-smooth length in [lo, ceil(1.4 lo)] that a single
-// LDS pass can hold if one exists, else the cheapest by the reference's cost
-// model (plan.go:192-259).
+// reference-twiddle mode: a 7-smooth length in [lo, ceil(1.4 lo)] (the window
+// of plan.go:241-259). A length that one LDS
+// pass holds wins outright; otherwise the cost is the HBM passes of the
+// Stockham decomposition, each weighted by its LDS stage count (measured on
+// MI355X: radix-16 passes run about 40% faster per byte than passes of four
+// small radices).
 i64 choose_conv_len(i64 lo, int prec) {
   DevInfo dev;
   if (current_device(&dev) != GOFFT_OK) dev.lds_bytes = 65536;
   const i64 pts = dev.lds_bytes / (i64)elt(prec);
+  const int cmp = prec == 0 ? 8 : 16;
+  const i64 rmax = std::min<i64>(pts / cmp * 16 / 17, 1024);
   i64 hi = lo + (lo * 4 + 9) / 10;
   i64 best = -1;
   double bestc = 0;
   int bestfit = 0;
   for (i64 m = lo; m <= hi; m++) {
     i64 x = m;
-    double cost = 0;
-    while (x % 4 == 0) { cost += 2.0; x /= 4; }
-    while (x % 2 == 0) { cost += 2.0; x /= 2; }
-    while (x % 3 == 0) { cost += 2.0; x /= 3; }
-    while (x % 5 == 0) { cost += 2.8; x /= 5; }
-    while (x % 7 == 0) { cost += 2.6; x /= 7; }
+    for (int p : {2, 3, 5, 7})
+      while (x % p == 0) x /= p;
     if (x != 1) continue;
-    cost *= (double)m;
     SubPlan tmp;
     int fit = (pitch(m) <= pts && make_sub((int)m, 1, tmp)) ? 1 : 0;
+    double cost;
+    if (fit) {
+      cost = (double)m * (1.0 + 0.25 * (double)radix_schedule((int)m).size());
+    } else {
+      std::vector<i64> lens;
+      bool ok = false;
+      for (int P = 2; P <= 6 && !ok; P++) ok = factor_passes(m, rmax, P, lens);
+      if (!ok) continue;
+      cost = 0;
+      for (i64 R : lens) cost += (double)m * (1.0 + 0.25 * (double)radix_schedule((int)R).size());
+    }
     if (best < 0 || fit > bestfit || (fit == bestfit && cost < bestc)) {
       best = m;
       bestc = cost;
@@ -1131,7 +1163,7 @@ class RaderEngine : public Engine {
     for (i64 m = 0; m < lim; m++) b[m] = root(ph[(q - (m % q)) % q], n);
     host_fft(b);
     for (auto& v : b) v /= (long double)cl;
-    RC_TRY(upload_complex(spec, b, prec));
+    RC_TRY(upload_complex(spec, b, prec));  // natural order (fused kernel)
     DevInfo dev;
     RC_TRY(current_device(&dev));
     const i64 pts = dev.lds_bytes / (i64)elt(prec);
@@ -1143,6 +1175,7 @@ class RaderEngine : public Engine {
     } else {
       cb = chunk_lines(2 * cl * (i64)elt(prec), batch);
       RC_TRY(chain.init(cl, cb, prec, false));
+      RC_TRY(chain.upload_spec(b, spec));
     }
     return GOFFT_OK;
   }
@@ -1222,14 +1255,21 @@ class BluesteinEngine : public Engine {
     } else {
       m = choose_conv_len(2 * n - 1, prec);
     }
+    std::vector<cld> hF, hI;  // kernel spectra (1/m folded in), natural order
     if (faithful) {
       std::vector<dcx> wF, wI, bF, bI;
       reference_bluestein_tables(n, m, wF, wI, bF, bI);
       const double invm = 1.0 / (double)m;  // exact: m is a power of two
       RC_TRY(upload_dcx(chirp, wF, prec));
       RC_TRY(upload_dcx(chirpI, wI, prec));
-      RC_TRY(upload_dcx(spec, bF, prec, invm));
-      RC_TRY(upload_dcx(specI, bI, prec, invm));
+      hF.resize(m);
+      hI.resize(m);
+      for (i64 i = 0; i < m; i++) {
+        hF[i] = cld(bF[i].re * invm, bF[i].im * invm);
+        hI[i] = cld(bI[i].re * invm, bI[i].im * invm);
+      }
+      RC_TRY(upload_complex(spec, hF, prec));
+      RC_TRY(upload_complex(specI, hI, prec));
     } else {
       std::vector<cld> w(n), b(m, cld(0, 0));
       for (i64 j = 0; j < n; j++) {
@@ -1245,6 +1285,7 @@ class BluesteinEngine : public Engine {
       for (auto& v : b) v /= (long double)m;
       RC_TRY(upload_complex(chirp, w, prec));
       RC_TRY(upload_complex(spec, b, prec));
+      hF = std::move(b);
     }
     DevInfo dev;
     RC_TRY(current_device(&dev));
@@ -1263,7 +1304,9 @@ class BluesteinEngine : public Engine {
         }
       }
       cb = chunk_lines(2 * m * (i64)elt(prec), batch);
-      return chain.init(m, cb, prec, true);
+      RC_TRY(chain.init(m, cb, prec, true));
+      RC_TRY(chain.upload_spec(hF, spec));
+      return chain.upload_spec(hI, specI);
     }
     if (pitch(m) <= pts && pick_cols((int)m, 16, target, sp, &C)) {
       fused = true;
@@ -1271,6 +1314,7 @@ class BluesteinEngine : public Engine {
     } else {
       cb = chunk_lines(2 * m * (i64)elt(prec), batch);
       RC_TRY(chain.init(m, cb, prec, false));
+      RC_TRY(chain.upload_spec(hF, spec));
     }
     return GOFFT_OK;
   }
