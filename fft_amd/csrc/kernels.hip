@@ -238,6 +238,66 @@ __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage&
   __syncthreads();
 }
 
+constexpr int cbrev(int x, int bits) {
+  int r = 0;
+  for (int b = 0; b < bits; ++b) r = (r << 1) | ((x >> b) & 1);
+  return r;
+}
+constexpr int clog2(int x) { return x <= 1 ? 0 : 1 + clog2(x / 2); }
+
+// TT = 2^t consecutive radix-2 layers of the reference's iterative transform
+// (kernels.Radix2), done in registers with the reference's per-layer twiddle
+// tables. Layer u pairs registers (i, i + TT/2^(u+1)) in place, using the
+// table of layer length L = Nsg*ns*2^(u+1) at entry (k + v*ns)*Nsg + cdig,
+// v = bitrev_u(i >> (t-u)); register i ends as output bitrev_t(i). This is
+// the same set of butterflies, operands and twiddles as t separate layers.
+template <typename T2, int TT, int MAXP>
+__device__ __forceinline__ void lds_stage_rec(T2* lds, const SubFFT& f, const Stage& s, const T2* stw,
+                                              int c, int lt, int cdig) {
+  constexpr int t = clog2(TT);
+  constexpr int NB = MAXP / TT > 0 ? MAXP / TT : 1;
+  T2* col = lds + c * f.Rp;
+  const int nb = s.nb, ns = s.ns, tpc = f.tpc, Nsg = s.kmul;
+  T2 v[NB][TT];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    const int j = lt + m * tpc;
+    if (j < nb) {
+#pragma unroll
+      for (int i = 0; i < TT; ++i) v[m][i] = col[lp(j + i * nb)];
+      const int q = (int)fastdiv_q((uint32_t)j, s.nsd);
+      const int k = j - q * ns;
+#pragma unroll
+      for (int u = 0; u < t; ++u) {
+        const int h = TT >> (u + 1);
+        const int off = ((s.tw_off + 1) << u) - 1;
+#pragma unroll
+        for (int i = 0; i < TT; ++i) {
+          if (i & h) continue;
+          const int vv = cbrev(i >> (t - u), u);
+          const T2 w = stw[off + (k + vv * ns) * Nsg + cdig];
+          const T2 b = cmul(v[m][i + h], w);
+          v[m][i + h] = csub(v[m][i], b);
+          v[m][i] = cadd(v[m][i], b);
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    const int j = lt + m * tpc;
+    if (j < nb) {
+      const int q = (int)fastdiv_q((uint32_t)j, s.nsd);
+      const int k = j - q * ns;
+      const int o = q * ns * TT + k;
+#pragma unroll
+      for (int i = 0; i < TT; ++i) col[lp(o + cbrev(i, t) * ns)] = v[m][i];
+    }
+  }
+  __syncthreads();
+}
+
 // Runs the whole R-point sub-FFT (forward) on every column in LDS. cdig is the
 // calling thread's column twiddle digit (only read by stages with cmul != 0).
 template <typename T2, int MAXP>
@@ -256,6 +316,10 @@ __device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw, int cdig = 0) {
       case 8: lds_stage<T2, 8, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       case 11: lds_stage<T2, 11, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       case 13: lds_stage<T2, 13, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 102: lds_stage_rec<T2, 2, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 104: lds_stage_rec<T2, 4, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 108: lds_stage_rec<T2, 8, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 116: lds_stage_rec<T2, 16, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       default: lds_stage<T2, 16, MAXP>(lds, f, s, stw, c, lt, cdig); break;
     }
   }

@@ -387,19 +387,32 @@ bool make_sub(int R, int C, SubPlan& sp) {
   return true;
 }
 
-// Radix-2-only schedule for one pass of the reference-twiddle transform:
-// stage s combines sub-transforms of 2^s points within a column; its twiddle
-// for local butterfly k of column digit c is the reference's layer table
-// w_L[c + k*Ns] with L = Ns * 2^(s+1), stored at offset L/2 - 1 of one table
-// holding every layer L = 2, 4, ..., n.
+// Schedule for one pass of the reference-twiddle transform: the pass's
+// radix-2 layers grouped four at a time into register stages (radix codes
+// 100 + 2^t, kernels.hip lds_stage_rec). A stage starting at local size ns
+// reads layer tables of the one table holding every layer L = 2, 4, ..., n
+// (layer L at offset L/2 - 1), indexed with the column digit c and the
+// pass's global Ns.
 bool make_sub_r2(int R, int C, i64 Ns, SubPlan& sp) {
   if (R < 2 || (R & (R - 1)) || C < 1 || C > kMaxCols || (C & (C - 1))) return false;
   const int q = ilog2(R);
-  if (q > kMaxStages) return false;
+  std::vector<int> ts;
+  for (int left = q; left > 0;) {
+    int t = std::min(left, 4);
+    ts.push_back(t);
+    left -= t;
+  }
+  if ((int)ts.size() > kMaxStages) return false;
   int T = 64;
   while (T < C) T <<= 1;
-  while ((i64)T * (kMaxP / 2) < (i64)C * R / 2 && T < 1024) T <<= 1;
-  if ((i64)T * (kMaxP / 2) < (i64)C * R / 2) return false;
+  while ((i64)T * kMaxP < (i64)C * R && T < 1024) T <<= 1;
+  if ((i64)T * kMaxP < (i64)C * R) return false;
+  const int tpc = T / C;
+  for (int t : ts) {
+    int nb = R >> t;
+    int cap = kMaxP >> t;
+    if ((nb + tpc - 1) / tpc > cap) return false;
+  }
   sp = SubPlan();
   sp.threads = T;
   SubFFT& f = sp.f;
@@ -407,19 +420,21 @@ bool make_sub_r2(int R, int C, i64 Ns, SubPlan& sp) {
   f.Rp = (int)pitch(R);
   f.C = C;
   f.log2C = ilog2(C);
-  f.tpc = T / C;
+  f.tpc = tpc;
   f.log2tpc = ilog2(f.tpc);
-  f.nstages = q;
+  f.nstages = (int)ts.size();
   f.Rd = fastdiv_make((uint32_t)R);
-  for (int st = 0; st < q; st++) {
-    Stage& g = f.st[st];
-    g.radix = 2;
-    g.ns = 1 << st;
-    g.nb = R / 2;
-    g.tw_off = (int)(Ns * ((i64)1 << st) - 1);
+  int ns = 1;
+  for (size_t i = 0; i < ts.size(); i++) {
+    Stage& g = f.st[i];
+    g.radix = 100 + (1 << ts[i]);
+    g.ns = ns;
+    g.nb = R >> ts[i];
+    g.tw_off = (int)(Ns * ns - 1);
     g.kmul = (int)Ns;
     g.cmul = 1;
-    g.nsd = fastdiv_make((uint32_t)g.ns);
+    g.nsd = fastdiv_make((uint32_t)ns);
+    ns <<= ts[i];
   }
   return true;
 }
