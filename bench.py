@@ -10,10 +10,23 @@ value is the complex64 (FP32) form's GFLOP/s with the reference's flop count
 5*N*log2(N) per transform; the plain complex128 (FP64) form is timed beside it.
 
 Prints ONE JSON line (the bench contract), with a `roofline` object for the
-dominant kernel and a `cpu_baseline` object (the C++ restatement of the
-reference's CPU algorithm timed on this host).
+dominant kernel (the Stockham pass kernel k_pass<float,16>) and a
+`cpu_baseline` object (the C++ restatement of the reference's CPU algorithm
+timed on this host).
+
+Roofline of the pass kernel: each length group of the step is also timed on
+its own, with the same warm-up and step count, through a plan of exactly that
+group; the groups whose launches are all pass-kernel launches give the
+kernel's time per step. The library reports each launch's data bytes (its
+input and output lines once each), so achieved = bytes / time. `traffic` (HBM
+bytes per launch from FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy run
+by the same command) is read from the committed counter summary when present.
+
+`python bench.py --summarize-trace STATS.csv` and `--summarize-pmc FETCH.csv
+WRITE.csv` print those summaries from rocprofv3's CSV output.
 """
 import argparse
+import csv
 import json
 import math
 import os
@@ -28,6 +41,9 @@ sys.path.insert(0, ROOT)
 LENGTHS = [1024, 4096, 65536, 6720, 1009, 10007, 20014]
 COUNT = 4096
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30  # one direction of the calibration copy
+PASS_KERNEL = "k_pass<float, 16>"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pass_f32_mixed4096_traffic.txt")
 
 
 def workload_lengths(seed=0):
@@ -60,14 +76,168 @@ def time_events(fn, steps, warmup, torch):
     return start.elapsed_time(stop) / steps
 
 
+def read_traffic():
+    """HBM bytes per pass-kernel launch from the committed counter summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def run_dtype(D, torch, dev, name, lengths, x64, steps, warmup):
+    tdtype, ndtype, elt = {"fp32": (torch.complex64, np.complex64, 8),
+                           "fp64": (torch.complex128, np.complex128, 16)}[name]
+    L = D.L
+    total = sum(lengths)
+    plan = D.ManyPlan(lengths, tdtype)
+    xin = torch.from_numpy(x64.astype(ndtype)).to(dev)
+    out = torch.empty_like(xin)
+    ms = time_events(lambda: plan.exec(xin, out=out), steps, warmup, torch)
+    work = {k: plan.kernel_work(k) for k in (L.KERNEL_PASS, L.KERNEL_CONV, L.KERNEL_REAL, L.KERNEL_AUX)}
+    res = {"ms": ms, "gflops": flops_of(lengths) / (ms * 1e6), "gbs": 2.0 * total * elt / (ms * 1e6),
+           "launches": plan.num_launches(), "work": work, "groups": {}}
+    del plan, xin, out
+    counts = {}
+    for n in lengths:
+        counts[n] = counts.get(n, 0) + 1
+    for n in sorted(counts):
+        cnt = counts[n]
+        gp = D.C2CPlan(n, cnt, tdtype)
+        gx = torch.from_numpy(x64[: n * cnt].astype(ndtype)).to(dev)
+        gy = torch.empty_like(gx)
+        gms = time_events(lambda: gp.exec(gx, out=gy), steps, warmup, torch)
+        gw = {k: gp.kernel_work(k) for k in (L.KERNEL_PASS, L.KERNEL_CONV, L.KERNEL_REAL, L.KERNEL_AUX)}
+        res["groups"][n] = {"count": cnt, "ms": gms, "launches": gp.num_launches(), "work": gw}
+        del gp, gx, gy
+    torch.cuda.empty_cache()
+    return res
+
+
+def pass_roofline(r, L):
+    """Achieved bandwidth of the pass kernel over the step's pass-only groups."""
+    ms = launches = nbytes = 0
+    for g in r["groups"].values():
+        w = g["work"]
+        pl, pb = w[L.KERNEL_PASS]
+        others = sum(w[k][0] for k in w if k != L.KERNEL_PASS)
+        if pl and not others:
+            ms += g["ms"]
+            launches += pl
+            nbytes += pb
+    step_launches = r["work"][L.KERNEL_PASS][0]
+    if launches != step_launches:
+        raise RuntimeError("pass-kernel launches of the groups (%d) differ from the step's (%d)"
+                           % (launches, step_launches))
+    avg_ms = ms / launches
+    achieved = nbytes / launches / (avg_ms * 1e6)
+    return {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
+        "kernel": PASS_KERNEL, "launches_per_step": launches,
+        "alg_bytes_per_launch": round(nbytes / launches), "avg_launch_ms": round(avg_ms, 5),
+        "kernel_ms_per_step": round(ms, 4),
+    }
+
+
+def copy_bandwidth(torch, dev, steps, warmup):
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=dev)
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    ms = time_events(lambda: b.copy_(a), steps, warmup, torch)
+    del a, b
+    torch.cuda.empty_cache()
+    return {"gbs": round(2.0 * COPY_BYTES / (ms * 1e6), 1), "bytes_each_way": COPY_BYTES, "ms": round(ms, 4)}
+
+
+def cpu_leg(lengths, x64):
+    """The oracle (C++ restatement of the reference's CPU path) over the whole
+    step, serial Plan.FFT per transform as the reference has no batch API."""
+    from oracle import oracle as O
+    O.lib()
+    xs = []
+    off = 0
+    for n in lengths:
+        xs.append(x64[off: off + n])
+        off += n
+    for n in set(lengths):
+        O.fft(xs[lengths.index(n)])  # plans built outside the timed region
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for v in xs:
+            O.fft(v)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return {"value": round(flops_of(lengths) / best / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "sample": "the whole step: all 4096 mixed-length transforms, complex128, one thread, serial "
+                      "Plan.FFT per transform (C++ restatement of go-fft's CPU algorithm, -O2 "
+                      "-ffp-contract=off), best of 3; host %s with %d logical cores"
+                      % (os.uname().machine, os.cpu_count() or 0),
+            "seconds": round(best, 3)}
+
+
+def summarize_trace(path):
+    """Text summary of rocprofv3 --stats kernel statistics (CSV)."""
+    rows = list(csv.DictReader(open(path)))
+    print("%-58s %7s %12s %12s %7s" % ("kernel", "calls", "total_ms", "avg_us", "pct"))
+    for r in rows:
+        name = r["Name"]
+        if len(name) > 58:
+            name = name[:55] + "..."
+        print("%-58s %7s %12.3f %12.2f %7.2f" % (name, r["Calls"], float(r["TotalDurationNs"]) / 1e6,
+                                                 float(r["AverageNs"]) / 1e3, float(r["Percentage"])))
+
+
+def _pmc_rows(path, counter):
+    return [r for r in csv.DictReader(open(path)) if r.get("Counter_Name", counter) == counter]
+
+
+def _mean(rows):
+    return sum(float(r["Counter_Value"]) for r in rows) / len(rows), len(rows)
+
+
+def summarize_pmc(fetch_csv, write_csv):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the pass kernel, each
+    scaled by what the counter reads for the 1 GiB device copy of the same run
+    (the copy dispatches with the largest grid)."""
+    out = {}
+    for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        rows = _pmc_rows(path, name)
+        copies = [r for r in rows if "copy" in r["Kernel_Name"].lower()]
+        if not copies:
+            raise SystemExit("no copy dispatch in %s" % path)
+        g = max(int(r["Grid_Size"]) for r in copies)
+        cal, ncal = _mean([r for r in copies if int(r["Grid_Size"]) == g])
+        kern = [r for r in rows if "k_pass<float" in r["Kernel_Name"]]
+        if not kern:
+            raise SystemExit("no pass-kernel dispatch in %s" % path)
+        raw, nk = _mean(kern)
+        factor = COPY_BYTES / (cal * 1024.0)
+        out[name] = (raw, nk, cal, ncal, factor, raw * 1024.0 * factor)
+    print("kernel: %s" % kern[0]["Kernel_Name"])
+    for name, (raw, nk, cal, ncal, factor, b) in out.items():
+        print("%s: %d pass-kernel dispatches, mean %.1f KiB raw; 1 GiB copy reads %.1f KiB "
+              "(%d dispatches) -> factor %.4f -> %.0f bytes per launch" % (name, nk, raw, cal, ncal, factor, b))
+    print("traffic_bytes_per_launch: %.0f" % (out["FETCH_SIZE"][5] + out["WRITE_SIZE"][5]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--summarize-trace", metavar="STATS_CSV")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     args = ap.parse_args()
+    if args.summarize_trace:
+        return summarize_trace(args.summarize_trace)
+    if args.summarize_pmc:
+        return summarize_pmc(*args.summarize_pmc)
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -83,80 +253,12 @@ def main():
     flops = flops_of(lengths)
     x64 = gaussian_complex(total, seed=0)
 
-    results = {}
-    for name, tdtype, ndtype in [("fp32", torch.complex64, np.complex64),
-                                 ("fp64", torch.complex128, np.complex128)]:
-        plan = D.ManyPlan(lengths, tdtype)
-        xin = torch.from_numpy(x64.astype(ndtype)).to(dev)
-        out = torch.empty_like(xin)
-        ms = time_events(lambda: plan.exec(xin, out=out), args.steps, args.warmup, torch)
-        elt = 8 if name == "fp32" else 16
-        alg_bytes = 2.0 * total * elt
-        # per-length groups, for the roofline of the dominant kernel
-        groups = {}
-        off = 0
-        offsets = {}
-        for n in lengths:
-            offsets.setdefault(n, []).append(off)
-            off += n
-        for n in sorted(set(lengths)):
-            cnt = len(offsets[n])
-            gp = D.C2CPlan(n, cnt, tdtype)
-            gx = torch.from_numpy(x64[: n * cnt].astype(ndtype)).to(dev)
-            gy = torch.empty_like(gx)
-            gms = time_events(lambda: gp.exec(gx, out=gy), max(5, args.steps // 2), 2, torch)
-            groups[n] = {"count": cnt, "ms": gms, "launches": gp.num_launches(),
-                         "alg_bytes": 2.0 * n * cnt * elt, "flops": cnt * 5.0 * n * math.log2(n)}
-            del gp, gx, gy
-        results[name] = {"ms": ms, "gflops": flops / (ms * 1e6), "gbs": alg_bytes / (ms * 1e6),
-                         "alg_bytes": alg_bytes, "groups": groups, "launches": plan.num_launches()}
-        del plan, xin, out
-        torch.cuda.empty_cache()
+    r32 = run_dtype(D, torch, dev, "fp32", lengths, x64, args.steps, args.warmup)
+    r64 = run_dtype(D, torch, dev, "fp64", lengths, x64, args.steps, args.warmup)
+    roofline = pass_roofline(r32, D.L)
+    copy = copy_bandwidth(torch, dev, args.steps, args.warmup)
+    cpu = None if args.no_cpu else cpu_leg(lengths, x64)
 
-    # Dominant kernel: the single-launch group with the largest share of the
-    # FP32 step (multi-launch groups are listed in `groups`).
-    g32 = results["fp32"]["groups"]
-    single = {n: g for n, g in g32.items() if g["launches"] == 1}
-    dom_n = max(single, key=lambda n: single[n]["ms"])
-    dom = single[dom_n]
-    achieved = dom["alg_bytes"] / (dom["ms"] * 1e6)
-    roofline = {
-        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "kernel": "k_pass<float,16> single-pass Stockham, length %d x %d transforms" % (dom_n, dom["count"]),
-        "alg_bytes_per_launch": dom["alg_bytes"], "ms_per_launch": round(dom["ms"], 5),
-    }
-
-    cpu = None
-    if not args.no_cpu:
-        from oracle import oracle as O
-        O.lib()
-        # Bounded sample: the first 512 transforms of the workload (about 1/8),
-        # serial like the reference (it has no batch API).
-        sample = lengths[:512]
-        off = 0
-        xs = []
-        for n in sample:
-            xs.append(x64[off: off + n])
-            off += n
-        for n in set(sample):
-            O.fft(xs[sample.index(n)])  # build plans outside the timed region
-        best = None
-        for _ in range(2):
-            t0 = time.perf_counter()
-            for v in xs:
-                O.fft(v)
-            dt = time.perf_counter() - t0
-            best = dt if best is None else min(best, dt)
-        cpu = {"value": round(flops_of(sample) / best / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
-               "kind": "port",
-               "sample": "first 512 of the 4096 mixed-length transforms, complex128, serial "
-                         "Plan.FFT per transform (C++ restatement of go-fft's CPU algorithm, "
-                         "-O2 -ffp-contract=off), best of 2; host %s, %d logical cores"
-                         % (os.uname().machine, os.cpu_count() or 0),
-               "seconds": round(best, 3)}
-
-    r32, r64 = results["fp32"], results["fp64"]
     line = {
         "metric": "FFT GFLOP/s (5N*log2N) & HBM GB/s: FP32 batched vs FP64 plain, 1 GPU",
         "value": round(r32["gflops"], 2),
@@ -183,6 +285,7 @@ def main():
         "launches_per_step": {"fp32": r32["launches"], "fp64": r64["launches"]},
         "groups_ms": {"fp32": {str(n): round(g["ms"], 4) for n, g in r32["groups"].items()},
                       "fp64": {str(n): round(g["ms"], 4) for n, g in r64["groups"].items()}},
+        "copy_1GiB": copy,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

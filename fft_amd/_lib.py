@@ -26,6 +26,7 @@ FLAG_REFERENCE_TWIDDLES = 1
 FLAG_ACCURATE_TWIDDLES = 2
 ND_C2C = 0
 ND_R2C = 1
+KERNEL_PASS, KERNEL_CONV, KERNEL_REAL, KERNEL_AUX = 0, 1, 2, 3
 
 _i64 = ctypes.c_int64
 _vp = ctypes.c_void_p
@@ -52,6 +53,8 @@ SIGNATURES = {
     "gofft_plan_out_elems": (_i64, [_vp, ctypes.c_int]),
     "gofft_plan_workspace_bytes": (_i64, [_vp]),
     "gofft_plan_num_launches": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gofft_plan_kernel_work": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_i64),
+                                              ctypes.POINTER(_i64)]),
     "gofft_exec_c2c": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "gofft_exec_r2c": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "gofft_exec_c2r": (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp]),

@@ -590,6 +590,17 @@ struct Ctx {
   char* ws;  // workspace for this engine (ws_bytes() bytes)
 };
 
+// Launches of one exec per kernel family (GOFFT_KERNEL_*) and the data bytes
+// they move: each launch's input and output lines once, tables excluded.
+struct Work {
+  i64 launches[GOFFT_KERNEL_COUNT] = {};
+  i64 bytes[GOFFT_KERNEL_COUNT] = {};
+  void add(int k, i64 l, i64 b) {
+    launches[k] += l;
+    bytes[k] += b;
+  }
+};
+
 class Engine {
  public:
   int prec = 0;
@@ -597,6 +608,8 @@ class Engine {
   virtual ~Engine() {}
   virtual size_t ws_bytes() const = 0;
   virtual int launches() const = 0;
+  // Adds the launches and data bytes of a run over `lines` lines.
+  virtual void work(Work& w, i64 lines) const = 0;
   // out = scale * conj?( FFT( conj?(in) ) ) per line, for the first `nl`
   // lines (all `batch` lines when nl < 0).
   virtual int run(const void* in, LineIO li, void* out, LineIO lo, int conj_in, int conj_out,
@@ -643,6 +656,9 @@ class TrivialEngine : public Engine {
   }
   size_t ws_bytes() const override { return 0; }
   int launches() const override { return n == 1 ? 1 : 0; }
+  void work(Work& w, i64 lines) const override {
+    if (n == 1 && lines > 0) w.add(GOFFT_KERNEL_AUX, 1, 2 * lines * (i64)elt(prec));
+  }
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
           const Ctx& c, i64 nl) const override {
     const i64 lines = nl < 0 ? batch : nl;
@@ -887,6 +903,11 @@ class StockhamEngine : public Engine {
     if (P() <= 1) return P();
     return (int)(P() * ((batch + chunk() - 1) / chunk()));
   }
+  void work(Work& w, i64 lines) const override {
+    if (lines <= 0 || P() == 0) return;
+    const i64 chunks = P() <= 1 ? 1 : (lines + chunk() - 1) / chunk();
+    w.add(GOFFT_KERNEL_PASS, P() * chunks, (i64)P() * 2 * n * inner * lines * (i64)elt(prec));
+  }
 
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
           const Ctx& c, i64 nl) const override {
@@ -999,6 +1020,13 @@ struct ConvChain {
 
   size_t ws_bytes() const { return 2 * (size_t)m * cb * elt(prec) + (size_t)cb * elt(prec); }
   int launches() const { return 2 * fwd.P() - 1; }
+  // One run over `lines` lines in chunks of cb: the first pass reads nin
+  // points of each line, the last writes nout.
+  void work(Work& w, i64 lines, i64 nin, i64 nout) const {
+    if (lines <= 0) return;
+    const i64 K = launches(), chunks = (lines + cb - 1) / cb;
+    w.add(GOFFT_KERNEL_PASS, K * chunks, lines * (i64)elt(prec) * (nin + 2 * m * (K - 1) + nout));
+  }
 
   // The mid kernel reads the spectrum at forward output position j + r*Ns
   // for column j and point r; storing it as spec[j*R + r] makes a column's
@@ -1200,6 +1228,11 @@ class RaderEngine : public Engine {
   int launches() const override {
     return fused ? 1 : (int)(chain.launches() * ((batch + cb - 1) / cb));
   }
+  void work(Work& w, i64 lines) const override {
+    if (lines <= 0) return;
+    if (fused) w.add(GOFFT_KERNEL_CONV, 1, 2 * n * lines * (i64)elt(prec));
+    else chain.work(w, lines, q, n);
+  }
 
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
           const Ctx& c, i64 nl) const override {
@@ -1339,6 +1372,11 @@ class BluesteinEngine : public Engine {
   int launches() const override {
     return fused ? 1 : (int)(chain.launches() * ((batch + cb - 1) / cb));
   }
+  void work(Work& w, i64 lines) const override {
+    if (lines <= 0) return;
+    if (fused) w.add(GOFFT_KERNEL_CONV, 1, 2 * n * lines * (i64)elt(prec));
+    else chain.work(w, lines, n, n);
+  }
 
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
           const Ctx& c, i64 nl) const override {
@@ -1472,6 +1510,25 @@ class RealEngine {
   int launches() const {
     if (single) return 1;
     return half->launches() + (even ? 1 : 2);
+  }
+  // One forward (mode 0 bins, 1 power) or inverse (bins read per line) run.
+  void work(Work& w, int inverse, int mode, i64 bins) const {
+    if (batch == 0 || n <= 0) return;
+    const i64 e = (i64)elt(prec), nb = n / 2 + 1;
+    const i64 rb = n * e / 2;  // bytes of one real line
+    const i64 sb = mode ? nb * e / 2 : nb * e;
+    const i64 ib = std::min<i64>(std::max<i64>(bins, 0), nb) * e;
+    if (single) {
+      w.add(GOFFT_KERNEL_REAL, 1, batch * (inverse ? ib + rb : rb + sb));
+      return;
+    }
+    half->work(w, batch);
+    if (even) {
+      w.add(GOFFT_KERNEL_AUX, 1, batch * (inverse ? ib + m * e : m * e + sb));
+      return;
+    }
+    w.add(GOFFT_KERNEL_AUX, 1, batch * (inverse ? ib + n * e : rb + n * e));
+    w.add(GOFFT_KERNEL_AUX, 1, batch * (inverse ? n * e + rb : nb * e + sb));
   }
 
   // mode 0: complex bins (stride ostride per line), mode 1: PSD with psd_scale.
@@ -1981,6 +2038,65 @@ int gofft_plan_num_launches(const gofft_plan* p, int direction) {
     case K_SPEC: return p->frames > 0 ? p->real->launches() + (p->spec_gather ? 1 : 0) : 0;
   }
   return -1;
+}
+
+int gofft_plan_kernel_work(const gofft_plan* p, int direction, int kernel, int64_t* launches,
+                           int64_t* bytes) {
+  if (!p || !launches || !bytes) return fail(GOFFT_ERR_INVALID_ARG, "null argument");
+  if (kernel < 0 || kernel >= GOFFT_KERNEL_COUNT) return fail(GOFFT_ERR_INVALID_ARG, "unknown kernel family");
+  if (direction != GOFFT_FORWARD && direction != GOFFT_INVERSE)
+    return fail(GOFFT_ERR_INVALID_ARG, "direction must be GOFFT_FORWARD or GOFFT_INVERSE");
+  const int inv = direction == GOFFT_INVERSE;
+  const i64 e = (i64)elt(p->prec);
+  Work w;
+  auto axes_work = [&](const std::vector<NDAxis>& axes, i64 total) {
+    bool any = false;
+    for (const auto& a : axes) {
+      if (a.n == 1 || !a.eng) continue;
+      any = true;
+      if (a.gather) {
+        const i64 elems = a.n * a.st * a.outer;
+        w.add(GOFFT_KERNEL_AUX, 2, 4 * elems * e);
+        a.eng->work(w, a.outer * a.st);
+      } else {
+        a.eng->work(w, a.outer);
+      }
+    }
+    if (!any && total > 0) w.add(GOFFT_KERNEL_AUX, 1, 2 * total * e);
+  };
+  switch (p->kind) {
+    case K_C2C:
+      if (p->n > 0) p->eng->work(w, p->batch);
+      break;
+    case K_MANY:
+      for (const auto& g : p->groups) g->eng->work(w, g->eng->batch);
+      break;
+    case K_R2C: p->real->work(w, inv, 0, p->n / 2 + 1); break;
+    case K_PSD: p->real->work(w, 0, 1, 0); break;
+    case K_SPEC:
+      if (p->frames > 0) {
+        if (p->spec_gather) w.add(GOFFT_KERNEL_AUX, 1, 2 * p->frames * p->seg * e / 2);
+        p->real->work(w, 0, 1, 0);
+      }
+      break;
+    case K_ND_C2C: axes_work(p->axes, p->nd_total * p->batch); break;
+    case K_ND_R2C: {
+      const i64 htotal = p->rows * p->rcols * p->batch;
+      if (p->batch == 0) break;
+      if (!inv) {
+        p->nd_row->work(w, 0, 0, p->rcols);
+        axes_work(p->axes, 0);
+      } else {
+        w.add(GOFFT_KERNEL_AUX, 1, 2 * htotal * e);
+        axes_work(p->axes, htotal);
+        p->nd_row->work(w, 1, 0, p->rcols);
+      }
+      break;
+    }
+  }
+  *launches = w.launches[kernel];
+  *bytes = w.bytes[kernel];
+  return GOFFT_OK;
 }
 
 int gofft_exec_c2c(const gofft_plan* p, const void* src, void* dst, int direction, void* stream) {

@@ -58,6 +58,12 @@ class _Plan:
     def num_launches(self, direction=0):
         return L.lib().gofft_plan_num_launches(self._h, direction)
 
+    def kernel_work(self, kernel, direction=0):
+        """(launches, data bytes) of kernel family `kernel` (L.KERNEL_*) in one exec."""
+        n, b = ctypes.c_int64(), ctypes.c_int64()
+        L.check(L.lib().gofft_plan_kernel_work(self._h, direction, kernel, ctypes.byref(n), ctypes.byref(b)))
+        return n.value, b.value
+
     def in_elems(self, direction=0):
         return L.lib().gofft_plan_in_elems(self._h, direction)
 

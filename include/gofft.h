@@ -122,6 +122,22 @@ int64_t gofft_plan_workspace_bytes(const gofft_plan* plan);
 /* Number of kernel launches one forward execution enqueues. */
 int gofft_plan_num_launches(const gofft_plan* plan, int direction);
 
+/* Kernel families counted by gofft_plan_kernel_work. */
+enum {
+  GOFFT_KERNEL_PASS = 0, /* Stockham pass kernel (one pass of a transform or convolution) */
+  GOFFT_KERNEL_CONV = 1, /* fused single-kernel Rader / Bluestein transform */
+  GOFFT_KERNEL_REAL = 2, /* fused single-kernel real-input transform */
+  GOFFT_KERNEL_AUX = 3,  /* elementwise kernels (copies, packing, scaling) */
+  GOFFT_KERNEL_COUNT = 4
+};
+
+/* Launches of family `kernel` in one exec of the plan in `direction`, and the
+   data bytes those launches read and write: each launch's input lines and
+   output lines once, tables excluded. For a measurement's roofline (bytes over
+   the launches' measured time). No reference counterpart. */
+int gofft_plan_kernel_work(const gofft_plan* p, int direction, int kernel, int64_t* launches,
+                           int64_t* bytes);
+
 /* --- execution on device memory -------------------------------------------- */
 
 /* c2c / c2c_many / nd-C2C plans. */
