@@ -180,6 +180,150 @@ def cpu_leg(lengths, x64):
             "seconds": round(best, 3)}
 
 
+# ---------------------------------------------------------------------------
+# The other workloads (one JSON line each with --workload NAME); the default
+# run measures the mixed-length batch above.
+
+
+def _line(name, ms, flops, nbytes, dtype, config, extra=None):
+    d = {"workload": name, "ms": round(ms, 4), "gflops": round(flops / (ms * 1e6), 2),
+         "gbs": round(nbytes / (ms * 1e6), 1), "dtype": dtype, "config": config}
+    if extra:
+        d.update(extra)
+    print(json.dumps(d), flush=True)
+
+
+def _sinusoids(torch, dev, n, dtype, seed=0):
+    """Sum of 64 sinusoids (f ~ U[0, 0.5) cycles/sample, amplitude ~ U[0, 1],
+    phase 0) plus N(0, 0.1^2) noise, built on the GPU in FP64."""
+    rng = np.random.default_rng(seed)
+    f = rng.uniform(0, 0.5, 64)
+    a = rng.uniform(0, 1, 64)
+    t = torch.arange(n, dtype=torch.float64, device=dev)
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+    for fj, aj in zip(f, a):
+        x += aj * torch.sin((2.0 * math.pi * fj) * t)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    x += 0.1 * torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+    del t
+    return x.to(dtype)
+
+
+def workload_roundtrip4096(D, torch, dev, steps, warmup):
+    n, batch = 4096, 256
+    x = torch.from_numpy(gaussian_complex(n * batch, 0)).to(dev)
+    p = D.C2CPlan(n, batch, torch.complex128)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+
+    def step():
+        p.exec(x, out=y)
+        p.exec(y, out=z, inverse=True)
+    ms = time_events(step, steps, warmup, torch)
+    _line("roundtrip4096", ms, 2 * batch * 5.0 * n * 12, 4 * n * batch * 16, "c128",
+          "FFT + IFFT, N=4096, batch 256, complex128 (64 MiB in total, resident in the 256 MB MALL)")
+
+
+def workload_spectral(D, torch, dev, steps, warmup):
+    n, seg, ov = 1 << 26, 1024, 512
+    win = [0.5 - 0.5 * math.cos(2 * math.pi * k / (seg - 1)) for k in range(seg)]
+    for name, rt in (("f32", torch.float32), ("f64", torch.float64)):
+        x = _sinusoids(torch, dev, n, rt)
+        eb = 4 if rt == torch.float32 else 8
+        sp = D.SpectrogramPlan(n, seg, ov, win, 1.0, rt)
+        out = torch.empty(sp.frames * sp.bins, dtype=rt, device=dev)
+        ms = time_events(lambda: sp.exec(x, out=out), steps, warmup, torch)
+        _line("spectrogram", ms, sp.frames * 2.5 * seg * 10, (n + sp.frames * sp.bins) * eb, name,
+              "Spectrogram of 2^26 samples, Hann(1024), overlap 512: %d frames x 513 bins" % sp.frames)
+        del sp, out
+        pp = D.PSDPlan(n, 1, 1.0, rt)
+        out = torch.empty(n // 2 + 1, dtype=rt, device=dev)
+        ms = time_events(lambda: pp.exec(x, out=out), steps, warmup, torch)
+        _line("psd", ms, 2.5 * n * 26, (n + n // 2 + 1) * eb, name, "one-sided PSD of 2^26 samples")
+        del pp, out
+        m = 1 << 24
+        rp = D.R2CPlan(m, 1, rt)
+        xs = x[:m].contiguous()
+        X = rp.forward(xs)
+        ms = time_events(lambda: rp.forward(xs, out=X), steps, warmup, torch)
+        _line("rfft", ms, 2.5 * m * 24, m * eb + (m // 2 + 1) * 2 * eb, name, "RFFT of 2^24 samples")
+        back = torch.empty_like(xs)
+        ms = time_events(lambda: rp.inverse(X, out=back), steps, warmup, torch)
+        _line("irfft", ms, 2.5 * m * 24, m * eb + (m // 2 + 1) * 2 * eb, name, "IRFFT to 2^24 samples")
+        del rp, X, back, xs, x
+        torch.cuda.empty_cache()
+
+
+def workload_nd(D, torch, dev, steps, warmup):
+    L = D.L
+    for shape, dts in (((4096, 4096), (torch.complex64, torch.complex128)), ((256, 256, 256), (torch.complex64,))):
+        N = int(np.prod(shape))
+        for ct in dts:
+            g = torch.Generator(device=dev)
+            g.manual_seed(0)
+            x = torch.randn(N, dtype=ct, device=dev, generator=g)
+            p = D.NDPlan(shape, L.ND_C2C, 1, ct)
+            y = torch.empty_like(x)
+            ms = time_events(lambda: p.exec(x, out=y), steps, warmup, torch)
+            eb = 8 if ct == torch.complex64 else 16
+            _line("fft2" if len(shape) == 2 else "fftn", ms, 5.0 * N * math.log2(N), 2 * N * eb,
+                  "c64" if eb == 8 else "c128", "x".join(map(str, shape)) + " complex, re/im N(0,1)")
+            del p, x, y
+            torch.cuda.empty_cache()
+    rows = cols = 8192
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    x = torch.rand(rows * cols, dtype=torch.float32, device=dev, generator=g)
+    p = D.NDPlan((rows, cols), L.ND_R2C, 1, torch.complex64)
+    X = p.rfft2(x)
+    ms = time_events(lambda: p.rfft2(x, out=X), steps, warmup, torch)
+    nb = rows * cols * 4 + rows * (cols // 2 + 1) * 8
+    _line("rfft2", ms, 2.5 * rows * cols * 26, nb, "f32", "8192x8192 float32 U[0,1)")
+    back = torch.empty_like(x)
+    ms = time_events(lambda: p.irfft2(X, out=back), steps, warmup, torch)
+    _line("irfft2", ms, 2.5 * rows * cols * 26, nb, "f32", "8192x8192 -> 8192x4097 bins and back")
+
+
+def workload_blur(D, torch, dev, steps, warmup):
+    L = D.L
+    batch, rows, cols = 64, 1024, 1024
+    rc = cols // 2 + 1
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    x = torch.rand(batch * rows * cols, dtype=torch.float32, device=dev, generator=g)
+    fr = torch.fft.fftfreq(rows, device=dev, dtype=torch.float64)[:, None]
+    fc = torch.fft.rfftfreq(cols, device=dev, dtype=torch.float64)[None, :]
+    H = torch.exp(-2.0 * (math.pi * 3.0) ** 2 * (fr ** 2 + fc ** 2)).to(torch.float32).reshape(-1)
+    H = H.repeat(batch)
+    p = D.NDPlan((rows, cols), L.ND_R2C, batch, torch.complex64)
+    y = p.irfft2(p.rfft2(x) * H) + 0.01 * torch.randn(batch * rows * cols, device=dev, generator=g)
+    S = torch.empty(batch * rows * rc, dtype=torch.complex64, device=dev)
+    r = torch.empty_like(x)
+    gS = torch.empty_like(S)
+    gx = torch.empty_like(x)
+
+    def step():
+        p.rfft2(x, out=S)
+        S.mul_(H)
+        p.irfft2(S, out=r)
+        r.sub_(y)
+        r.mul_(2.0)
+        p.vjp(r, gS, direction=1)
+        gS.mul_(H)
+        p.vjp(gS, gx, direction=0)
+    ms = time_events(step, steps, warmup, torch)
+    per = 2.5 * rows * cols * 20 * batch
+    passb = batch * (rows * cols * 4 + rows * rc * 8)
+    _line("blur_grad", ms, 4 * per, 4 * passb, "f32",
+          "loss ||IRFFT2(RFFT2(x) H) - y||^2 and its gradient, 64 x 1024x1024 float32, Gaussian H sigma 3 px; "
+          "one step = RFFT2, IRFFT2 and the two adjoint transforms plus elementwise work")
+
+
+WORKLOADS = {"roundtrip4096": workload_roundtrip4096, "spectral": workload_spectral, "nd": workload_nd,
+             "blur": workload_blur}
+
+
 def summarize_trace(path):
     """Text summary of rocprofv3 --stats kernel statistics (CSV)."""
     rows = list(csv.DictReader(open(path)))
@@ -231,6 +375,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", choices=["mixed", "all"] + sorted(WORKLOADS), default="mixed",
+                    help="mixed: the headline line; the others print one line per transform")
     ap.add_argument("--summarize-trace", metavar="STATS_CSV")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     args = ap.parse_args()
@@ -248,6 +394,10 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.workload != "mixed":
+        for name in (sorted(WORKLOADS) if args.workload == "all" else [args.workload]):
+            WORKLOADS[name](D, torch, dev, args.steps, args.warmup)
+        return
     lengths = workload_lengths(0)
     total = sum(lengths)
     flops = flops_of(lengths)

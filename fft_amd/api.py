@@ -21,6 +21,7 @@ __all__ = [
     "Panic", "Plan", "RealPlan", "NewPlan", "NewRealPlan", "FFT", "IFFT", "FFTReal", "RFFT",
     "IRFFT", "FFTN", "IFFTN", "FFT2", "IFFT2", "RFFT2", "IRFFT2", "PSD", "Spectrogram", "Hann",
     "Hamming", "Blackman", "BlackmanHarris", "Bartlett", "FFTFreq", "RFFTFreq",
+    "FFTVJP", "IFFTVJP", "RFFTVJP", "IRFFTVJP", "FFTNVJP", "IFFTNVJP", "RFFT2VJP", "IRFFT2VJP",
 ]
 
 
@@ -260,6 +261,106 @@ def IRFFT2(data, shape, dtype=None):
     bins = min(len(data), rows * (cols // 2 + 1))
     X = _to_dev(data[:bins].astype(ct) if bins > 0 else np.zeros(1, dtype=ct))
     return _to_host(p.irfft2(X, bins=bins))
+
+
+# ---------------------------------------------------------------------------
+# Vector-Jacobian products (no reference counterpart). For a real loss L, the
+# gradient with respect to a complex array z is dL/dRe(z) + i dL/dIm(z).
+
+
+def _vjp_c2c(gy, inverse, dtype):
+    gy = np.asarray(gy)
+    dt = _ctype(dtype or (gy.dtype if gy.dtype.kind == "c" else np.complex128))
+    n = len(gy)
+    if n == 0:
+        return np.zeros(0, dtype=dt)
+    p = _c2c_plan(n, dt)
+    return _to_host(p.vjp(_to_dev(gy.astype(dt, copy=False)), inverse=inverse))
+
+
+def FFTVJP(gy, dtype=None):
+    """Gradient through y = FFT(x): n * IFFT(gy)."""
+    return _vjp_c2c(gy, False, dtype)
+
+
+def IFFTVJP(gy, dtype=None):
+    """Gradient through y = IFFT(x): FFT(gy) / n."""
+    return _vjp_c2c(gy, True, dtype)
+
+
+def RFFTVJP(gy, n, dtype=None):
+    """Gradient (real, length n) through y = RFFT(x) from the n/2+1 bins gy."""
+    gy = np.asarray(gy)
+    dt = _rtype(dtype or (np.float32 if gy.dtype == np.complex64 else np.float64))
+    if n <= 0:
+        return np.zeros(0, dtype=dt)
+    if len(gy) != n // 2 + 1:
+        raise Panic("fft: gradient length must be n/2+1")
+    p = _r2c_plan(n, dt)
+    return _to_host(p.vjp_forward(_to_dev(gy.astype(_ctype(dt), copy=False))))
+
+
+def IRFFTVJP(gx, dtype=None):
+    """Gradient (n/2+1 bins) through x = IRFFT(X, n) from the real gx of length n."""
+    gx = np.asarray(gx)
+    dt = _rtype(dtype or (gx.dtype if gx.dtype.kind == "f" else np.float64))
+    n = len(gx)
+    if n == 0:
+        return np.zeros(0, dtype=_ctype(dt))
+    p = _r2c_plan(n, dt)
+    return _to_host(p.vjp_inverse(_to_dev(gx.astype(dt, copy=False))))
+
+
+def _vjp_n(gy, shape, inverse, dtype):
+    gy = np.asarray(gy)
+    shape = [int(s) for s in shape]
+    _validate_shape(shape, len(gy))
+    dt = _ctype(dtype or (gy.dtype if gy.dtype.kind == "c" else np.complex128))
+    if len(gy) <= 1:
+        return gy.astype(dt).copy()
+    p = D.NDPlan(shape, L.ND_C2C, 1, torch.complex64 if dt == np.complex64 else torch.complex128)
+    return _to_host(p.vjp_c2c(_to_dev(gy.astype(dt, copy=False)), inverse=inverse))
+
+
+def FFTNVJP(gy, shape, dtype=None):
+    """Gradient through y = FFTN(x, shape)."""
+    return _vjp_n(gy, shape, False, dtype)
+
+
+def IFFTNVJP(gy, shape, dtype=None):
+    """Gradient through y = IFFTN(x, shape)."""
+    return _vjp_n(gy, shape, True, dtype)
+
+
+def _r2c2_shape(shape):
+    rows, cols = int(shape[0]), int(shape[1])
+    if rows <= 0 or cols <= 0:
+        raise Panic("fft: shape lengths must be positive")
+    return rows, cols
+
+
+def RFFT2VJP(gy, shape, dtype=None):
+    """Gradient (real rows*cols) through y = RFFT2(x, shape) from rows*(cols/2+1) bins."""
+    rows, cols = _r2c2_shape(shape)
+    gy = np.asarray(gy)
+    if len(gy) != rows * (cols // 2 + 1):
+        raise Panic("fft: gradient length must be rows*(cols/2+1)")
+    dt = _rtype(dtype or (np.float32 if gy.dtype == np.complex64 else np.float64))
+    p = D.NDPlan([rows, cols], L.ND_R2C, 1, torch.float32 if dt == np.float32 else torch.float64)
+    out = torch.empty(rows * cols, dtype=D.REAL_OF[p.code], device=_DEV)
+    return _to_host(p.vjp(_to_dev(gy.astype(_ctype(dt), copy=False)), out, direction=0))
+
+
+def IRFFT2VJP(gx, shape, dtype=None):
+    """Gradient (rows*(cols/2+1) bins) through x = IRFFT2(X, shape) from the real gx."""
+    rows, cols = _r2c2_shape(shape)
+    gx = np.asarray(gx)
+    if len(gx) != rows * cols:
+        raise Panic("fft: shape product does not match len(data)")
+    dt = _rtype(dtype or (gx.dtype if gx.dtype.kind == "f" else np.float64))
+    p = D.NDPlan([rows, cols], L.ND_R2C, 1, torch.float32 if dt == np.float32 else torch.float64)
+    out = torch.empty(rows * (cols // 2 + 1), dtype=D.COMPLEX_OF[p.code], device=_DEV)
+    return _to_host(p.vjp(_to_dev(gx.astype(dt, copy=False)), out, direction=1))
 
 
 def PSD(x, d=1.0, dtype=None):

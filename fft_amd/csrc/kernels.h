@@ -133,9 +133,10 @@ struct RealFwdArgs {
   long long nlines;
   long long in_stride;      // real samples between consecutive lines
   long long out_stride;     // output elements between consecutive lines
-  int mode;                 // 0 complex bins, 1 PSD
+  int mode;                 // 0 complex bins, 1 PSD, 2 weighted complex bins
   int odd_n;                // unused for even n (reserved)
   double psd_scale;         // d / n
+  double w_edge, w_mid;     // mode 2: DC/Nyquist bins times w_edge, others times w_mid
   SubFFT f;
 };
 
@@ -150,6 +151,7 @@ struct RealInvArgs {
   long long in_bins;        // bins present per line (missing ones read as 0)
   long long out_stride;     // real samples between consecutive lines
   double scale;             // 1/m
+  double w_mid;             // weight of the bins other than DC and Nyquist (1: IRFFT)
   SubFFT f;
 };
 
@@ -200,23 +202,31 @@ hipError_t launch_rader_scatter(int prec, const void* x, const void* conv, void*
 hipError_t launch_chirp_in(int prec, const void* in, void* out, const void* w, long long n,
                            long long m, long long nlines, long long is, long long os, int conj_in,
                            const long long* ioff, hipStream_t s);
-// RFFT untangle of a packed m-point spectrum Z into m+1 bins (mode 1: PSD).
+// RFFT untangle of a packed m-point spectrum Z into m+1 bins (mode 1: PSD,
+// mode 2: bins weighted w_edge at DC and Nyquist, w_mid elsewhere).
 hipError_t launch_untangle(int prec, const void* Z, void* out, const void* utw, long long m,
                            long long nlines, long long zs, long long os, int mode,
-                           double psd_scale, hipStream_t s);
-// IRFFT pre-pass: Hermitian half spectrum -> packed m-point spectrum Z.
+                           double psd_scale, double w_edge, double w_mid, hipStream_t s);
+// IRFFT pre-pass: Hermitian half spectrum -> packed m-point spectrum Z, the
+// bins other than DC and Nyquist weighted by w_mid (1 for the IRFFT).
 hipError_t launch_herm_pack(int prec, const void* X, void* Z, const void* utw, long long m,
                             long long nlines, long long xs, long long bins, long long zs,
-                            hipStream_t s);
+                            double w_mid, hipStream_t s);
+// out[l*os + k] = in[l*is + k] * w_mid, k < bins, except DC and (even n) k = n/2:
+// (re(in) * w_edge, 0).
+hipError_t launch_weight_bins(int prec, const void* in, void* out, long long bins, long long n,
+                              long long nlines, long long is, long long os, double w_edge,
+                              double w_mid, hipStream_t s);
 // Real lines -> complex lines (imaginary 0): out[l*os + i] = (in[l*is + i], 0), i < n.
 hipError_t launch_promote(int prec, const void* in, void* out, long long n, long long nlines,
                           long long is, long long os, hipStream_t s);
 // Complex line copy: out[l*os + k] = in[l*is + k], k < n.
 hipError_t launch_copy_lines(int prec, const void* in, void* out, long long n, long long nlines,
                              long long is, long long os, hipStream_t s);
-// Hermitian mirror of `bins` supplied bins into a full length-n spectrum.
+// Hermitian mirror of `bins` supplied bins into a full length-n spectrum,
+// the bins other than DC weighted by w_mid (1 for the IRFFT; n is odd).
 hipError_t launch_herm_full(int prec, const void* in, void* out, long long n, long long nlines,
-                            long long is, long long bins, long long os, hipStream_t s);
+                            long long is, long long bins, long long os, double w_mid, hipStream_t s);
 // Real part of complex lines, optionally scaled: out[l*os + i] = re(in[l*is + i]) * scale.
 hipError_t launch_take_real(int prec, const void* in, void* out, long long n, long long nlines,
                             long long is, long long os, double scale, hipStream_t s);

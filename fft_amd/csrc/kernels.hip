@@ -619,6 +619,10 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
     }
     if (a.mode == 0) {
       reinterpret_cast<T2*>(a.out)[l * a.out_stride + k] = X;
+    } else if (a.mode == 2) {
+      const bool edge = k == 0 || k == R;  // imaginary part exactly 0 at DC and Nyquist
+      reinterpret_cast<T2*>(a.out)[l * a.out_stride + k] =
+          edge ? mkc<T2>(X.x * (T)a.w_edge, 0) : cscale(X, (T)a.w_mid);
     } else {
       T p = (X.x * X.x + X.y * X.y) * (T)a.psd_scale;
       if (k != 0 && k != R) p *= (T)2;
@@ -651,6 +655,7 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
       const int r = fi - c * R;
       const long long l = l0 + c;
       if (l < a.nlines && r < a.in_bins) buf[u] = in[l * a.in_stride + r];
+      if (r > 0) buf[u] = cscale(buf[u], (T)a.w_mid);
     }
   }
 #pragma unroll
@@ -924,7 +929,8 @@ __global__ void k_chirp_in(const void* in_, void* out_, const void* w_, long lon
 
 template <typename T>
 __global__ void k_untangle(const void* Z_, void* out_, const void* utw_, long long m, long long nlines,
-                           long long zs, long long os, int mode, double psd_scale) {
+                           long long zs, long long os, int mode, double psd_scale, double w_edge,
+                           double w_mid) {
   typedef typename Vec2<T>::type T2;
   const T2* Zall = reinterpret_cast<const T2*>(Z_);
   const T2* utw = reinterpret_cast<const T2*>(utw_);
@@ -946,6 +952,9 @@ __global__ void k_untangle(const void* Z_, void* out_, const void* utw_, long lo
     }
     if (mode == 0) {
       reinterpret_cast<T2*>(out_)[l * os + k] = X;
+    } else if (mode == 2) {
+      const bool edge = k == 0 || k == m;  // imaginary part exactly 0 at DC and Nyquist
+      reinterpret_cast<T2*>(out_)[l * os + k] = edge ? mkc<T2>(X.x * (T)w_edge, 0) : cscale(X, (T)w_mid);
     } else {
       T p = (X.x * X.x + X.y * X.y) * (T)psd_scale;
       if (k != 0 && k != m) p *= (T)2;
@@ -956,7 +965,7 @@ __global__ void k_untangle(const void* Z_, void* out_, const void* utw_, long lo
 
 template <typename T>
 __global__ void k_herm_pack(const void* X_, void* Z_, const void* utw_, long long m, long long nlines,
-                            long long xs, long long bins, long long zs) {
+                            long long xs, long long bins, long long zs, double w_mid) {
   typedef typename Vec2<T>::type T2;
   const T2* Xall = reinterpret_cast<const T2*>(X_);
   const T2* utw = reinterpret_cast<const T2*>(utw_);
@@ -972,8 +981,8 @@ __global__ void k_herm_pack(const void* X_, void* Z_, const void* utw_, long lon
       T xm = m < bins ? X[m].x : (T)0;
       z = mkc<T2>((x0 + xm) * (T)0.5, (x0 - xm) * (T)0.5);
     } else {
-      T2 xk = k < bins ? X[k] : mkc<T2>(0, 0);
-      T2 xmk = (m - k) < bins ? X[m - k] : mkc<T2>(0, 0);
+      T2 xk = k < bins ? cscale(X[k], (T)w_mid) : mkc<T2>(0, 0);
+      T2 xmk = (m - k) < bins ? cscale(X[m - k], (T)w_mid) : mkc<T2>(0, 0);
       T xer = (xk.x + xmk.x) * (T)0.5, xei = (xk.y - xmk.y) * (T)0.5;
       T dr = (xk.x - xmk.x) * (T)0.5, di = (xk.y + xmk.y) * (T)0.5;
       T2 w = utw[k];
@@ -1016,7 +1025,7 @@ __global__ void k_copy_lines(const void* in_, void* out_, long long n, long long
 // (realplan.go:197-213).
 template <typename T>
 __global__ void k_herm_full(const void* in_, void* out_, long long n, long long nlines, long long is,
-                            long long bins, long long os) {
+                            long long bins, long long os, double w_mid) {
   typedef typename Vec2<T>::type T2;
   const T2* in = reinterpret_cast<const T2*>(in_);
   T2* out = reinterpret_cast<T2*>(out_);
@@ -1032,7 +1041,24 @@ __global__ void k_herm_full(const void* in_, void* out_, long long n, long long 
       long long kk = n - k;  // 0 < kk < n - kk
       if (kk < bins) v = cconj(in[l * is + kk]);
     }
+    if (k != 0) v = cscale(v, (T)w_mid);
     out[l * os + k] = v;
+  }
+}
+
+template <typename T>
+__global__ void k_weight_bins(const void* in_, void* out_, long long bins, long long n, long long nlines,
+                              long long is, long long os, double w_edge, double w_mid) {
+  typedef typename Vec2<T>::type T2;
+  const T2* in = reinterpret_cast<const T2*>(in_);
+  T2* out = reinterpret_cast<T2*>(out_);
+  long long total = bins * nlines;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long l = i / bins, k = i - l * bins;
+    const bool edge = k == 0 || (n % 2 == 0 && k == n / 2);
+    const T2 v = in[l * is + k];
+    out[l * os + k] = edge ? mkc<T2>(v.x * (T)w_edge, 0) : cscale(v, (T)w_mid);
   }
 }
 
@@ -1233,27 +1259,27 @@ hipError_t launch_chirp_in(int prec, const void* in, void* out, const void* w, l
 
 hipError_t launch_untangle(int prec, const void* Z, void* out, const void* utw, long long m,
                            long long nlines, long long zs, long long os, int mode,
-                           double psd_scale, hipStream_t s) {
+                           double psd_scale, double w_edge, double w_mid, hipStream_t s) {
   unsigned g = grid_for((m + 1) * nlines, 256);
   if (prec == 0)
     hipLaunchKernelGGL(k_untangle<double>, dim3(g), dim3(256), 0, s, Z, out, utw, m, nlines, zs, os,
-                       mode, psd_scale);
+                       mode, psd_scale, w_edge, w_mid);
   else
     hipLaunchKernelGGL(k_untangle<float>, dim3(g), dim3(256), 0, s, Z, out, utw, m, nlines, zs, os,
-                       mode, psd_scale);
+                       mode, psd_scale, w_edge, w_mid);
   return hipGetLastError();
 }
 
 hipError_t launch_herm_pack(int prec, const void* X, void* Z, const void* utw, long long m,
                             long long nlines, long long xs, long long bins, long long zs,
-                            hipStream_t s) {
+                            double w_mid, hipStream_t s) {
   unsigned g = grid_for(m * nlines, 256);
   if (prec == 0)
     hipLaunchKernelGGL(k_herm_pack<double>, dim3(g), dim3(256), 0, s, X, Z, utw, m, nlines, xs, bins,
-                       zs);
+                       zs, w_mid);
   else
     hipLaunchKernelGGL(k_herm_pack<float>, dim3(g), dim3(256), 0, s, X, Z, utw, m, nlines, xs, bins,
-                       zs);
+                       zs, w_mid);
   return hipGetLastError();
 }
 
@@ -1286,8 +1312,15 @@ hipError_t launch_copy_lines(int prec, const void* in, void* out, long long n, l
 }
 
 hipError_t launch_herm_full(int prec, const void* in, void* out, long long n, long long nlines,
-                            long long is, long long bins, long long os, hipStream_t s) {
-  GOFFT_LAUNCH_EW(k_herm_full, n * nlines, in, out, n, nlines, is, bins, os);
+                            long long is, long long bins, long long os, double w_mid, hipStream_t s) {
+  GOFFT_LAUNCH_EW(k_herm_full, n * nlines, in, out, n, nlines, is, bins, os, w_mid);
+  return hipGetLastError();
+}
+
+hipError_t launch_weight_bins(int prec, const void* in, void* out, long long bins, long long n,
+                              long long nlines, long long is, long long os, double w_edge,
+                              double w_mid, hipStream_t s) {
+  GOFFT_LAUNCH_EW(k_weight_bins, bins * nlines, in, out, bins, n, nlines, is, os, w_edge, w_mid);
   return hipGetLastError();
 }
 

@@ -1531,9 +1531,10 @@ class RealEngine {
     w.add(GOFFT_KERNEL_AUX, 1, batch * (inverse ? n * e + rb : nb * e + sb));
   }
 
-  // mode 0: complex bins (stride ostride per line), mode 1: PSD with psd_scale.
+  // mode 0: complex bins (stride ostride per line), mode 1: PSD with psd_scale,
+  // mode 2: complex bins times w_edge (DC, Nyquist) or w_mid (the others).
   int forward(const void* in, i64 istride, void* out, i64 ostride, int mode, double psd_scale,
-              const void* window, const Ctx& c) const {
+              const void* window, const Ctx& c, double w_edge = 1.0, double w_mid = 1.0) const {
     if (batch == 0 || n == 0) return GOFFT_OK;
     if (single) {
       RealFwdArgs a{};
@@ -1547,6 +1548,8 @@ class RealEngine {
       a.out_stride = ostride;
       a.mode = mode;
       a.psd_scale = psd_scale;
+      a.w_edge = w_edge;
+      a.w_mid = w_mid;
       a.f = sp.f;
       HIP_TRY(launch_real_fwd(prec, kMaxP, a, sp.threads, c.s));
       return GOFFT_OK;
@@ -1557,7 +1560,7 @@ class RealEngine {
       char* Z = c.ws;
       Ctx ic{c.s, Z + (size_t)m * batch * elt(prec)};
       RC_TRY(half->run(in, LineIO{nullptr, istride / 2}, Z, LineIO{nullptr, m}, 0, 0, 1.0, ic));
-      HIP_TRY(launch_untangle(prec, Z, out, utw.p, m, batch, m, ostride, mode, psd_scale, c.s));
+      HIP_TRY(launch_untangle(prec, Z, out, utw.p, m, batch, m, ostride, mode, psd_scale, w_edge, w_mid, c.s));
       return GOFFT_OK;
     }
     char* A = c.ws;
@@ -1567,13 +1570,17 @@ class RealEngine {
     RC_TRY(half->run(A, LineIO{nullptr, n}, B, LineIO{nullptr, n}, 0, 0, 1.0, ic));
     if (mode == 0)
       HIP_TRY(launch_copy_lines(prec, B, out, n / 2 + 1, batch, n, ostride, c.s));
+    else if (mode == 2)
+      HIP_TRY(launch_weight_bins(prec, B, out, n / 2 + 1, n, batch, n, ostride, w_edge, w_mid, c.s));
     else
       HIP_TRY(launch_power(prec, B, out, n / 2 + 1, n, batch, n, ostride, psd_scale, c.s));
     return GOFFT_OK;
   }
 
-  // IRFFT: `bins` bins per input line (stride istride), n reals per output line.
-  int inverse(const void* in, i64 istride, i64 bins, void* out, i64 ostride, const Ctx& c) const {
+  // IRFFT: `bins` bins per input line (stride istride), n reals per output
+  // line; the output times gain, the bins other than DC and Nyquist times w_mid.
+  int inverse(const void* in, i64 istride, i64 bins, void* out, i64 ostride, const Ctx& c, double gain = 1.0,
+              double w_mid = 1.0) const {
     if (batch == 0 || n <= 0) return GOFFT_OK;
     if (single) {
       RealInvArgs a{};
@@ -1585,7 +1592,8 @@ class RealEngine {
       a.in_stride = istride;
       a.in_bins = bins;
       a.out_stride = ostride;
-      a.scale = 1.0 / (double)m;
+      a.scale = gain / (double)m;
+      a.w_mid = w_mid;
       a.f = sp.f;
       HIP_TRY(launch_real_inv(prec, kMaxP, a, sp.threads, c.s));
       return GOFFT_OK;
@@ -1594,15 +1602,15 @@ class RealEngine {
       if (ostride % 2) return fail(GOFFT_ERR_UNSUPPORTED, "odd real line stride");
       char* Z = c.ws;
       Ctx ic{c.s, Z + (size_t)m * batch * elt(prec)};
-      HIP_TRY(launch_herm_pack(prec, in, Z, utw.p, m, batch, istride, bins, m, c.s));
-      RC_TRY(half->run(Z, LineIO{nullptr, m}, out, LineIO{nullptr, ostride / 2}, 1, 1, 1.0 / (double)m, ic));
+      HIP_TRY(launch_herm_pack(prec, in, Z, utw.p, m, batch, istride, bins, m, w_mid, c.s));
+      RC_TRY(half->run(Z, LineIO{nullptr, m}, out, LineIO{nullptr, ostride / 2}, 1, 1, gain / (double)m, ic));
       return GOFFT_OK;
     }
     char* A = c.ws;
     char* B = c.ws + (size_t)n * batch * elt(prec);
     Ctx ic{c.s, B + (size_t)n * batch * elt(prec)};
-    HIP_TRY(launch_herm_full(prec, in, A, n, batch, istride, bins, n, c.s));
-    RC_TRY(half->run(A, LineIO{nullptr, n}, B, LineIO{nullptr, n}, 1, 1, 1.0 / (double)n, ic));
+    HIP_TRY(launch_herm_full(prec, in, A, n, batch, istride, bins, n, w_mid, c.s));
+    RC_TRY(half->run(A, LineIO{nullptr, n}, B, LineIO{nullptr, n}, 1, 1, gain / (double)n, ic));
     HIP_TRY(launch_take_real(prec, B, out, n, batch, n, ostride, 1.0, c.s));
     return GOFFT_OK;
   }
@@ -2234,19 +2242,54 @@ int gofft_vjp_c2c(const gofft_plan* p, const void* gy, void* gx, int direction, 
   return fail(GOFFT_ERR_INVALID_ARG, "plan is not a complex-to-complex plan");
 }
 
+// y = RFFT(x): dL/dx_j = Re sum_k gy_k W^{-jk} over the n/2+1 bins, which is
+// n * IRFFT(w * gy) with w = 1 at DC and Nyquist and 1/2 elsewhere (the
+// IRFFT counts every interior bin twice through its Hermitian mirror).
 int gofft_vjp_r2c(const gofft_plan* p, const void* gy, void* gx, void* stream) {
-  (void)p; (void)gy; (void)gx; (void)stream;
-  return fail(GOFFT_ERR_UNSUPPORTED, "gofft_vjp_r2c: not built yet");
+  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = nullptr;
+  RC_TRY(p->ws(s, &ws));
+  if (p->n <= 0) return GOFFT_OK;
+  const i64 nb = p->n / 2 + 1;
+  return p->real->inverse(gy, nb, nb, gx, p->n, Ctx{s, ws}, (double)p->n, 0.5);
 }
 
+// x = IRFFT(X) reads Re X_0, Re X_{n/2} once and every other bin twice:
+// gX = RFFT(gx) / n with those bins doubled (Im of DC and Nyquist is 0).
 int gofft_vjp_c2r(const gofft_plan* p, const void* gx, void* gX, void* stream) {
-  (void)p; (void)gx; (void)gX; (void)stream;
-  return fail(GOFFT_ERR_UNSUPPORTED, "gofft_vjp_c2r: not built yet");
+  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = nullptr;
+  RC_TRY(p->ws(s, &ws));
+  if (p->n <= 0) return GOFFT_OK;
+  const double inv = 1.0 / (double)p->n;
+  return p->real->forward(gx, p->n, gX, p->n / 2 + 1, 2, 0.0, nullptr, Ctx{s, ws}, inv, 2.0 * inv);
 }
 
+// direction 0, y = RFFT2(x): gx = rows*cols * IRFFT2(w * gy) with the r2c
+// weights w over the bin axis (they commute with the column transforms).
+// direction 1, x = IRFFT2(X): gX = w' * RFFT2(gx) / (rows*cols) with the c2r
+// weights w'.
 int gofft_vjp_nd_r2c(const gofft_plan* p, const void* g_out, void* g_in, int direction, void* stream) {
-  (void)p; (void)g_out; (void)g_in; (void)direction; (void)stream;
-  return fail(GOFFT_ERR_UNSUPPORTED, "gofft_vjp_nd_r2c: not built yet");
+  if (!p || p->kind != K_ND_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an nd r2c plan");
+  if (direction != GOFFT_FORWARD && direction != GOFFT_INVERSE)
+    return fail(GOFFT_ERR_INVALID_ARG, "direction must be GOFFT_FORWARD or GOFFT_INVERSE");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = nullptr;
+  RC_TRY(p->ws(s, &ws));
+  const i64 B = p->batch;
+  if (B == 0) return GOFFT_OK;
+  const i64 htotal = p->rows * p->rcols * B;
+  if (direction == GOFFT_FORWARD) {
+    char* H = ws;
+    char* rest = ws + (size_t)htotal * elt(p->prec);
+    RC_TRY(run_nd_axes(p->axes, p->prec, htotal, g_out, H, 1, 1.0, s, rest));
+    return p->nd_row->inverse(H, p->rcols, p->rcols, g_in, p->cols, Ctx{s, rest}, (double)p->cols, 0.5);
+  }
+  const double inv = 1.0 / ((double)p->rows * (double)p->cols);
+  RC_TRY(p->nd_row->forward(g_out, p->cols, g_in, p->rcols, 2, 0.0, nullptr, Ctx{s, ws}, inv, 2.0 * inv));
+  return run_nd_axes(p->axes, p->prec, htotal, g_in, g_in, 0, 1.0, s, ws);
 }
 
 // ---------------------------------------------------------------------------

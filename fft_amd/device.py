@@ -185,7 +185,16 @@ class NDPlan(_Plan):
         return out
 
     def vjp(self, g, out, direction=0, stream=None):
+        """ND_R2C plans: VJP of RFFT2 (direction 0: g complex half spectrum,
+        out real) or of IRFFT2 (direction 1: g real, out complex)."""
         L.check(L.lib().gofft_vjp_nd_r2c(self._h, _ptr(g), _ptr(out), direction, _stream(stream)))
+        return out
+
+    def vjp_c2c(self, gy, out=None, inverse=False, stream=None):
+        """ND_C2C plans: VJP of FFTN (inverse=False) or IFFTN (inverse=True)."""
+        if out is None:
+            out = torch.empty_like(gy)
+        L.check(L.lib().gofft_vjp_c2c(self._h, _ptr(gy), _ptr(out), int(bool(inverse)), _stream(stream)))
         return out
 
 

@@ -190,3 +190,39 @@ def test_windows_and_freqs(api):
     close(api.FFTFreq(5, 1), [0, 0.2, 0.4, -0.4, -0.2])
     close(api.RFFTFreq(4, 0.25), [0, 1, 2])
     assert len(api.FFTFreq(0, 1)) == 0 and len(api.RFFTFreq(-5, 1)) == 0
+
+
+def test_vjp_api_adjoint_identities(api):
+    # For each transform T, its VJP G satisfies Re<T(x), g> = Re<x, G(g)>.
+    rng = np.random.default_rng(3)
+
+    def c(n):
+        return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+    def re_dot(a, b):
+        return float(np.real(np.vdot(b, a)))
+
+    for n in [8, 17, 1000, 1009]:
+        x, g = c(n), c(n)
+        assert math.isclose(re_dot(api.FFT(x), g), re_dot(x, api.FFTVJP(g)), rel_tol=1e-11)
+        assert math.isclose(re_dot(api.IFFT(x), g), re_dot(x, api.IFFTVJP(g)), rel_tol=1e-11)
+        xr, gr = rng.standard_normal(n), rng.standard_normal(n)
+        gb = c(n // 2 + 1)
+        assert math.isclose(re_dot(api.RFFT(xr), gb), re_dot(xr, api.RFFTVJP(gb, n)), rel_tol=1e-11)
+        assert math.isclose(re_dot(api.IRFFT(gb, n), gr), re_dot(gb, api.IRFFTVJP(gr)), rel_tol=1e-11)
+    for shape in [(4, 6), (3, 5, 7)]:
+        m = int(np.prod(shape))
+        x, g = c(m), c(m)
+        assert math.isclose(re_dot(api.FFTN(x, shape), g), re_dot(x, api.FFTNVJP(g, shape)), rel_tol=1e-11)
+        assert math.isclose(re_dot(api.IFFTN(x, shape), g), re_dot(x, api.IFFTNVJP(g, shape)), rel_tol=1e-11)
+    rows, cols = 6, 10
+    xr, gr = rng.standard_normal(rows * cols), rng.standard_normal(rows * cols)
+    gb = c(rows * (cols // 2 + 1))
+    assert math.isclose(re_dot(api.RFFT2(xr, (rows, cols)), gb), re_dot(xr, api.RFFT2VJP(gb, (rows, cols))),
+                        rel_tol=1e-11)
+    assert math.isclose(re_dot(api.IRFFT2(gb, (rows, cols)), gr), re_dot(gb, api.IRFFT2VJP(gr, (rows, cols))),
+                        rel_tol=1e-11)
+    with pytest.raises(api.Panic, match="shape lengths must be positive"):
+        api.RFFT2VJP(gb, (0, cols))
+    with pytest.raises(api.Panic, match="gradient length"):
+        api.RFFTVJP(gb, 7)
