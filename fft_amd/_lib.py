@@ -53,6 +53,7 @@ SIGNATURES = {
     "gofft_plan_out_elems": (_i64, [_vp, ctypes.c_int]),
     "gofft_plan_workspace_bytes": (_i64, [_vp]),
     "gofft_plan_num_launches": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gofft_plan_describe": (_i64, [_vp, ctypes.c_char_p, _i64]),
     "gofft_plan_kernel_work": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_i64),
                                               ctypes.POINTER(_i64)]),
     "gofft_exec_c2c": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),

@@ -15,6 +15,9 @@ namespace gofft {
 
 constexpr int kMaxStages = 16;   // LDS stages per sub-FFT (R <= 2^16 needs <= 16)
 constexpr int kMaxCols = 64;     // columns per workgroup
+// Static LDS of k_pass: seven 8-byte per-column tables and one complex
+// (at most 16 bytes) per column.
+constexpr int kPassStaticLds = kMaxCols * (7 * 8 + 16);
 
 // Division by a runtime constant d for n < 2^20, d < 2^20 (see fastdiv_make).
 struct FastDiv {
@@ -42,8 +45,9 @@ struct Stage {
   int ns;
   int nb;      // R / radix butterflies per column
   int tw_off;  // offset of this stage's twiddles in the stage table
-  int kmul;    // twiddles of butterfly k start at tw_off + k*kmul + cdig*cmul
-  int cmul;    // (cdig: the column's twiddle digit; normally kmul = radix-1, cmul = 0)
+  int kmul;    // twiddles of butterfly k start at tw_off + k*kmul + cdig*cmul (cdig: the
+  int cmul;    // column's twiddle digit); generic stages kmul = radix-1, cmul = 0;
+               // grouped radix-2 stages kmul = the pass's Ns, cmul = 1
   FastDiv nsd;
 };
 
@@ -74,7 +78,8 @@ inline void colmap_magic(ColMap& m, long long max_cols) {
 // Sub-FFT description shared by all kernels.
 struct SubFFT {
   int R;         // points per column
-  int Rp;        // LDS column pitch (R + R/16 + 1, see kernels.hip)
+  int Rp;        // LDS column pitch R + (R >> psh) + 1
+  int psh;       // LDS pad shift: point r of a column sits at r + (r >> psh)
   int C;         // columns per workgroup (power of two)
   int log2C;
   int tpc;       // threads per column (power of two)

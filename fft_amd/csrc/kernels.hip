@@ -3,11 +3,13 @@
 // Layout conventions: complex data is interleaved (re, im) in the working
 // precision (double2 for the plain complex128 path, float2 for the complex64
 // path). A workgroup owns C columns of R points; LDS holds point r of column c
-// at lds[c * Rp + lp(r)], with an odd column pitch Rp = R + R/16 + 1 and one
-// pad slot after every 16 points, so that neither walking across columns
-// (load/store phases) nor the stride-radix writes of the first Stockham stages
-// hit one LDS bank. Every LDS stage reads all its butterfly inputs into
-// registers, synchronises, then writes the outputs (in-place Stockham).
+// at lds[c * Rp + lp(r, psh)], with an odd column pitch Rp = R + (R >> psh) + 1
+// and one pad slot after every 2^psh points (psh = 4 normally), so that
+// neither walking across columns (load/store phases) nor the stride-radix
+// writes of the first Stockham stages hit one LDS bank. A single-column
+// convolution that only fits the 160 KB of LDS with less padding uses psh = 6.
+// Every LDS stage reads all its butterfly inputs into registers, synchronises,
+// then writes the outputs (in-place Stockham).
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -198,21 +200,34 @@ template <int RAD, typename T2> __device__ __forceinline__ void dft(T2* v) {
 // One LDS stage over all columns of the workgroup. Thread (c, lt) owns the
 // butterflies j = lt, lt + tpc, ... of column c.
 
-__device__ __forceinline__ int lp(int p) { return p + (p >> 4); }
+__device__ __forceinline__ int lp(int p, int sh) { return p + (p >> sh); }
 
 template <typename T2, int RAD, int MAXP>
 __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage& s, const T2* stw,
                                           int c, int lt, int cdig) {
   constexpr int NB = MAXP / RAD > 0 ? MAXP / RAD : 1;
   T2* col = lds + c * f.Rp;
-  const int nb = s.nb, ns = s.ns, tpc = f.tpc;
+  const int nb = s.nb, ns = s.ns, tpc = f.tpc, sh = f.psh;
+  // Uniform per stage: reads j + i*nb and writes o + i*ns are affine when
+  // nb (ns) is a multiple of the pad block; the first stage's writes (ns = 1)
+  // when RAD divides the pad block.
+  const bool rd_aff = (nb & ((1 << sh) - 1)) == 0;
+  const bool wr_aff = (ns & ((1 << sh) - 1)) == 0 || (ns == 1 && ((1 << sh) % RAD) == 0);
+  const int rd_step = nb + (nb >> sh);
+  const int wr_step = ns == 1 ? 1 : ns + (ns >> sh);
   T2 v[NB][RAD];
 #pragma unroll
   for (int m = 0; m < NB; ++m) {
     int j = lt + m * tpc;
     if (j < nb) {
+      if (rd_aff) {
+        const T2* p = col + lp(j, sh);
 #pragma unroll
-      for (int i = 0; i < RAD; ++i) v[m][i] = col[lp(j + i * nb)];
+        for (int i = 0; i < RAD; ++i) v[m][i] = p[i * rd_step];
+      } else {
+#pragma unroll
+        for (int i = 0; i < RAD; ++i) v[m][i] = col[lp(j + i * nb, sh)];
+      }
       if (ns > 1 || s.cmul) {
         int q = (int)fastdiv_q((uint32_t)j, s.nsd);
         int k = j - q * ns;
@@ -231,8 +246,14 @@ __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage&
       int q = (int)fastdiv_q((uint32_t)j, s.nsd);
       int k = j - q * ns;
       const int o = q * ns * RAD + k;
+      if (wr_aff) {
+        T2* p = col + lp(o, sh);
 #pragma unroll
-      for (int i = 0; i < RAD; ++i) col[lp(o + i * ns)] = v[m][i];
+        for (int i = 0; i < RAD; ++i) p[i * wr_step] = v[m][i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < RAD; ++i) col[lp(o + i * ns, sh)] = v[m][i];
+      }
     }
   }
   __syncthreads();
@@ -264,7 +285,7 @@ __device__ __forceinline__ void lds_stage_rec(T2* lds, const SubFFT& f, const St
     const int j = lt + m * tpc;
     if (j < nb) {
 #pragma unroll
-      for (int i = 0; i < TT; ++i) v[m][i] = col[lp(j + i * nb)];
+      for (int i = 0; i < TT; ++i) v[m][i] = col[lp(j + i * nb, f.psh)];
       const int q = (int)fastdiv_q((uint32_t)j, s.nsd);
       const int k = j - q * ns;
 #pragma unroll
@@ -292,7 +313,7 @@ __device__ __forceinline__ void lds_stage_rec(T2* lds, const SubFFT& f, const St
       const int k = j - q * ns;
       const int o = q * ns * TT + k;
 #pragma unroll
-      for (int i = 0; i < TT; ++i) col[lp(o + cbrev(i, t) * ns)] = v[m][i];
+      for (int i = 0; i < TT; ++i) col[lp(o + cbrev(i, t) * ns, f.psh)] = v[m][i];
     }
   }
   __syncthreads();
@@ -316,10 +337,11 @@ __device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw, int cdig = 0) {
       case 8: lds_stage<T2, 8, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       case 11: lds_stage<T2, 11, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       case 13: lds_stage<T2, 13, MAXP>(lds, f, s, stw, c, lt, cdig); break;
-      case 102: lds_stage_rec<T2, 2, MAXP>(lds, f, s, stw, c, lt, cdig); break;
-      case 104: lds_stage_rec<T2, 4, MAXP>(lds, f, s, stw, c, lt, cdig); break;
-      case 108: lds_stage_rec<T2, 8, MAXP>(lds, f, s, stw, c, lt, cdig); break;
-      case 116: lds_stage_rec<T2, 16, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      // grouped radix-2 layers: only the 16-point form is planned with them
+      case 102: if constexpr (MAXP == 16) lds_stage_rec<T2, 2, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 104: if constexpr (MAXP == 16) lds_stage_rec<T2, 4, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 108: if constexpr (MAXP == 16) lds_stage_rec<T2, 8, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 116: if constexpr (MAXP == 16) lds_stage_rec<T2, 16, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       default: lds_stage<T2, 16, MAXP>(lds, f, s, stw, c, lt, cdig); break;
     }
   }
@@ -360,16 +382,6 @@ __device__ __forceinline__ double2 pass_twiddle(const double2* hi, const double2
 
 __device__ __forceinline__ double2 dmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// Sum over aligned groups of `width` lanes (width a power of two <= 64); every
-// lane of a group ends with the group total.
-template <typename T2> __device__ __forceinline__ T2 group_sum(T2 v, int width) {
-  for (int o = 1; o < width; o <<= 1) {
-    v.x += __shfl_xor(v.x, o, 64);
-    v.y += __shfl_xor(v.y, o, 64);
-  }
-  return v;
 }
 
 extern __shared__ unsigned char g_dyn_lds[];
@@ -421,29 +433,50 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   __syncthreads();
   const T2* in = reinterpret_cast<const T2*>(a.in);
   const int total = C * R;  // <= MAXP * blockDim.x (guaranteed by the planner)
+  // Walking across columns, or along a single column, a thread keeps one
+  // column c = tid mod C and visits r = tid / C + u * (nthr / C): its
+  // addresses are one base plus a fixed step.
+  const int fc = tid & (C - 1);
+  const int fr0 = tid >> f.log2C, frs = nthr >> f.log2C;
+  const bool in_fixed = !a.im.rowmajor || C == 1;
+  const bool out_fixed = !a.om.rowmajor || C == 1;
   // Issue every global load of this thread before using any of them, so that
   // MAXP loads per lane are in flight at once.
   T2 buf[MAXP];
+  if (in_fixed) {
+    const long long base = s_ib[fc];
+    const long long il = s_il[fc];
+    if (base >= 0 && a.in_hook == 0) {
+      const T2* p = in + base + il + (long long)fr0 * a.im.g;
+      const long long st = (long long)frs * a.im.g;
 #pragma unroll
-  for (int u = 0; u < MAXP; ++u) {
-    const int fi = tid + u * nthr;
-    buf[u] = mkc<T2>(0, 0);
-    if (fi < total) {
-      int c, r;
-      if (a.im.rowmajor) {
-        c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-        r = fi - c * R;
-      } else {
-        c = fi & (C - 1);
-        r = fi >> f.log2C;
-      }
-      const long long base = s_ib[c];
-      if (base >= 0) {
-        const long long i = s_il[c] + (long long)r * a.im.g;
-        if (a.in_hook == 0) {
-          buf[u] = in[base + i];
-        } else if (i < a.nvalid) {
+      for (int u = 0; u < MAXP; ++u) buf[u] = fr0 + u * frs < R ? p[u * st] : mkc<T2>(0, 0);
+    } else {
+#pragma unroll
+      for (int u = 0; u < MAXP; ++u) {
+        const int r = fr0 + u * frs;
+        buf[u] = mkc<T2>(0, 0);
+        const long long i = il + (long long)r * a.im.g;
+        if (base >= 0 && r < R && i < a.nvalid)
           buf[u] = in[base + (a.in_hook == 2 ? (long long)a.hook_perm[i] : i)];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < MAXP; ++u) {
+      const int fi = tid + u * nthr;
+      buf[u] = mkc<T2>(0, 0);
+      if (fi < total) {
+        const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+        const int r = fi - c * R;
+        const long long base = s_ib[c];
+        if (base >= 0) {
+          const long long i = s_il[c] + (long long)r * a.im.g;
+          if (a.in_hook == 0) {
+            buf[u] = in[base + i];
+          } else if (i < a.nvalid) {
+            buf[u] = in[base + (a.in_hook == 2 ? (long long)a.hook_perm[i] : i)];
+          }
         }
       }
     }
@@ -460,17 +493,22 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
       tw_step = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)f.tpc * k, a.tw_shift);
     }
   }
+  // The LDS slots of a fixed column: lp(fr0 + u * frs) is affine in u when
+  // frs is a whole number of pad blocks.
+  const bool lds_aff = (frs & ((1 << f.psh) - 1)) == 0;
+  T2* const lcol = lds + fc * Rp;
+  const int lstep = frs + (frs >> f.psh);
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
     const int fi = tid + u * nthr;
     if (fi < total) {
       int c, r;
-      if (a.im.rowmajor) {
+      if (in_fixed) {
+        c = fc;
+        r = fr0 + u * frs;
+      } else {
         c = (int)fastdiv_q((uint32_t)fi, f.Rd);
         r = fi - c * R;
-      } else {
-        c = fi & (C - 1);
-        r = fi >> f.log2C;
       }
       T2 v = buf[u];
       if (a.tw_M) {
@@ -482,7 +520,10 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
         const long long i = s_il[c] + (long long)r * a.im.g;
         if (i < a.nvalid) v = cmul(v, reinterpret_cast<const T2*>(a.hook_w)[i]);
       }
-      lds[c * Rp + lp(r)] = v;
+      if (in_fixed && lds_aff)
+        lcol[lp(fr0, f.psh) + u * lstep] = v;
+      else
+        lds[c * Rp + lp(r, f.psh)] = v;
     }
   }
   __syncthreads();
@@ -495,9 +536,9 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
     if (s_ob[c] >= 0) {
       for (int r = lt; r < R; r += f.tpc) {
         const long long k = s_mk[c] + (long long)r * a.mm.g;
-        T2 v = lds[c * Rp + lp(r)];
+        T2 v = lds[c * Rp + lp(r, f.psh)];
         if (a.dc && k == 0) reinterpret_cast<T2*>(a.dc)[s_ln[c]] = v;  // Rader: A[0]
-        lds[c * Rp + lp(r)] = cconj(cmul(v, spec[k]));
+        lds[c * Rp + lp(r, f.psh)] = cconj(cmul(v, spec[k]));
       }
     }
     __syncthreads();
@@ -506,21 +547,39 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   T2* out = reinterpret_cast<T2*>(a.out);
   const T sc = (T)a.scale;
   const bool do_scale = a.scale != 1.0;
+  if (out_fixed && a.out_hook == 0) {
+    const long long base = s_ob[fc];
+    if (base >= 0) {
+      T2* po = out + base + s_ol[fc] + (long long)fr0 * a.om.g;
+      const long long st = (long long)frs * a.om.g;
+#pragma unroll
+      for (int u = 0; u < MAXP; ++u) {
+        const int r = fr0 + u * frs;
+        if (r < R) {
+          T2 v = lds_aff ? lcol[lp(fr0, f.psh) + u * lstep] : lcol[lp(r, f.psh)];
+          if (a.conj_out) v.y = -v.y;
+          if (do_scale) v = cscale(v, sc);
+          po[u * st] = v;
+        }
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
     const int fi = tid + u * nthr;
     if (fi < total) {
       int c, r;
-      if (a.om.rowmajor) {
+      if (out_fixed) {
+        c = fc;
+        r = fr0 + u * frs;
+      } else {
         c = (int)fastdiv_q((uint32_t)fi, f.Rd);
         r = fi - c * R;
-      } else {
-        c = fi & (C - 1);
-        r = fi >> f.log2C;
       }
       const long long base = s_ob[c];
       if (base >= 0) {
-        T2 v = lds[c * Rp + lp(r)];
+        T2 v = lds[c * Rp + lp(r, f.psh)];
         const long long p = s_ol[c] + (long long)r * a.om.g;
         if (a.out_hook == 0) {
           if (a.conj_out) v.y = -v.y;
@@ -588,7 +647,7 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
       const int r = fi - c * R;
       T2 v = buf[u];
       if (win) v = mkc<T2>(v.x * win[2 * r], v.y * win[2 * r + 1]);
-      lds[c * Rp + lp(r)] = v;
+      lds[c * Rp + lp(r, f.psh)] = v;
     }
   }
   __syncthreads();
@@ -607,7 +666,7 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
       T2 z0 = Z[0];
       X = mkc<T2>(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0);
     } else {
-      T2 zk = Z[lp(k)], zmk = Z[lp(R - k)];
+      T2 zk = Z[lp(k, f.psh)], zmk = Z[lp(R - k, f.psh)];
       T xer = (zk.x + zmk.x) * (T)0.5;
       T xei = (zk.y - zmk.y) * (T)0.5;
       T xor_ = (zk.y + zmk.y) * (T)0.5;
@@ -664,7 +723,7 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
     if (fi < total) {
       const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
       const int r = fi - c * R;
-      lds[c * Rp + lp(r)] = buf[u];
+      lds[c * Rp + lp(r, f.psh)] = buf[u];
     }
   }
   __syncthreads();
@@ -686,7 +745,7 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
       continue;
     }
     int mk_ = R - k;
-    T2 xk = X[lp(k)], xmk = X[lp(mk_)];
+    T2 xk = X[lp(k, f.psh)], xmk = X[lp(mk_, f.psh)];
     // Z[k] = xe + i*xo with xe = (Xk + conj Xmk)/2, d = (Xk - conj Xmk)/2,
     // xo = conj(W^k) d; Z[m-k] symmetric.
     T2 w = utw[k];
@@ -697,9 +756,9 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
       T2 zk = mkc<T2>(xer - xoi, xei + xor_);
       if (k != mk_) {
         T2 zmk = mkc<T2>(xer + xoi, -(xei - xor_));
-        X[lp(mk_)] = cconj(zmk);
+        X[lp(mk_, f.psh)] = cconj(zmk);
       }
-      X[lp(k)] = cconj(zk);
+      X[lp(k, f.psh)] = cconj(zk);
     }
   }
   __syncthreads();
@@ -711,7 +770,7 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
     int r = fi - c * R;
     long long l = l0 + c;
     if (l >= a.nlines) continue;
-    T2 z = lds[c * Rp + lp(r)];
+    T2 z = lds[c * Rp + lp(r, f.psh)];
     // out_stride is even (n = 2R), so each sample pair is one aligned T2
     *reinterpret_cast<T2*>(out + l * a.out_stride + 2 * (long long)r) = mkc<T2>(z.x * sc, -z.y * sc);
   }
@@ -723,7 +782,6 @@ template <typename T, int MAXP>
 __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
-  __shared__ T2 s_red[16];  // one partial per wave
   const SubFFT& f = a.f;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int C = f.C, R = f.R, Rp = f.Rp;
@@ -733,59 +791,55 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   const int n = a.n;
   const int q = n - 1;
   const int total = C * R;
-  T2 buf[MAXP];
+  // Rounds of LB buffered loads per thread (one round unless C * R exceeds
+  // 16 * blockDim.x, which only the 24-point form allows).
+  constexpr int LB = 16;
+  const int rounds = MAXP == LB ? 1 : (total + LB * nthr - 1) / (LB * nthr);
+  for (int rd = 0; rd < rounds; ++rd) {
+    const int base = rd * LB * nthr;
+    T2 buf[LB];
 #pragma unroll
-  for (int u = 0; u < MAXP; ++u) {
-    const int fi = tid + u * nthr;
-    buf[u] = mkc<T2>(0, 0);
-    if (fi < total) {
-      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-      const int r = fi - c * R;
-      const long long l = l0 + c;
-      if (l < a.nlines) {
-        const T2* x = in + (a.in_off ? a.in_off[l] : l * a.in_stride);
-        if (a.kind == 2) {
-          if (r < n) buf[u] = x[r];
-        } else if (r < q) {
-          buf[u] = x[a.perm[r]];
+    for (int u = 0; u < LB; ++u) {
+      const int fi = base + tid + u * nthr;
+      buf[u] = mkc<T2>(0, 0);
+      if (fi < total) {
+        const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+        const int r = fi - c * R;
+        const long long l = l0 + c;
+        if (l < a.nlines) {
+          const T2* x = in + (a.in_off ? a.in_off[l] : l * a.in_stride);
+          if (a.kind == 2) {
+            if (r < n) buf[u] = x[r];
+          } else if (r < q) {
+            buf[u] = x[a.perm[r]];
+          }
         }
       }
     }
-  }
 #pragma unroll
-  for (int u = 0; u < MAXP; ++u) {
-    const int fi = tid + u * nthr;
-    if (fi < total) {
-      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-      const int r = fi - c * R;
-      T2 v = buf[u];
-      if (a.conj_in) v.y = -v.y;
-      if (a.kind == 2 && r < n) v = cmul(v, chirp[r]);
-      lds[c * Rp + lp(r)] = v;
+    for (int u = 0; u < LB; ++u) {
+      const int fi = base + tid + u * nthr;
+      if (fi < total) {
+        const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
+        const int r = fi - c * R;
+        T2 v = buf[u];
+        if (a.conj_in) v.y = -v.y;
+        if (a.kind == 2 && r < n) v = cmul(v, chirp[r]);
+        lds[c * Rp + lp(r, f.psh)] = v;
+      }
     }
   }
   __syncthreads();
   const int c = tid >> f.log2tpc;
   const int lt = tid & (f.tpc - 1);
-  T2 colsum = mkc<T2>(0, 0);
-  if (a.kind != 2) {
-    // Rader DC: sum of x[1..n-1] (all gathered samples) plus x[0].
-    T2 acc = mkc<T2>(0, 0);
-    for (int r = lt; r < q; r += f.tpc) acc = cadd(acc, lds[c * Rp + lp(r)]);
-    acc = group_sum(acc, f.tpc < 64 ? f.tpc : 64);
-    if (f.tpc > 64) {
-      if ((tid & 63) == 0) s_red[tid >> 6] = acc;
-      __syncthreads();
-      acc = mkc<T2>(0, 0);
-      const int w0 = (c << f.log2tpc) >> 6, nw = f.tpc >> 6;
-      for (int w = 0; w < nw; ++w) acc = cadd(acc, s_red[w0 + w]);
-    }
-    colsum = acc;
-  }
   const T2* stw = reinterpret_cast<const T2*>(a.stw);
   lds_fft<T2, MAXP>(lds, f, stw);
+  // Rader DC: X[0] = x[0] + sum of the gathered samples, and that sum is the
+  // forward transform's A[0]. The thread with lt == 0 reads it before it
+  // multiplies point 0 itself.
+  const T2 colsum = a.kind != 2 && lt == 0 ? lds[c * Rp] : mkc<T2>(0, 0);
   const T2* spec = reinterpret_cast<const T2*>(a.spec);
-  for (int r = lt; r < R; r += f.tpc) lds[c * Rp + lp(r)] = cconj(cmul(lds[c * Rp + lp(r)], spec[r]));
+  for (int r = lt; r < R; r += f.tpc) lds[c * Rp + lp(r, f.psh)] = cconj(cmul(lds[c * Rp + lp(r, f.psh)], spec[r]));
   __syncthreads();
   lds_fft<T2, MAXP>(lds, f, stw);
   const long long l = l0 + c;
@@ -794,7 +848,7 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   const T sc = (T)a.scale;
   if (a.kind == 2) {
     for (int k = lt; k < n && live; k += f.tpc) {
-      T2 v = cmul(cconj(lds[c * Rp + lp(k)]), chirp[k]);
+      T2 v = cmul(cconj(lds[c * Rp + lp(k, f.psh)]), chirp[k]);
       if (a.conj_out) v.y = -v.y;
       out[k] = cscale(v, sc);
     }
@@ -804,7 +858,7 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
     if (a.conj_in) x0.y = -x0.y;
     const int off = a.kind == 1 ? q : 0;
     for (int k = lt; k < q && live; k += f.tpc) {
-      T2 v = cadd(x0, cconj(lds[c * Rp + lp(k + off)]));
+      T2 v = cadd(x0, cconj(lds[c * Rp + lp(k + off, f.psh)]));
       if (a.conj_out) v.y = -v.y;
       out[a.iperm[k]] = cscale(v, sc);
     }
@@ -1198,7 +1252,10 @@ hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipSt
     GOFFT_DISPATCH_MAXP(k_conv, double, a, bytes, threads, groups, s);
   } else {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
-    GOFFT_DISPATCH_MAXP(k_conv, float, a, bytes, threads, groups, s);
+    if (maxp == 24)
+      hipLaunchKernelGGL((k_conv<float, 24>), dim3(groups), dim3(threads), bytes, s, a);
+    else
+      GOFFT_DISPATCH_MAXP(k_conv, float, a, bytes, threads, groups, s);
   }
   return hipGetLastError();
 }

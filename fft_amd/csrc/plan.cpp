@@ -58,7 +58,8 @@ int fail(int code, const std::string& msg) {
 
 struct DevInfo {
   int id = -1;
-  int lds_bytes = 65536;  // usable dynamic LDS per workgroup
+  int lds_bytes = 65536;  // dynamic LDS per workgroup beside k_pass's static tables
+  int lds_full = 65536;   // dynamic LDS per workgroup of the kernels without static LDS
   int cus = 0;
 };
 std::mutex g_dev_mu;
@@ -75,11 +76,14 @@ int current_device(DevInfo* out) {
     int lds = 0, cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, id));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, id));
-    // Leave room for the kernels' static LDS (column tables, reductions).
-    d.lds_bytes = lds - 2048;
+    // k_pass keeps per-column tables in static LDS; k_conv, k_real_fwd and
+    // k_real_inv have none.
+    d.lds_full = lds;
+    d.lds_bytes = lds - kPassStaticLds;
     if (const char* e = std::getenv("GOFFT_LDS_BYTES")) {
       int v = std::atoi(e);
       if (v > 0 && v < d.lds_bytes) d.lds_bytes = v;
+      if (v > 0 && v < d.lds_full) d.lds_full = v;
     }
     d.cus = cus;
     it = g_devs.emplace(id, d).first;
@@ -276,14 +280,20 @@ int upload_real(DevBuf& b, const std::vector<long double>& v, int prec) {
 // ---------------------------------------------------------------------------
 // Sub-FFT schedules.
 
-std::vector<int> radix_schedule(int R) {
+int ilog2c(int x) {
+  int l = 0;
+  while ((1 << l) < x) l++;
+  return l;
+}
+
+std::vector<int> radix_schedule(int R, int maxrad = 16) {
   std::vector<int> r;
   int x = R;
-  while (x % 16 == 0) {
+  while (maxrad >= 16 && x % 16 == 0) {
     r.push_back(16);
     x /= 16;
   }
-  if (x % 8 == 0) {
+  while (x % 8 == 0) {
     r.push_back(8);
     x /= 8;
   }
@@ -304,11 +314,12 @@ std::vector<int> radix_schedule(int R) {
   return r;
 }
 
-constexpr int kMaxP = 16;  // points per thread per stage (kernel template)
+constexpr int kMaxP = 16;   // points per thread per stage (kernel template)
+constexpr int kWideP = 24;  // the complex64 fused convolution's wide form
 
 // LDS column pitch for R points (kernels.hip: one pad slot per 16 points plus
 // one, odd so adjacent columns fall in different banks).
-i64 pitch(i64 R) { return R + (R >> 4) + 1; }
+i64 pitch(i64 R, int sh = 4) { return R + (R >> sh) + 1; }
 
 // Preferred LDS footprint (points) of a batched single-pass workgroup: about
 // 64 KB, so that two or more workgroups share a CU's 160 KB.
@@ -317,8 +328,20 @@ i64 soft_pts(int prec) { return prec == 0 ? 4352 : 8704; }
 struct SubPlan {
   SubFFT f{};
   int threads = 0;
+  int maxp = kMaxP;     // points per thread per stage (kernel template)
   std::vector<cld> tw;  // stage twiddles
 };
+
+std::string describe_sub(i64 R, const SubPlan& sp) {
+  std::string s = std::to_string(R) + "[C" + std::to_string(sp.f.C) + " T" + std::to_string(sp.threads) +
+                  (sp.maxp != kMaxP ? " P" + std::to_string(sp.maxp) : "") +
+                  (sp.f.psh != 4 ? " pad" + std::to_string(1 << sp.f.psh) : "") + ":";
+  for (int i = 0; i < sp.f.nstages; i++) {
+    const int r = sp.f.st[i].radix;
+    s += (i ? "," : "") + (r > 100 ? "r2^" + std::to_string(ilog2c(r - 100)) : std::to_string(r));
+  }
+  return s + "]";
+}
 
 int ilog2(int x) {
   int l = 0;
@@ -327,27 +350,28 @@ int ilog2(int x) {
 }
 
 // Builds the LDS schedule for C columns of R points; false if impossible.
-bool make_sub(int R, int C, SubPlan& sp) {
+bool make_sub(int R, int C, SubPlan& sp, int min_threads = 0, int sh = 4, int maxp = kMaxP, int maxrad = 16) {
   if (R < 1 || C < 1 || C > kMaxCols || (C & (C - 1))) return false;
   std::vector<int> rads;
   if (R > 1) {
-    rads = radix_schedule(R);
+    rads = radix_schedule(R, maxrad);
     if (rads.empty()) return false;
   }
   if ((int)rads.size() > kMaxStages) return false;
   int T = 64;
-  while (T < C) T <<= 1;
-  while ((i64)T * kMaxP < (i64)C * R && T < 1024) T <<= 1;
+  while (T < C || T < min_threads) T <<= 1;
+  while ((i64)T * maxp < (i64)C * R && T < 1024) T <<= 1;
   bool ok = false;
   for (; T <= 1024; T <<= 1) {
     int tpc = T / C;
-    // every thread holds at most kMaxP points in the load/store phases
-    ok = (i64)T * kMaxP >= (i64)C * R;
+    // every thread holds at most maxp points per stage (and per load round
+    // of the pass kernels, which have one round)
+    ok = maxp != kMaxP || (i64)T * maxp >= (i64)C * R;
     for (int rad : rads) {
       int nb = R / rad;
       int per = (nb + tpc - 1) / tpc;
-      int cap = kMaxP / rad > 0 ? kMaxP / rad : 1;
-      if (rad > kMaxP || per > cap) {
+      int cap = maxp / rad > 0 ? maxp / rad : 1;
+      if (rad > maxp || per > cap) {
         ok = false;
         break;
       }
@@ -357,9 +381,11 @@ bool make_sub(int R, int C, SubPlan& sp) {
   if (!ok) return false;
   sp = SubPlan();
   sp.threads = T;
+  sp.maxp = maxp;
   SubFFT& f = sp.f;
   f.R = R;
-  f.Rp = (int)pitch(R);
+  f.Rp = (int)pitch(R, sh);
+  f.psh = sh;
   f.C = C;
   f.log2C = ilog2(C);
   f.tpc = T / C;
@@ -418,6 +444,7 @@ bool make_sub_r2(int R, int C, i64 Ns, SubPlan& sp) {
   SubFFT& f = sp.f;
   f.R = R;
   f.Rp = (int)pitch(R);
+  f.psh = 4;
   f.C = C;
   f.log2C = ilog2(C);
   f.tpc = tpc;
@@ -577,6 +604,24 @@ bool pick_cols(int R, int cap, i64 pts, SubPlan& sp, int* Cout) {
   return false;
 }
 
+// Schedule of a fused single-kernel convolution of length m (k_conv): C
+// columns within the preferred LDS budget when they fit, else one column with
+// the whole LDS. The complex64 kernel also has a 24-points-per-thread form
+// (radix <= 8 stages), and may pad every 32 or 64 points instead of 16, which
+// takes a single column up to 20480 points.
+bool fused_conv_plan(i64 m, int prec, i64 pts, SubPlan& sp) {
+  int C = 0;
+  if (pitch(m) <= pts && pick_cols((int)m, 16, std::min<i64>(pts, std::max<i64>(pitch(m), soft_pts(prec))), sp, &C))
+    return true;
+  if (prec != GOFFT_C64 || m > 24 * 1024) return false;
+  for (int sh : {4, 5, 6})
+    if (pitch(m, sh) <= pts)
+      for (int maxp : {kMaxP, kWideP})
+        for (int maxrad : {16, 8})
+          if (make_sub((int)m, 1, sp, 0, sh, maxp, maxrad)) return true;
+  return false;
+}
+
 // ---------------------------------------------------------------------------
 // Engines: a complex transform of length n over `batch` lines.
 
@@ -610,6 +655,8 @@ class Engine {
   virtual int launches() const = 0;
   // Adds the launches and data bytes of a run over `lines` lines.
   virtual void work(Work& w, i64 lines) const = 0;
+  // One line of text naming the engine and its launches.
+  virtual std::string describe() const = 0;
   // out = scale * conj?( FFT( conj?(in) ) ) per line, for the first `nl`
   // lines (all `batch` lines when nl < 0).
   virtual int run(const void* in, LineIO li, void* out, LineIO lo, int conj_in, int conj_out,
@@ -659,6 +706,7 @@ class TrivialEngine : public Engine {
   void work(Work& w, i64 lines) const override {
     if (n == 1 && lines > 0) w.add(GOFFT_KERNEL_AUX, 1, 2 * lines * (i64)elt(prec));
   }
+  std::string describe() const override { return "trivial n=" + std::to_string(n); }
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
           const Ctx& c, i64 nl) const override {
     const i64 lines = nl < 0 ? batch : nl;
@@ -827,13 +875,18 @@ class StockhamEngine : public Engine {
     DevInfo dev;
     RC_TRY(current_device(&dev));
     const i64 pts = dev.lds_bytes / (i64)elt(prec);
-    const int cmp = prec == 0 ? 8 : 16;
+    int cmp = prec == 0 ? 8 : 16;
+    int tmin = 0;
+    if (const char* e = std::getenv("GOFFT_PASS_COLS")) cmp = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("GOFFT_PASS_THREADS")) tmin = std::max(0, std::atoi(e));
     passes.clear();
     i64 Ns = 1;
     for (i64 R : lens) {
       Pass p;
       int Cp = 0;
-      if (!pick_cols((int)R, cmp, pts, p.sp, &Cp))
+      bool ok = pick_cols((int)R, cmp, pts, p.sp, &Cp);
+      if (ok && tmin > p.sp.threads) ok = make_sub((int)R, Cp, p.sp, tmin);
+      if (!ok)
         return fail(GOFFT_ERR_UNSUPPORTED, "no LDS schedule for pass length " + std::to_string(R));
       p.R = R;
       p.Ns = Ns;
@@ -907,6 +960,17 @@ class StockhamEngine : public Engine {
     if (lines <= 0 || P() == 0) return;
     const i64 chunks = P() <= 1 ? 1 : (lines + chunk() - 1) / chunk();
     w.add(GOFFT_KERNEL_PASS, P() * chunks, (i64)P() * 2 * n * inner * lines * (i64)elt(prec));
+  }
+  std::string passes_text() const {
+    std::string s;
+    for (int i = 0; i < P(); i++) s += (i ? " x " : "") + describe_sub(passes[i].R, passes[i].sp);
+    return s;
+  }
+  std::string describe() const override {
+    std::string s = "stockham n=" + std::to_string(n) + " lines=" + std::to_string(batch);
+    if (st != 1) s += " stride=" + std::to_string(st);
+    if (P() > 1) s += " chunk=" + std::to_string(chunk());
+    return s + " passes=" + passes_text();
   }
 
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
@@ -1020,6 +1084,10 @@ struct ConvChain {
 
   size_t ws_bytes() const { return 2 * (size_t)m * cb * elt(prec) + (size_t)cb * elt(prec); }
   int launches() const { return 2 * fwd.P() - 1; }
+  std::string describe() const {
+    return "chain m=" + std::to_string(m) + " chunk=" + std::to_string(cb) + " fwd=" + fwd.passes_text() +
+           " inv=" + inv.passes_text();
+  }
   // One run over `lines` lines in chunks of cb: the first pass reads nin
   // points of each line, the last writes nout.
   void work(Work& w, i64 lines, i64 nin, i64 nout) const {
@@ -1129,8 +1197,9 @@ struct ConvChain {
 // small radices).
 i64 choose_conv_len(i64 lo, int prec) {
   DevInfo dev;
-  if (current_device(&dev) != GOFFT_OK) dev.lds_bytes = 65536;
+  current_device(&dev);
   const i64 pts = dev.lds_bytes / (i64)elt(prec);
+  const i64 pts_full = dev.lds_full / (i64)elt(prec);
   const int cmp = prec == 0 ? 8 : 16;
   const i64 rmax = std::min<i64>(pts / cmp * 16 / 17, 1024);
   i64 hi = lo + (lo * 4 + 9) / 10;
@@ -1143,7 +1212,7 @@ i64 choose_conv_len(i64 lo, int prec) {
       while (x % p == 0) x /= p;
     if (x != 1) continue;
     SubPlan tmp;
-    int fit = (pitch(m) <= pts && make_sub((int)m, 1, tmp)) ? 1 : 0;
+    int fit = fused_conv_plan(m, prec, pts_full, tmp) ? 1 : 0;
     double cost;
     if (fit) {
       cost = (double)m * (1.0 + 0.25 * (double)radix_schedule((int)m).size());
@@ -1209,10 +1278,7 @@ class RaderEngine : public Engine {
     RC_TRY(upload_complex(spec, b, prec));  // natural order (fused kernel)
     DevInfo dev;
     RC_TRY(current_device(&dev));
-    const i64 pts = dev.lds_bytes / (i64)elt(prec);
-    int C = 0;
-    if (pitch(cl) <= pts &&
-        pick_cols((int)cl, 16, std::min<i64>(pts, std::max<i64>(pitch(cl), soft_pts(prec))), sp, &C)) {
+    if (fused_conv_plan(cl, prec, dev.lds_full / (i64)elt(prec), sp)) {
       fused = true;
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
@@ -1232,6 +1298,11 @@ class RaderEngine : public Engine {
     if (lines <= 0) return;
     if (fused) w.add(GOFFT_KERNEL_CONV, 1, 2 * n * lines * (i64)elt(prec));
     else chain.work(w, lines, q, n);
+  }
+  std::string describe() const override {
+    std::string s = "rader n=" + std::to_string(n) + " lines=" + std::to_string(batch) +
+                    (cyclic ? " cyclic" : " linear") + " cl=" + std::to_string(cl);
+    return s + (fused ? " fused " + describe_sub(cl, sp) : " " + chain.describe());
   }
 
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
@@ -1257,7 +1328,7 @@ class RaderEngine : public Engine {
       a.conj_out = co;
       a.scale = scale;
       a.f = sp.f;
-      HIP_TRY(launch_conv(prec, kMaxP, a, sp.threads, c.s));
+      HIP_TRY(launch_conv(prec, sp.maxp, a, sp.threads, c.s));
       return GOFFT_OK;
     }
     ConvChain::Hooks h;
@@ -1337,8 +1408,7 @@ class BluesteinEngine : public Engine {
     }
     DevInfo dev;
     RC_TRY(current_device(&dev));
-    const i64 pts = dev.lds_bytes / (i64)elt(prec);
-    const i64 target = std::min<i64>(pts, std::max<i64>(pitch(m), soft_pts(prec)));
+    const i64 pts = dev.lds_full / (i64)elt(prec);  // k_conv: no static LDS
     int C = 0;
     if (faithful) {
       if (pitch(m) <= std::min<i64>(pts, soft_pts(prec))) {
@@ -1356,7 +1426,7 @@ class BluesteinEngine : public Engine {
       RC_TRY(chain.upload_spec(hF, spec));
       return chain.upload_spec(hI, specI);
     }
-    if (pitch(m) <= pts && pick_cols((int)m, 16, target, sp, &C)) {
+    if (fused_conv_plan(m, prec, pts, sp)) {
       fused = true;
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
@@ -1376,6 +1446,11 @@ class BluesteinEngine : public Engine {
     if (lines <= 0) return;
     if (fused) w.add(GOFFT_KERNEL_CONV, 1, 2 * n * lines * (i64)elt(prec));
     else chain.work(w, lines, n, n);
+  }
+  std::string describe() const override {
+    std::string s = "bluestein n=" + std::to_string(n) + " lines=" + std::to_string(batch) +
+                    (faithful ? " reference-twiddles" : "") + " m=" + std::to_string(m);
+    return s + (fused ? " fused " + describe_sub(m, sp) : " " + chain.describe());
   }
 
   int run(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
@@ -1407,7 +1482,7 @@ class BluesteinEngine : public Engine {
       a.conj_out = co;
       a.scale = scale;
       a.f = sp.f;
-      HIP_TRY(launch_conv(prec, kMaxP, a, sp.threads, c.s));
+      HIP_TRY(launch_conv(prec, sp.maxp, a, sp.threads, c.s));
       return GOFFT_OK;
     }
     ConvChain::Hooks h;
@@ -1510,6 +1585,11 @@ class RealEngine {
   int launches() const {
     if (single) return 1;
     return half->launches() + (even ? 1 : 2);
+  }
+  std::string describe() const {
+    std::string s = "real n=" + std::to_string(n) + " lines=" + std::to_string(batch);
+    if (single) return s + " fused " + describe_sub(m, sp);
+    return s + (even ? " packed: " : " promoted: ") + half->describe();
   }
   // One forward (mode 0 bins, 1 power) or inverse (bins read per line) run.
   void work(Work& w, int inverse, int mode, i64 bins) const {
@@ -2046,6 +2126,35 @@ int gofft_plan_num_launches(const gofft_plan* p, int direction) {
     case K_SPEC: return p->frames > 0 ? p->real->launches() + (p->spec_gather ? 1 : 0) : 0;
   }
   return -1;
+}
+
+int64_t gofft_plan_describe(const gofft_plan* p, char* buf, int64_t len) {
+  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  std::string s;
+  auto line = [&](const std::string& t) { s += t + "\n"; };
+  switch (p->kind) {
+    case K_C2C: line(p->eng->describe()); break;
+    case K_MANY:
+      for (const auto& g : p->groups) line(g->eng->describe());
+      break;
+    case K_R2C:
+    case K_PSD:
+    case K_SPEC:
+      if (p->real) line(p->real->describe());
+      break;
+    case K_ND_C2C:
+    case K_ND_R2C:
+      if (p->nd_row) line("rows: " + p->nd_row->describe());
+      for (const auto& a : p->axes)
+        if (a.eng) line(std::string(a.gather ? "axis (gathered): " : "axis: ") + a.eng->describe());
+      break;
+  }
+  if (buf && len > 0) {
+    const size_t k = std::min<size_t>((size_t)len - 1, s.size());
+    std::memcpy(buf, s.data(), k);
+    buf[k] = 0;
+  }
+  return (int64_t)s.size() + 1;
 }
 
 int gofft_plan_kernel_work(const gofft_plan* p, int direction, int kernel, int64_t* launches,

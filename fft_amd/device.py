@@ -58,6 +58,13 @@ class _Plan:
     def num_launches(self, direction=0):
         return L.lib().gofft_plan_num_launches(self._h, direction)
 
+    def describe(self):
+        """The plan's engines and launches as text."""
+        n = L.lib().gofft_plan_describe(self._h, None, 0)
+        buf = ctypes.create_string_buffer(max(1, n))
+        L.lib().gofft_plan_describe(self._h, buf, n)
+        return buf.value.decode()
+
     def kernel_work(self, kernel, direction=0):
         """(launches, data bytes) of kernel family `kernel` (L.KERNEL_*) in one exec."""
         n, b = ctypes.c_int64(), ctypes.c_int64()

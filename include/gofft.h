@@ -122,6 +122,12 @@ int64_t gofft_plan_workspace_bytes(const gofft_plan* plan);
 /* Number of kernel launches one forward execution enqueues. */
 int gofft_plan_num_launches(const gofft_plan* plan, int direction);
 
+/* Text description of the plan's engines and launches (pass lengths, columns
+   and threads per workgroup, LDS stage radices), one line per engine, written
+   NUL-terminated into buf (truncated to len). Returns the length needed,
+   terminator included. No reference counterpart (diagnostics). */
+int64_t gofft_plan_describe(const gofft_plan* plan, char* buf, int64_t len);
+
 /* Kernel families counted by gofft_plan_kernel_work. */
 enum {
   GOFFT_KERNEL_PASS = 0, /* Stockham pass kernel (one pass of a transform or convolution) */
