@@ -1071,6 +1071,11 @@ struct ConvChain {
     } else {
       std::vector<i64> lens;
       RC_TRY(fwd.multi_lengths(lens));
+      // The middle launch runs the forward's last pass length twice (forward
+      // and inverse): give it the length with the fewest LDS stages.
+      auto stages = [](i64 R) { return radix_schedule((int)R).size(); };
+      auto mid = std::min_element(lens.begin(), lens.end(), [&](i64 x, i64 y) { return stages(x) < stages(y); });
+      std::iter_swap(mid, lens.end() - 1);
       RC_TRY(fwd.init_lengths(lens));
       std::reverse(lens.begin(), lens.end());
       RC_TRY(inv.init_lengths(lens));
