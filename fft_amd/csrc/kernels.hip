@@ -389,7 +389,10 @@ extern __shared__ unsigned char g_dyn_lds[];
 // ---------------------------------------------------------------------------
 // Generic pass: load C columns (optional pass twiddle W_M^(r*k), optional
 // conjugation), R-point FFT in LDS, store (optional conjugation and scale).
-template <typename T, int MAXP>
+// EXT: the convolution-chain features (load/store hooks, the middle launch's
+// spectrum multiply and second FFT). Plain passes use the form without them,
+// which compiles to fewer registers.
+template <typename T, int MAXP, bool EXT>
 __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
@@ -397,6 +400,9 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   __shared__ long long s_mk[kMaxCols], s_ln[kMaxCols];
   __shared__ T2 s_x0[kMaxCols];
   const SubFFT& f = a.f;
+  const int in_hook = EXT ? a.in_hook : 0;
+  const int out_hook = EXT ? a.out_hook : 0;
+  const void* const mid_spec = EXT ? a.mid_spec : nullptr;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int C = f.C, R = f.R, Rp = f.Rp;
   const long long g0 = (long long)blockIdx.x * C;
@@ -410,8 +416,8 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
       s_ob[tid] = lb;
       s_tk[tid] = k;
       s_ln[tid] = d3;
-      if (a.mid_spec) s_mk[tid] = col_digits(a.mm, g, nullptr, 0, nullptr, &lb, nullptr);
-      if (a.out_hook == 2) {
+      if (mid_spec) s_mk[tid] = col_digits(a.mm, g, nullptr, 0, nullptr, &lb, nullptr);
+      if (out_hook == 2) {
         const T2* xr = reinterpret_cast<const T2*>(a.rader_x);
         T2 x0 = xr[a.rader_x_off ? a.rader_x_off[d3] : d3 * a.rader_x_stride];
         if (a.x0_conj) x0.y = -x0.y;
@@ -446,7 +452,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   if (in_fixed) {
     const long long base = s_ib[fc];
     const long long il = s_il[fc];
-    if (base >= 0 && a.in_hook == 0) {
+    if (base >= 0 && in_hook == 0) {
       const T2* p = in + base + il + (long long)fr0 * a.im.g;
       const long long st = (long long)frs * a.im.g;
 #pragma unroll
@@ -458,7 +464,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
         buf[u] = mkc<T2>(0, 0);
         const long long i = il + (long long)r * a.im.g;
         if (base >= 0 && r < R && i < a.nvalid)
-          buf[u] = in[base + (a.in_hook == 2 ? (long long)a.hook_perm[i] : i)];
+          buf[u] = in[base + (in_hook == 2 ? (long long)a.hook_perm[i] : i)];
       }
     }
   } else {
@@ -472,10 +478,10 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
         const long long base = s_ib[c];
         if (base >= 0) {
           const long long i = s_il[c] + (long long)r * a.im.g;
-          if (a.in_hook == 0) {
+          if (in_hook == 0) {
             buf[u] = in[base + i];
           } else if (i < a.nvalid) {
-            buf[u] = in[base + (a.in_hook == 2 ? (long long)a.hook_perm[i] : i)];
+            buf[u] = in[base + (in_hook == 2 ? (long long)a.hook_perm[i] : i)];
           }
         }
       }
@@ -516,7 +522,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
         tw_w = dmul2(tw_w, tw_step);
       }
       if (a.conj_in) v.y = -v.y;
-      if (a.in_hook == 1 && s_ib[c] >= 0) {
+      if (in_hook == 1 && s_ib[c] >= 0) {
         const long long i = s_il[c] + (long long)r * a.im.g;
         if (i < a.nvalid) v = cmul(v, reinterpret_cast<const T2*>(a.hook_w)[i]);
       }
@@ -528,9 +534,9 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   }
   __syncthreads();
   lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw), (int)s_tk[tid >> f.log2tpc]);
-  if (a.mid_spec) {
+  if (mid_spec) {
     // forward output position k of point r of column c: s_mk[c] + r * mm.g
-    const T2* spec = reinterpret_cast<const T2*>(a.mid_spec);
+    const T2* spec = reinterpret_cast<const T2*>(mid_spec);
     const int c = tid >> f.log2tpc;
     const int lt = tid & (f.tpc - 1);
     if (s_ob[c] >= 0) {
@@ -547,7 +553,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   T2* out = reinterpret_cast<T2*>(a.out);
   const T sc = (T)a.scale;
   const bool do_scale = a.scale != 1.0;
-  if (out_fixed && a.out_hook == 0) {
+  if (out_fixed && out_hook == 0) {
     const long long base = s_ob[fc];
     if (base >= 0) {
       T2* po = out + base + s_ol[fc] + (long long)fr0 * a.om.g;
@@ -581,11 +587,11 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
       if (base >= 0) {
         T2 v = lds[c * Rp + lp(r, f.psh)];
         const long long p = s_ol[c] + (long long)r * a.om.g;
-        if (a.out_hook == 0) {
+        if (out_hook == 0) {
           if (a.conj_out) v.y = -v.y;
           if (do_scale) v = cscale(v, sc);
           out[base + p] = v;
-        } else if (a.out_hook == 1) {
+        } else if (out_hook == 1) {
           if (p < a.nvalid) {
             v = cmul(cconj(v), reinterpret_cast<const T2*>(a.hook_w)[p]);
             if (a.conj_out) v.y = -v.y;
@@ -1210,13 +1216,21 @@ template <typename T> static size_t elt2() { return 2 * sizeof(T); }
   } while (0)
 
 hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s) {
+  (void)maxp;
   long long groups = (a.ncols + a.f.C - 1) / a.f.C;
+  const bool ext = a.in_hook || a.out_hook || a.mid_spec;
   if (prec == 0) {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
-    GOFFT_DISPATCH_MAXP(k_pass, double, a, bytes, threads, groups, s);
+    if (ext)
+      hipLaunchKernelGGL((k_pass<double, 16, true>), dim3(groups), dim3(threads), bytes, s, a);
+    else
+      hipLaunchKernelGGL((k_pass<double, 16, false>), dim3(groups), dim3(threads), bytes, s, a);
   } else {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
-    GOFFT_DISPATCH_MAXP(k_pass, float, a, bytes, threads, groups, s);
+    if (ext)
+      hipLaunchKernelGGL((k_pass<float, 16, true>), dim3(groups), dim3(threads), bytes, s, a);
+    else
+      hipLaunchKernelGGL((k_pass<float, 16, false>), dim3(groups), dim3(threads), bytes, s, a);
   }
   return hipGetLastError();
 }
