@@ -319,14 +319,16 @@ __device__ __forceinline__ void lds_stage_rec(T2* lds, const SubFFT& f, const St
   __syncthreads();
 }
 
-// Runs the whole R-point sub-FFT (forward) on every column in LDS. cdig is the
+// Runs stages [s0, s1) of the R-point sub-FFT (forward) on every column in
+// LDS (all of them by default). cdig is the
 // calling thread's column twiddle digit (only read by stages with cmul != 0).
 template <typename T2, int MAXP>
-__device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw, int cdig = 0) {
+__device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw, int cdig = 0, int s0 = 0, int s1 = -1) {
   const int tid = threadIdx.x;
   const int c = tid >> f.log2tpc;
   const int lt = tid & (f.tpc - 1);
-  for (int si = 0; si < f.nstages; ++si) {
+  if (s1 < 0) s1 = f.nstages;
+  for (int si = s0; si < s1; ++si) {
     const Stage& s = f.st[si];
     switch (s.radix) {
       case 2: lds_stage<T2, 2, MAXP>(lds, f, s, stw, c, lt, cdig); break;
@@ -504,6 +506,14 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   const bool lds_aff = (frs & ((1 << f.psh) - 1)) == 0;
   T2* const lcol = lds + fc * Rp;
   const int lstep = frs + (frs >> f.psh);
+  // When a fixed-column thread's R / frs loaded points are exactly the inputs
+  // j + i * nb (nb = frs) of first-stage butterfly j = fr0, that stage runs
+  // from the load registers; likewise a last stage whose outputs k + i * ns
+  // (ns = frs) are the thread's store points writes straight to memory.
+  // (radix-16 stages only, so that the register arrays stay static)
+  const bool first_reg = !EXT && in_fixed && f.st[0].radix == 16 && R == 16 * frs;
+  const bool last_reg = !EXT && out_fixed && f.nstages >= 2 &&
+                        f.st[f.nstages - 1].radix == 16 && R == 16 * frs;
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
     const int fi = tid + u * nthr;
@@ -526,14 +536,25 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
         const long long i = s_il[c] + (long long)r * a.im.g;
         if (i < a.nvalid) v = cmul(v, reinterpret_cast<const T2*>(a.hook_w)[i]);
       }
-      if (in_fixed && lds_aff)
+      if (first_reg)
+        buf[u] = v;
+      else if (in_fixed && lds_aff)
         lcol[lp(fr0, f.psh) + u * lstep] = v;
       else
         lds[c * Rp + lp(r, f.psh)] = v;
     }
   }
+  if (first_reg) {
+    // stage 0 (ns = 1, no twiddles): outputs of butterfly fr0 at fr0 * 16 + i,
+    // inside one pad block
+    dft<16>(buf);
+    T2* p = lcol + lp(fr0 * 16, f.psh);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) p[i] = buf[i];
+  }
   __syncthreads();
-  lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw), (int)s_tk[tid >> f.log2tpc]);
+  lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw), (int)s_tk[tid >> f.log2tpc], first_reg ? 1 : 0,
+                    last_reg ? f.nstages - 1 : f.nstages);
   if (mid_spec) {
     // forward output position k of point r of column c: s_mk[c] + r * mm.g
     const T2* spec = reinterpret_cast<const T2*>(mid_spec);
@@ -553,6 +574,31 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   T2* out = reinterpret_cast<T2*>(a.out);
   const T sc = (T)a.scale;
   const bool do_scale = a.scale != 1.0;
+  if (last_reg) {
+    // last stage from LDS into registers, then straight to memory: butterfly
+    // j = fr0 (< ns), inputs fr0 + i * frs, outputs fr0 + i * frs
+    const Stage& sl = f.st[f.nstages - 1];
+    T2 v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = lds_aff ? lcol[lp(fr0, f.psh) + i * lstep] : lcol[lp(fr0 + i * frs, f.psh)];
+    const T2* tw = reinterpret_cast<const T2*>(a.stw) + sl.tw_off + fr0 * sl.kmul;
+#pragma unroll
+    for (int i = 1; i < 16; ++i) v[i] = cmul(v[i], tw[i - 1]);
+    dft<16>(v);
+    const long long base = s_ob[fc];
+    if (base >= 0) {
+      T2* po = out + base + s_ol[fc] + (long long)fr0 * a.om.g;
+      const long long st = (long long)frs * a.om.g;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        T2 w = v[i];
+        if (a.conj_out) w.y = -w.y;
+        if (do_scale) w = cscale(w, sc);
+        po[i * st] = w;
+      }
+    }
+    return;
+  }
   if (out_fixed && out_hook == 0) {
     const long long base = s_ob[fc];
     if (base >= 0) {
