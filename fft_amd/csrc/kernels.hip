@@ -188,11 +188,51 @@ template <typename T2> __device__ __forceinline__ void dft16(T2* v) {
   for (int i = 0; i < 16; ++i) v[i] = y[i];
 }
 
+template <int RAD, typename T2> __device__ __forceinline__ void dft(T2* v);
+
+constexpr int cinv(int a, int m) {  // a^-1 mod m (gcd(a, m) = 1)
+  int r = 1;
+  while ((a * r) % m != 1) ++r;
+  return r;
+}
+
+// Prime-factor (Good-Thomas) DFT of N = A*B, gcd(A, B) = 1, with no twiddles:
+// input n = (B*n1 + A*n2) mod N, output k = (k1*B*(B^-1 mod A) +
+// k2*A*(A^-1 mod B)) mod N; A-point DFTs over n1, then B-point DFTs over n2.
+template <int A, int B, typename T2> __device__ __forceinline__ void dft_pfa(T2* v) {
+  constexpr int N = A * B;
+  constexpr int EA = B * cinv(B % A, A) % N, EB = A * cinv(A % B, B) % N;
+  T2 t[N];
+#pragma unroll
+  for (int n2 = 0; n2 < B; ++n2) {
+    T2 a[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) a[n1] = v[(B * n1 + A * n2) % N];
+    dft<A>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) t[k1 * B + n2] = a[k1];
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < A; ++k1) {
+    T2 b[B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) b[n2] = t[k1 * B + n2];
+    dft<B>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) v[(k1 * EA + k2 * EB) % N] = b[k2];
+  }
+}
+
 template <int RAD, typename T2> __device__ __forceinline__ void dft(T2* v) {
   if constexpr (RAD == 2) dft2(v);
   else if constexpr (RAD == 4) dft4(v[0], v[1], v[2], v[3]);
   else if constexpr (RAD == 8) dft8(v);
   else if constexpr (RAD == 16) dft16(v);
+  else if constexpr (RAD == 6) dft_pfa<2, 3>(v);
+  else if constexpr (RAD == 10) dft_pfa<2, 5>(v);
+  else if constexpr (RAD == 12) dft_pfa<4, 3>(v);
+  else if constexpr (RAD == 14) dft_pfa<2, 7>(v);
+  else if constexpr (RAD == 15) dft_pfa<3, 5>(v);
   else dft_odd<RAD>(v);
 }
 
@@ -339,6 +379,10 @@ __device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw, int cdig = 0, i
       case 8: lds_stage<T2, 8, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       case 11: lds_stage<T2, 11, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       case 13: lds_stage<T2, 13, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      // composite prime-factor radices: planned for the 16-point form only
+      case 6: if constexpr (MAXP == 16) lds_stage<T2, 6, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 10: if constexpr (MAXP == 16) lds_stage<T2, 10, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 12: if constexpr (MAXP == 16) lds_stage<T2, 12, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       // grouped radix-2 layers: only the 16-point form is planned with them
       case 102: if constexpr (MAXP == 16) lds_stage_rec<T2, 2, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       case 104: if constexpr (MAXP == 16) lds_stage_rec<T2, 4, MAXP>(lds, f, s, stw, c, lt, cdig); break;

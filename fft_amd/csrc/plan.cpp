@@ -315,6 +315,65 @@ std::vector<int> radix_schedule(int R, int maxrad = 16) {
 }
 
 constexpr int kMaxP = 16;   // points per thread per stage (kernel template)
+// Radix-16 stages first, then the remaining factors in as few stages as the
+// composite prime-factor radices 6, 10 and 12 allow (kernels.hip dft_pfa);
+// empty when R has a prime factor above 13.
+std::vector<int> composite_schedule(int R) {
+  std::vector<int> r;
+  int x = R;
+  while (x % 16 == 0) {
+    r.push_back(16);
+    x /= 16;
+  }
+  int e = 0;
+  while (x % 2 == 0) {
+    e++;
+    x /= 2;
+  }
+  std::vector<int> odd;
+  for (int p : {3, 5, 7, 11, 13})
+    while (x % p == 0) {
+      odd.push_back(p);
+      x /= p;
+    }
+  if (x != 1) return {};
+  // Exhaustive search over the few remaining factors: e twos (e < 4) and
+  // the odd primes; radices 2, 4, 8, the odd primes, 6 = 2*3, 10 = 2*5,
+  // 12 = 4*3.
+  std::vector<int> best;
+  std::function<void(int, std::vector<int>, std::vector<int>)> go = [&](int e2, std::vector<int> rest,
+                                                                        std::vector<int> cur) {
+    if (!best.empty() && cur.size() >= best.size()) return;
+    if (e2 == 0 && rest.empty()) {
+      best = cur;
+      return;
+    }
+    if (!rest.empty()) {
+      const int p = rest.back();
+      rest.pop_back();
+      auto with = [&](int rad, int use2) {
+        std::vector<int> c2 = cur;
+        c2.push_back(rad);
+        go(e2 - use2, rest, c2);
+      };
+      with(p, 0);
+      if (p == 3 && e2 >= 1) with(6, 1);
+      if (p == 5 && e2 >= 1) with(10, 1);
+      if (p == 3 && e2 >= 2) with(12, 2);
+      return;
+    }
+    for (int k = std::min(e2, 3); k >= 1; k--) {
+      std::vector<int> c2 = cur;
+      c2.push_back(1 << k);
+      go(e2 - k, rest, c2);
+    }
+  };
+  go(e, odd, {});
+  std::sort(best.begin(), best.end(), [](int a, int b) { return a > b; });
+  r.insert(r.end(), best.begin(), best.end());
+  return r;
+}
+
 constexpr int kWideP = 24;  // the complex64 fused convolution's wide form
 
 // LDS column pitch for R points (kernels.hip: one pad slot per 16 points plus
@@ -358,27 +417,31 @@ bool make_sub(int R, int C, SubPlan& sp, int min_threads = 0, int sh = 4, int ma
     if (rads.empty()) return false;
   }
   if ((int)rads.size() > kMaxStages) return false;
+  // every thread holds at most maxp points per stage (and per load round of
+  // the pass kernels, which have one round)
+  auto feasible = [&](const std::vector<int>& rs, int T) {
+    const int tpc = T / C;
+    if (maxp == kMaxP && (i64)T * maxp < (i64)C * R) return false;
+    for (int rad : rs) {
+      const int nb = R / rad;
+      const int per = (nb + tpc - 1) / tpc;
+      const int cap = maxp / rad > 0 ? maxp / rad : 1;
+      if (rad > maxp || per > cap) return false;
+    }
+    return true;
+  };
   int T = 64;
   while (T < C || T < min_threads) T <<= 1;
   while ((i64)T * maxp < (i64)C * R && T < 1024) T <<= 1;
   bool ok = false;
-  for (; T <= 1024; T <<= 1) {
-    int tpc = T / C;
-    // every thread holds at most maxp points per stage (and per load round
-    // of the pass kernels, which have one round)
-    ok = maxp != kMaxP || (i64)T * maxp >= (i64)C * R;
-    for (int rad : rads) {
-      int nb = R / rad;
-      int per = (nb + tpc - 1) / tpc;
-      int cap = maxp / rad > 0 ? maxp / rad : 1;
-      if (rad > maxp || per > cap) {
-        ok = false;
-        break;
-      }
-    }
-    if (ok) break;
-  }
+  for (; T <= 1024; T <<= 1)
+    if ((ok = feasible(rads, T))) break;
   if (!ok) return false;
+  // Fewer stages with composite radices when they fit the same threads.
+  if (maxp == kMaxP && maxrad >= 16 && R > 1) {
+    std::vector<int> comp = composite_schedule(R);
+    if (!comp.empty() && comp.size() < rads.size() && feasible(comp, T)) rads = comp;
+  }
   sp = SubPlan();
   sp.threads = T;
   sp.maxp = maxp;
