@@ -1823,6 +1823,14 @@ struct gofft_plan {
   bool spec_gather = false;
   size_t ws_need = 0;
   mutable WorkspacePool pool;
+  // Mixed-length plans: streams (default priority) on which length groups run
+  // beside the caller's stream, joined back by events. lane_of[g] is group
+  // g's lane, 0 being the caller's stream.
+  std::vector<hipStream_t> lanes;
+  hipEvent_t fork_ev = nullptr;
+  std::vector<hipEvent_t> join_ev;
+  std::vector<int> lane_of;
+  mutable std::mutex lane_mu;  // one fork/join enqueue at a time
 
   int ws(hipStream_t s, char** out) const {
     if (ws_need == 0) {
@@ -1831,9 +1839,72 @@ struct gofft_plan {
     }
     return pool.get(s, ws_need, out);
   }
+
+  ~gofft_plan() {
+    for (hipStream_t l : lanes) (void)hipStreamDestroy(l);
+    for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+  }
 };
 
 namespace {
+
+// Lanes of a mixed-length plan (GOFFT_LANES streams, default 4, at most one
+// per group): each group, largest data first, goes to the lane with the least
+// data so far. Lane workspaces are allocated here, so that an exec enqueues
+// no allocation.
+int assign_lanes(gofft_plan& p) {
+  int want = 4;
+  if (const char* e = std::getenv("GOFFT_LANES")) want = std::max(1, std::min(16, std::atoi(e)));
+  const int n = (int)std::min<size_t>((size_t)want, p.groups.size());
+  p.lane_of.assign(p.groups.size(), 0);
+  if (n <= 1) return GOFFT_OK;
+  std::vector<double> load(n, 0.0);
+  for (size_t g = 0; g < p.groups.size(); g++) {
+    const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    p.lane_of[g] = k;
+    load[k] += (double)p.groups[g]->len * (double)p.groups[g]->eng->batch;
+  }
+  p.lanes.resize(n - 1, nullptr);
+  p.join_ev.resize(n - 1, nullptr);
+  for (int i = 0; i < n - 1; i++) {
+    HIP_TRY(hipStreamCreateWithFlags(&p.lanes[i], hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&p.join_ev[i], hipEventDisableTiming));
+    char* w = nullptr;
+    RC_TRY(p.ws(p.lanes[i], &w));
+  }
+  HIP_TRY(hipEventCreateWithFlags(&p.fork_ev, hipEventDisableTiming));
+  return GOFFT_OK;
+}
+
+// Runs every group of a mixed-length plan: on the caller's stream alone, or
+// forked over the plan's lanes and joined back.
+int run_groups(const gofft_plan* p, const void* src, void* dst, int ci, int co, bool inverse_scale,
+               hipStream_t s) {
+  auto one = [&](const gofft_plan::Group& g, hipStream_t st) -> int {
+    char* ws = nullptr;
+    RC_TRY(p->ws(st, &ws));
+    Ctx c{st, ws};
+    const i64* o = (const i64*)g.off.p;
+    return g.eng->run(src, LineIO{o, 0}, dst, LineIO{o, 0}, ci, co, inverse_scale ? 1.0 / (double)g.len : 1.0, c);
+  };
+  if (p->lanes.empty()) {
+    for (const auto& g : p->groups) RC_TRY(one(*g, s));
+    return GOFFT_OK;
+  }
+  std::lock_guard<std::mutex> lk(p->lane_mu);
+  HIP_TRY(hipEventRecord(p->fork_ev, s));
+  for (hipStream_t l : p->lanes) HIP_TRY(hipStreamWaitEvent(l, p->fork_ev, 0));
+  for (size_t g = 0; g < p->groups.size(); g++) {
+    const int k = p->lane_of[g];
+    RC_TRY(one(*p->groups[g], k == 0 ? s : p->lanes[k - 1]));
+  }
+  for (size_t i = 0; i < p->lanes.size(); i++) {
+    HIP_TRY(hipEventRecord(p->join_ev[i], p->lanes[i]));
+    HIP_TRY(hipStreamWaitEvent(s, p->join_ev[i], 0));
+  }
+  return GOFFT_OK;
+}
 
 int new_plan(gofft_plan** out, std::unique_ptr<gofft_plan>& p) {
   *out = p.release();
@@ -2015,6 +2086,7 @@ int gofft_plan_c2c_many(gofft_plan** plan, const int64_t* lengths, int64_t count
     p->ws_need = std::max(p->ws_need, g->eng->ws_bytes());
     p->groups.push_back(std::move(g));
   }
+  RC_TRY(assign_lanes(*p));
   return new_plan(plan, p);
 }
 
@@ -2298,14 +2370,7 @@ int gofft_exec_c2c(const gofft_plan* p, const void* src, void* dst, int directio
     return p->eng->run(src, LineIO{nullptr, p->n}, dst, LineIO{nullptr, p->n}, inv, inv,
                        inv ? 1.0 / (double)p->n : 1.0, c);
   }
-  if (p->kind == K_MANY) {
-    for (const auto& g : p->groups) {
-      Ctx c{s, ws};
-      const i64* o = (const i64*)g->off.p;
-      RC_TRY(g->eng->run(src, LineIO{o, 0}, dst, LineIO{o, 0}, inv, inv, inv ? 1.0 / (double)g->len : 1.0, c));
-    }
-    return GOFFT_OK;
-  }
+  if (p->kind == K_MANY) return run_groups(p, src, dst, inv, inv, inv, s);
   if (p->kind == K_ND_C2C) {
     if (p->nd_total * p->batch == 0) return GOFFT_OK;
     return run_nd_axes(p->axes, p->prec, p->nd_total * p->batch, src, dst, inv,
@@ -2405,12 +2470,7 @@ int gofft_vjp_c2c(const gofft_plan* p, const void* gy, void* gx, int direction, 
     }
     if (p->kind == K_MANY) {
       int adj = direction == GOFFT_FORWARD;
-      for (const auto& g : p->groups) {
-        Ctx c{s, ws};
-        const i64* o = (const i64*)g->off.p;
-        RC_TRY(g->eng->run(gy, LineIO{o, 0}, gx, LineIO{o, 0}, adj, adj, adj ? 1.0 : 1.0 / (double)g->len, c));
-      }
-      return GOFFT_OK;
+      return run_groups(p, gy, gx, adj, adj, !adj, s);
     }
     int adj = direction == GOFFT_FORWARD;
     return run_nd_axes(p->axes, p->prec, p->nd_total * p->batch, gy, gx, adj,
