@@ -14,16 +14,21 @@ dominant kernel (the Stockham pass kernel k_pass<float,16>) and a
 `cpu_baseline` object (the C++ restatement of the reference's CPU algorithm
 timed on this host).
 
-Roofline of the pass kernel: each length group of the step is also timed on
-its own, with the same warm-up and step count, through a plan of exactly that
-group; the groups whose launches are all pass-kernel launches give the
-kernel's time per step. The library reports each launch's data bytes (its
-input and output lines once each), so achieved = bytes / time. `traffic` (HBM
-bytes per launch from FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy run
-by the same command) is read from the committed counter summary when present.
+Roofline of the pass kernel: the step runs its length groups side by side on
+several streams, so its launches overlap. Each length group is therefore also
+timed on its own, on one stream, with the same warm-up and step count,
+through a plan of exactly that group; the groups whose launches are all
+pass-kernel launches give the kernel's time per step. The library reports
+each launch's data bytes (its input and output lines once each), so achieved
+= bytes / time. `trace_window` names those isolated dispatches among the run's
+k_pass<float, ...> dispatches, for comparison with a kernel trace. `traffic`
+(HBM bytes per launch from FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy
+run by the same command) is read from the committed counter summary when
+present.
 
-`python bench.py --summarize-trace STATS.csv` and `--summarize-pmc FETCH.csv
-WRITE.csv` print those summaries from rocprofv3's CSV output.
+`python bench.py --summarize-trace STATS.csv [SKIP TAKE]` and
+`--summarize-pmc FETCH.csv WRITE.csv` print those summaries from rocprofv3's
+CSV output.
 """
 import argparse
 import csv
@@ -42,7 +47,7 @@ LENGTHS = [1024, 4096, 65536, 6720, 1009, 10007, 20014]
 COUNT = 4096
 HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 1 << 30  # one direction of the calibration copy
-PASS_KERNEL = "k_pass<float, 16>"
+PASS_KERNEL = "k_pass<float, 16, *>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pass_f32_mixed4096_traffic.txt")
 
 
@@ -117,8 +122,9 @@ def run_dtype(D, torch, dev, name, lengths, x64, steps, warmup):
     return res
 
 
-def pass_roofline(r, L):
-    """Achieved bandwidth of the pass kernel over the step's pass-only groups."""
+def pass_roofline(r, L, steps, warmup):
+    """Achieved bandwidth of the pass kernel over the step's pass-only groups,
+    each timed alone on one stream."""
     ms = launches = nbytes = 0
     for g in r["groups"].values():
         w = g["work"]
@@ -140,6 +146,10 @@ def pass_roofline(r, L):
         "kernel": PASS_KERNEL, "launches_per_step": launches,
         "alg_bytes_per_launch": round(nbytes / launches), "avg_launch_ms": round(avg_ms, 5),
         "kernel_ms_per_step": round(ms, 4),
+        "measured_on": "each length group alone on one stream (the step overlaps them)",
+        # the run's k_pass<float, ...> dispatches: (warmup + steps) * launches
+        # in the step, then the same count in the per-group runs
+        "trace_window": {"skip": (steps + warmup) * step_launches, "take": (steps + warmup) * launches},
     }
 
 
@@ -324,16 +334,28 @@ WORKLOADS = {"roundtrip4096": workload_roundtrip4096, "spectral": workload_spect
              "blur": workload_blur}
 
 
-def summarize_trace(path):
-    """Text summary of rocprofv3 --stats kernel statistics (CSV)."""
+def summarize_trace(path, skip=None, take=None):
+    """Text summary of rocprofv3 --stats kernel statistics (CSV); with SKIP
+    and TAKE, also the average duration of k_pass<float, ...> dispatches
+    [SKIP, SKIP + TAKE) from the kernel trace beside it."""
     rows = list(csv.DictReader(open(path)))
-    print("%-58s %7s %12s %12s %7s" % ("kernel", "calls", "total_ms", "avg_us", "pct"))
+    print("%-62s %7s %12s %12s %7s" % ("kernel", "calls", "total_ms", "avg_us", "pct"))
     for r in rows:
         name = r["Name"]
-        if len(name) > 58:
-            name = name[:55] + "..."
-        print("%-58s %7s %12.3f %12.2f %7.2f" % (name, r["Calls"], float(r["TotalDurationNs"]) / 1e6,
+        if len(name) > 62:
+            name = name[:59] + "..."
+        print("%-62s %7s %12.3f %12.2f %7.2f" % (name, r["Calls"], float(r["TotalDurationNs"]) / 1e6,
                                                  float(r["AverageNs"]) / 1e3, float(r["Percentage"])))
+    if skip is None:
+        return
+    trace = os.path.join(os.path.dirname(path), os.path.basename(path).replace("kernel_stats", "kernel_trace"))
+    disp = [r for r in csv.DictReader(open(trace)) if "k_pass<float" in r["Kernel_Name"]]
+    disp.sort(key=lambda r: int(r["Dispatch_Id"]))
+    win = disp[skip: skip + take]
+    dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in win]
+    print()
+    print("k_pass<float, 16, *> dispatches %d..%d of %d (the per-group runs): average %.2f us"
+          % (skip, skip + len(win), len(disp), sum(dur) / max(1, len(dur))))
 
 
 def _pmc_rows(path, counter):
@@ -377,11 +399,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", choices=["mixed", "all"] + sorted(WORKLOADS), default="mixed",
                     help="mixed: the headline line; the others print one line per transform")
-    ap.add_argument("--summarize-trace", metavar="STATS_CSV")
+    ap.add_argument("--summarize-trace", nargs="+", metavar="STATS_CSV [SKIP TAKE]")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     args = ap.parse_args()
     if args.summarize_trace:
-        return summarize_trace(args.summarize_trace)
+        t = args.summarize_trace
+        return summarize_trace(t[0], *(int(v) for v in t[1:3])) if len(t) >= 3 else summarize_trace(t[0])
     if args.summarize_pmc:
         return summarize_pmc(*args.summarize_pmc)
     if args.gpus != 1:
@@ -405,7 +428,7 @@ def main():
 
     r32 = run_dtype(D, torch, dev, "fp32", lengths, x64, args.steps, args.warmup)
     r64 = run_dtype(D, torch, dev, "fp64", lengths, x64, args.steps, args.warmup)
-    roofline = pass_roofline(r32, D.L)
+    roofline = pass_roofline(r32, D.L, args.steps, args.warmup)
     copy = copy_bandwidth(torch, dev, args.steps, args.warmup)
     cpu = None if args.no_cpu else cpu_leg(lengths, x64)
 
