@@ -142,6 +142,8 @@ struct RealFwdArgs {
   int odd_n;                // unused for even n (reserved)
   double psd_scale;         // d / n
   double w_edge, w_mid;     // mode 2: DC/Nyquist bins times w_edge, others times w_mid
+  int vec_in;               // sample pairs are 2T-aligned (even stride, aligned base)
+  FastDiv nbd;              // division by R + 1 (bins per line)
   SubFFT f;
 };
 
@@ -157,6 +159,7 @@ struct RealInvArgs {
   long long out_stride;     // real samples between consecutive lines
   double scale;             // 1/m
   double w_mid;             // weight of the bins other than DC and Nyquist (1: IRFFT)
+  FastDiv halfd;            // division by R/2 + 1 (bin pairs per line)
   SubFFT f;
 };
 

@@ -731,7 +731,7 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
       const long long l = l0 + c;
       if (l < a.nlines) {
         const T* p = in + l * a.in_stride + 2 * (long long)r;
-        buf[u] = mkc<T2>(p[0], p[1]);
+        buf[u] = a.vec_in ? *reinterpret_cast<const T2*>(p) : mkc<T2>(p[0], p[1]);
       }
     }
   }
@@ -742,7 +742,10 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
       const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
       const int r = fi - c * R;
       T2 v = buf[u];
-      if (win) v = mkc<T2>(v.x * win[2 * r], v.y * win[2 * r + 1]);
+      if (win) {
+        const T2 w = reinterpret_cast<const T2*>(win)[r];
+        v = mkc<T2>(v.x * w.x, v.y * w.y);
+      }
       lds[c * Rp + lp(r, f.psh)] = v;
     }
   }
@@ -752,7 +755,7 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
   const int nb = R + 1;
   const int tot2 = C * nb;
   for (int fi = tid; fi < tot2; fi += nthr) {
-    int c = fi / nb;
+    int c = (int)fastdiv_q((uint32_t)fi, a.nbd);
     int k = fi - c * nb;
     long long l = l0 + c;
     if (l >= a.nlines) continue;
@@ -828,7 +831,7 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
   const int half = R / 2 + 1;
   const int tot2 = C * half;
   for (int fi = tid; fi < tot2; fi += nthr) {
-    int c = fi / half;
+    int c = (int)fastdiv_q((uint32_t)fi, a.halfd);
     int k = fi - c * half;
     long long l = l0 + c;
     T2* X = lds + c * Rp;

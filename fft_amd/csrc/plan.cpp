@@ -1698,6 +1698,8 @@ class RealEngine {
       a.psd_scale = psd_scale;
       a.w_edge = w_edge;
       a.w_mid = w_mid;
+      a.vec_in = istride % 2 == 0 && (uintptr_t)in % (2 * relt(prec)) == 0;
+      a.nbd = fastdiv_make((uint32_t)(sp.f.R + 1));
       a.f = sp.f;
       HIP_TRY(launch_real_fwd(prec, kMaxP, a, sp.threads, c.s));
       return GOFFT_OK;
@@ -1742,6 +1744,7 @@ class RealEngine {
       a.out_stride = ostride;
       a.scale = gain / (double)m;
       a.w_mid = w_mid;
+      a.halfd = fastdiv_make((uint32_t)(sp.f.R / 2 + 1));
       a.f = sp.f;
       HIP_TRY(launch_real_inv(prec, kMaxP, a, sp.threads, c.s));
       return GOFFT_OK;
