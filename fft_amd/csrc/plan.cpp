@@ -13,7 +13,9 @@
 #include <cmath>
 #include <complex>
 #include <cstdlib>
+#include <cstdio>
 #include <functional>
+#include <tuple>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -713,6 +715,11 @@ class Engine {
  public:
   int prec = 0;
   i64 n = 0, batch = 0;
+  // Set by the autotuner: the convolution length (0: the cost model's) and
+  // the columns per workgroup of multi-pass passes (0: the default).
+  i64 force_m = 0;
+  int force_cols = 0;
+  bool tuned = false;
   virtual ~Engine() {}
   virtual size_t ws_bytes() const = 0;
   virtual int launches() const = 0;
@@ -938,17 +945,13 @@ class StockhamEngine : public Engine {
     DevInfo dev;
     RC_TRY(current_device(&dev));
     const i64 pts = dev.lds_bytes / (i64)elt(prec);
-    int cmp = prec == 0 ? 8 : 16;
-    int tmin = 0;
-    if (const char* e = std::getenv("GOFFT_PASS_COLS")) cmp = std::max(1, std::atoi(e));
-    if (const char* e = std::getenv("GOFFT_PASS_THREADS")) tmin = std::max(0, std::atoi(e));
+    const int cmp = force_cols > 0 ? force_cols : prec == 0 ? 8 : 16;
     passes.clear();
     i64 Ns = 1;
     for (i64 R : lens) {
       Pass p;
       int Cp = 0;
       bool ok = pick_cols((int)R, cmp, pts, p.sp, &Cp);
-      if (ok && tmin > p.sp.threads) ok = make_sub((int)R, Cp, p.sp, tmin);
       if (!ok)
         return fail(GOFFT_ERR_UNSUPPORTED, "no LDS schedule for pass length " + std::to_string(R));
       p.R = R;
@@ -1030,7 +1033,8 @@ class StockhamEngine : public Engine {
     return s;
   }
   std::string describe() const override {
-    std::string s = "stockham n=" + std::to_string(n) + " lines=" + std::to_string(batch);
+    std::string s = std::string(tuned ? "tuned " : "") + "stockham n=" + std::to_string(n) +
+                    " lines=" + std::to_string(batch);
     if (st != 1) s += " stride=" + std::to_string(st);
     if (P() > 1) s += " chunk=" + std::to_string(chunk());
     return s + " passes=" + passes_text();
@@ -1111,7 +1115,7 @@ struct ConvChain {
     const void* spec = nullptr;
   };
 
-  int init(i64 m_, i64 cb_, int prec_, bool recurrence) {
+  int init(i64 m_, i64 cb_, int prec_, bool recurrence, int cols = 0) {
     m = m_;
     cb = cb_;
     prec = prec_;
@@ -1119,6 +1123,7 @@ struct ConvChain {
       e->prec = prec;
       e->n = m;
       e->batch = cb;
+      e->force_cols = cols;
     }
     if (recurrence) {
       DevInfo dev;
@@ -1263,7 +1268,16 @@ struct ConvChain {
 // Stockham decomposition, each weighted by its LDS stage count (measured on
 // MI355X: radix-16 passes run about 40% faster per byte than passes of four
 // small radices).
+std::vector<i64> conv_candidates(i64 lo, int prec, int k);
+
 i64 choose_conv_len(i64 lo, int prec) {
+  std::vector<i64> c = conv_candidates(lo, prec, 1);
+  return c.empty() ? -1 : c[0];
+}
+
+// The k best convolution lengths in [lo, ceil(1.4 lo)] by the cost model:
+// lengths that one workgroup holds first, then by cost.
+std::vector<i64> conv_candidates(i64 lo, int prec, int k) {
   DevInfo dev;
   current_device(&dev);
   const i64 pts = dev.lds_bytes / (i64)elt(prec);
@@ -1271,9 +1285,7 @@ i64 choose_conv_len(i64 lo, int prec) {
   const int cmp = prec == 0 ? 8 : 16;
   const i64 rmax = std::min<i64>(pts / cmp * 16 / 17, 1024);
   i64 hi = lo + (lo * 4 + 9) / 10;
-  i64 best = -1;
-  double bestc = 0;
-  int bestfit = 0;
+  std::vector<std::tuple<int, double, i64>> all;  // (-fit, cost, m)
   for (i64 m = lo; m <= hi; m++) {
     i64 x = m;
     for (int p : {2, 3, 5, 7})
@@ -1292,13 +1304,12 @@ i64 choose_conv_len(i64 lo, int prec) {
       cost = 0;
       for (i64 R : lens) cost += (double)m * (1.0 + 0.25 * (double)radix_schedule((int)R).size());
     }
-    if (best < 0 || fit > bestfit || (fit == bestfit && cost < bestc)) {
-      best = m;
-      bestc = cost;
-      bestfit = fit;
-    }
+    all.emplace_back(-fit, cost, m);
   }
-  return best;
+  std::sort(all.begin(), all.end());
+  std::vector<i64> out;
+  for (size_t i = 0; i < all.size() && (int)out.size() < k; i++) out.push_back(std::get<2>(all[i]));
+  return out;
 }
 
 // Rader's algorithm for prime n (rader.go:39-172): gather by the generator
@@ -1325,7 +1336,7 @@ class RaderEngine : public Engine {
       cl = q;
     } else {
       cyclic = false;
-      cl = choose_conv_len(2 * q, prec);
+      cl = force_m > 0 ? force_m : choose_conv_len(2 * q, prec);
     }
     std::vector<int> ph(q), iph(q);
     i64 x = 1;
@@ -1351,7 +1362,7 @@ class RaderEngine : public Engine {
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
       cb = chunk_lines(2 * cl * (i64)elt(prec), batch);
-      RC_TRY(chain.init(cl, cb, prec, false));
+      RC_TRY(chain.init(cl, cb, prec, false, force_cols));
       RC_TRY(chain.upload_spec(b, spec));
     }
     return GOFFT_OK;
@@ -1368,7 +1379,7 @@ class RaderEngine : public Engine {
     else chain.work(w, lines, q, n);
   }
   std::string describe() const override {
-    std::string s = "rader n=" + std::to_string(n) + " lines=" + std::to_string(batch) +
+    std::string s = std::string(tuned ? "tuned " : "") + "rader n=" + std::to_string(n) + " lines=" + std::to_string(batch) +
                     (cyclic ? " cyclic" : " linear") + " cl=" + std::to_string(cl);
     return s + (fused ? " fused " + describe_sub(cl, sp) : " " + chain.describe());
   }
@@ -1440,7 +1451,7 @@ class BluesteinEngine : public Engine {
       m = 1;
       while (m < 2 * n - 1) m <<= 1;
     } else {
-      m = choose_conv_len(2 * n - 1, prec);
+      m = force_m > 0 ? force_m : choose_conv_len(2 * n - 1, prec);
     }
     std::vector<cld> hF, hI;  // kernel spectra (1/m folded in), natural order
     if (faithful) {
@@ -1499,7 +1510,7 @@ class BluesteinEngine : public Engine {
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
       cb = chunk_lines(2 * m * (i64)elt(prec), batch);
-      RC_TRY(chain.init(m, cb, prec, false));
+      RC_TRY(chain.init(m, cb, prec, false, force_cols));
       RC_TRY(chain.upload_spec(hF, spec));
     }
     return GOFFT_OK;
@@ -1516,7 +1527,7 @@ class BluesteinEngine : public Engine {
     else chain.work(w, lines, n, n);
   }
   std::string describe() const override {
-    std::string s = "bluestein n=" + std::to_string(n) + " lines=" + std::to_string(batch) +
+    std::string s = std::string(tuned ? "tuned " : "") + "bluestein n=" + std::to_string(n) + " lines=" + std::to_string(batch) +
                     (faithful ? " reference-twiddles" : "") + " m=" + std::to_string(m);
     return s + (fused ? " fused " + describe_sub(m, sp) : " " + chain.describe());
   }
@@ -1569,7 +1580,8 @@ class BluesteinEngine : public Engine {
   }
 };
 
-std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, int* rc) {
+std::unique_ptr<Engine> make_engine_fixed(i64 n, i64 batch, int prec, unsigned flags, int* rc, i64 force_m,
+                                          int force_cols) {
   *rc = GOFFT_OK;
   int r = route(n);
   if (r == 0) {
@@ -1585,6 +1597,8 @@ std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, 
     e->prec = prec;
     e->n = n;
     e->batch = batch;
+    e->force_m = force_m;
+    e->force_cols = force_cols;
     *rc = e->init(flags);
     return *rc == GOFFT_OK ? std::move(e) : nullptr;
   }
@@ -1593,6 +1607,8 @@ std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, 
     e->prec = prec;
     e->n = n;
     e->batch = batch;
+    e->force_m = force_m;
+    e->force_cols = force_cols;
     *rc = e->init(flags);
     return *rc == GOFFT_OK ? std::move(e) : nullptr;
   }
@@ -1600,8 +1616,153 @@ std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, 
   e->prec = prec;
   e->n = n;
   e->batch = batch;
+  e->force_m = force_m;
+  e->force_cols = force_cols;
   *rc = e->init(flags);
   return *rc == GOFFT_OK ? std::move(e) : nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner (GOFFT_FLAG_AUTOTUNE, or GOFFT_AUTOTUNE=1): where the planner has
+// a choice that the device decides best, the candidates are built and timed
+// on the current device, and the fastest kept. Choices: the convolution
+// length of Rader's linear case and of Bluestein outside the reference-
+// twiddle mode (the cost model's three best, the reference's bestConvLen
+// question, plan.go:192-259), and the columns per workgroup of multi-pass
+// transforms and convolution chains. Results are cached per (route, length,
+// lines, precision) for the process, and in the file GOFFT_TUNE_CACHE names
+// (one "route n lines prec m cols" line each) across processes.
+
+struct TuneChoice {
+  i64 m = 0;
+  int cols = 0;
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, i64, i64, int>, TuneChoice> g_tune;
+bool g_tune_loaded = false;
+
+bool autotune_on(unsigned flags) {
+  if (flags & GOFFT_FLAG_AUTOTUNE) return true;
+  const char* e = std::getenv("GOFFT_AUTOTUNE");
+  return e && std::atoi(e) != 0;
+}
+
+void tune_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char* path = std::getenv("GOFFT_TUNE_CACHE");
+  if (!path) return;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return;
+  long long r, n, b, pr, m, c;
+  while (std::fscanf(f, "%lld %lld %lld %lld %lld %lld", &r, &n, &b, &pr, &m, &c) == 6)
+    g_tune[std::make_tuple((int)r, (i64)n, (i64)b, (int)pr)] = TuneChoice{(i64)m, (int)c};
+  std::fclose(f);
+}
+
+void tune_store_locked(const std::tuple<int, i64, i64, int>& key, const TuneChoice& c) {
+  g_tune[key] = c;
+  const char* path = std::getenv("GOFFT_TUNE_CACHE");
+  if (!path) return;
+  if (FILE* f = std::fopen(path, "a")) {
+    std::fprintf(f, "%d %lld %lld %d %lld %d\n", std::get<0>(key), (long long)std::get<1>(key),
+                 (long long)std::get<2>(key), std::get<3>(key), (long long)c.m, c.cols);
+    std::fclose(f);
+  }
+}
+
+// Average milliseconds of one run of e over `lines` lines of zeros.
+int time_engine(const Engine& e, i64 lines, float* ms) {
+  DevBuf in, out, ws;  // freed after the stream below is drained
+  hipStream_t s = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct Cleanup {
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Cleanup() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  } cl{s};
+  const size_t bytes = (size_t)e.n * (size_t)lines * elt(e.prec);
+  RC_TRY(in.alloc(bytes));
+  RC_TRY(out.alloc(bytes));
+  if (e.ws_bytes()) RC_TRY(ws.alloc(e.ws_bytes()));
+  HIP_TRY(launch_fill(e.prec, in.p, 2 * (long long)e.n * lines, 0.0, s));
+  Ctx c{s, (char*)ws.p};
+  for (int i = 0; i < 2; i++) RC_TRY(e.run(in.p, LineIO{nullptr, e.n}, out.p, LineIO{nullptr, e.n}, 0, 0, 1.0, c, lines));
+  HIP_TRY(hipEventCreate(&cl.a));
+  HIP_TRY(hipEventCreate(&cl.b));
+  const int reps = 5;
+  HIP_TRY(hipEventRecord(cl.a, s));
+  for (int i = 0; i < reps; i++)
+    RC_TRY(e.run(in.p, LineIO{nullptr, e.n}, out.p, LineIO{nullptr, e.n}, 0, 0, 1.0, c, lines));
+  HIP_TRY(hipEventRecord(cl.b, s));
+  HIP_TRY(hipEventSynchronize(cl.b));
+  float t = 0;
+  HIP_TRY(hipEventElapsedTime(&t, cl.a, cl.b));
+  *ms = t / reps;
+  return GOFFT_OK;
+}
+
+std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, int* rc) {
+  const int r = route(n);
+  const bool faithful_blue =
+      r == 4 && prec == GOFFT_C128 && (flags & GOFFT_FLAG_REFERENCE_TWIDDLES) && !(flags & GOFFT_FLAG_ACCURATE_TWIDDLES);
+  const bool conv = (r == 3 && !factors_are_small(n - 1)) || (r == 4 && !faithful_blue);
+  if (!autotune_on(flags) || batch <= 0 || r == 0 || faithful_blue || (!conv && r != 1 && r != 2))
+    return make_engine_fixed(n, batch, prec, flags, rc, 0, 0);
+  const auto key = std::make_tuple(r, n, batch, prec);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      auto e = make_engine_fixed(n, batch, prec, flags, rc, it->second.m, it->second.cols);
+      if (e) e->tuned = true;
+      return e;
+    }
+  }
+  // Candidates: (conv length, columns); columns only matter for multi-pass
+  // passes and chains, so a single-pass candidate is tried once.
+  std::vector<TuneChoice> cand;
+  const std::vector<int> cols = prec == GOFFT_C64 ? std::vector<int>{16, 8, 32} : std::vector<int>{8, 4, 16};
+  if (conv) {
+    const i64 lo = r == 3 ? 2 * (n - 1) : 2 * n - 1;
+    for (i64 m : conv_candidates(lo, prec, 3))
+      for (int c : cols) cand.push_back(TuneChoice{m, c});
+  } else {
+    for (int c : cols) cand.push_back(TuneChoice{0, c});
+  }
+  // Time at most 512 MB of lines, or all of them.
+  const i64 lines = std::max<i64>(1, std::min<i64>(batch, ((i64)512 << 20) / std::max<i64>(1, n * (i64)elt(prec))));
+  std::unique_ptr<Engine> best;
+  TuneChoice bestc;
+  float bestms = 0;
+  std::vector<std::string> seen;
+  for (const TuneChoice& c : cand) {
+    int rc2 = GOFFT_OK;
+    auto e = make_engine_fixed(n, batch, prec, flags, &rc2, c.m, c.cols);
+    if (!e) continue;
+    const std::string d = e->describe();
+    if (std::find(seen.begin(), seen.end(), d) != seen.end()) continue;  // same plan as a tried one
+    seen.push_back(d);
+    float ms = 0;
+    if (time_engine(*e, lines, &ms) != GOFFT_OK) continue;
+    if (!best || ms < bestms) {
+      best = std::move(e);
+      bestc = c;
+      bestms = ms;
+    }
+  }
+  if (!best) return make_engine_fixed(n, batch, prec, flags, rc, 0, 0);
+  best->tuned = true;
+  *rc = GOFFT_OK;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_store_locked(key, bestc);
+  return best;
 }
 
 // ---------------------------------------------------------------------------

@@ -56,6 +56,13 @@ extern "C" {
 #define GOFFT_FLAG_REFERENCE_TWIDDLES 1u
 /* Force accurate (directly evaluated) twiddles everywhere. */
 #define GOFFT_FLAG_ACCURATE_TWIDDLES 2u
+/* Time the planner's alternatives on the current device at plan creation and
+ * keep the fastest: convolution lengths of Rader and Bluestein (the role of
+ * the reference's bestConvLen, plan.go:192-259) and columns per workgroup of
+ * multi-pass transforms. Results are cached per (route, length, lines,
+ * precision), and in the file named by GOFFT_TUNE_CACHE if set. The env var
+ * GOFFT_AUTOTUNE=1 sets this flag for every plan. No reference counterpart. */
+#define GOFFT_FLAG_AUTOTUNE 4u
 
 /* N-D kinds */
 #define GOFFT_ND_C2C 0
