@@ -714,58 +714,56 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
   const SubFFT& f = a.f;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int C = f.C, R = f.R, Rp = f.Rp;
-  const long long l0 = (long long)blockIdx.x * C;
-  const T* in = reinterpret_cast<const T*>(a.in);
-  const T* win = reinterpret_cast<const T*>(a.window);
-  const int total = C * R;
+  const int tid = threadIdx.x;
+  const int R = f.R, Rp = f.Rp, tpc = f.tpc, sh = f.psh;
+  // Each thread keeps one line: c = tid / tpc, points r = lt + u * tpc
+  // (adjacent lanes read adjacent sample pairs).
+  const int c = tid >> f.log2tpc;
+  const int lt = tid & (tpc - 1);
+  const long long l = (long long)blockIdx.x * f.C + c;
+  const bool live = l < a.nlines;
+  const T* p = reinterpret_cast<const T*>(a.in) + (live ? l * a.in_stride : 0) + 2 * (long long)lt;
+  const T2* win = reinterpret_cast<const T2*>(a.window);
+  T2* const col = lds + c * Rp;
   T2 buf[MAXP];
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
-    const int fi = tid + u * nthr;
+    const int r = lt + u * tpc;
     buf[u] = mkc<T2>(0, 0);
-    if (fi < total) {
-      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-      const int r = fi - c * R;
-      const long long l = l0 + c;
-      if (l < a.nlines) {
-        const T* p = in + l * a.in_stride + 2 * (long long)r;
-        buf[u] = a.vec_in ? *reinterpret_cast<const T2*>(p) : mkc<T2>(p[0], p[1]);
-      }
+    if (live && r < R) {
+      const T* q = p + 2 * (long long)u * tpc;
+      buf[u] = a.vec_in ? *reinterpret_cast<const T2*>(q) : mkc<T2>(q[0], q[1]);
     }
   }
+  // lp(lt + u * tpc) is affine in u when tpc is a whole number of pad blocks
+  const bool aff = (tpc & ((1 << sh) - 1)) == 0;
+  const int step = tpc + (tpc >> sh);
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
-    const int fi = tid + u * nthr;
-    if (fi < total) {
-      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-      const int r = fi - c * R;
+    const int r = lt + u * tpc;
+    if (r < R) {
       T2 v = buf[u];
       if (win) {
-        const T2 w = reinterpret_cast<const T2*>(win)[r];
+        const T2 w = win[r];
         v = mkc<T2>(v.x * w.x, v.y * w.y);
       }
-      lds[c * Rp + lp(r, f.psh)] = v;
+      if (aff)
+        col[lp(lt, sh) + u * step] = v;
+      else
+        col[lp(r, sh)] = v;
     }
   }
   __syncthreads();
   lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
+  if (!live) return;
   const T2* utw = reinterpret_cast<const T2*>(a.utw);
-  const int nb = R + 1;
-  const int tot2 = C * nb;
-  for (int fi = tid; fi < tot2; fi += nthr) {
-    int c = (int)fastdiv_q((uint32_t)fi, a.nbd);
-    int k = fi - c * nb;
-    long long l = l0 + c;
-    if (l >= a.nlines) continue;
-    const T2* Z = lds + c * Rp;
+  for (int k = lt; k <= R; k += tpc) {
     T2 X;
     if (k == 0 || k == R) {
-      T2 z0 = Z[0];
+      T2 z0 = col[0];
       X = mkc<T2>(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0);
     } else {
-      T2 zk = Z[lp(k, f.psh)], zmk = Z[lp(R - k, f.psh)];
+      T2 zk = col[lp(k, sh)], zmk = col[lp(R - k, sh)];
       T xer = (zk.x + zmk.x) * (T)0.5;
       T xei = (zk.y - zmk.y) * (T)0.5;
       T xor_ = (zk.y + zmk.y) * (T)0.5;
@@ -782,9 +780,9 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
       reinterpret_cast<T2*>(a.out)[l * a.out_stride + k] =
           edge ? mkc<T2>(X.x * (T)a.w_edge, 0) : cscale(X, (T)a.w_mid);
     } else {
-      T p = (X.x * X.x + X.y * X.y) * (T)a.psd_scale;
-      if (k != 0 && k != R) p *= (T)2;
-      reinterpret_cast<T*>(a.out)[l * a.out_stride + k] = p;
+      T pw = (X.x * X.x + X.y * X.y) * (T)a.psd_scale;
+      if (k != 0 && k != R) pw *= (T)2;
+      reinterpret_cast<T*>(a.out)[l * a.out_stride + k] = pw;
     }
   }
 }
@@ -797,81 +795,72 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
   const SubFFT& f = a.f;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int C = f.C, R = f.R, Rp = f.Rp;
-  const long long l0 = (long long)blockIdx.x * C;
-  const T2* in = reinterpret_cast<const T2*>(a.in);
+  const int tid = threadIdx.x;
+  const int R = f.R, Rp = f.Rp, tpc = f.tpc, sh = f.psh;
+  // Each thread keeps one line: c = tid / tpc, points r = lt + u * tpc.
+  const int c = tid >> f.log2tpc;
+  const int lt = tid & (tpc - 1);
+  const long long l = (long long)blockIdx.x * f.C + c;
+  const bool live = l < a.nlines;
+  const T2* in = reinterpret_cast<const T2*>(a.in) + (live ? l * a.in_stride : 0);
   const T2* utw = reinterpret_cast<const T2*>(a.utw);
-  const int total = C * R;
+  T2* const X = lds + c * Rp;
   T2 buf[MAXP];
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
-    const int fi = tid + u * nthr;
+    const int r = lt + u * tpc;
     buf[u] = mkc<T2>(0, 0);
-    if (fi < total) {
-      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-      const int r = fi - c * R;
-      const long long l = l0 + c;
-      if (l < a.nlines && r < a.in_bins) buf[u] = in[l * a.in_stride + r];
-      if (r > 0) buf[u] = cscale(buf[u], (T)a.w_mid);
-    }
+    if (live && r < R && r < a.in_bins) buf[u] = in[r];
+    if (r > 0) buf[u] = cscale(buf[u], (T)a.w_mid);
   }
+  const bool aff = (tpc & ((1 << sh) - 1)) == 0;
+  const int step = tpc + (tpc >> sh);
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
-    const int fi = tid + u * nthr;
-    if (fi < total) {
-      const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-      const int r = fi - c * R;
-      lds[c * Rp + lp(r, f.psh)] = buf[u];
+    const int r = lt + u * tpc;
+    if (r < R) {
+      if (aff)
+        X[lp(lt, sh) + u * step] = buf[u];
+      else
+        X[lp(r, sh)] = buf[u];
     }
   }
   __syncthreads();
   // Pairs (k, R-k) for k = 0 .. R/2, each owned by one thread; k = 0 pairs DC
   // with the Nyquist bin X[R] read from memory.
-  const int half = R / 2 + 1;
-  const int tot2 = C * half;
-  for (int fi = tid; fi < tot2; fi += nthr) {
-    int c = (int)fastdiv_q((uint32_t)fi, a.halfd);
-    int k = fi - c * half;
-    long long l = l0 + c;
-    T2* X = lds + c * Rp;
+  for (int k = lt; k <= R / 2; k += tpc) {
     if (k == 0) {
       T x0 = X[0].x;
       T xm = 0;
-      if (l < a.nlines && R < a.in_bins) xm = in[l * a.in_stride + R].x;
+      if (live && R < a.in_bins) xm = in[R].x;
       // conj for the inverse-by-forward trick
       X[0] = mkc<T2>((x0 + xm) * (T)0.5, -((x0 - xm) * (T)0.5));
       continue;
     }
-    int mk_ = R - k;
-    T2 xk = X[lp(k, f.psh)], xmk = X[lp(mk_, f.psh)];
+    const int mk_ = R - k;
+    T2 xk = X[lp(k, sh)], xmk = X[lp(mk_, sh)];
     // Z[k] = xe + i*xo with xe = (Xk + conj Xmk)/2, d = (Xk - conj Xmk)/2,
     // xo = conj(W^k) d; Z[m-k] symmetric.
     T2 w = utw[k];
-    {
-      T xer = (xk.x + xmk.x) * (T)0.5, xei = (xk.y - xmk.y) * (T)0.5;
-      T dr = (xk.x - xmk.x) * (T)0.5, di = (xk.y + xmk.y) * (T)0.5;
-      T xor_ = w.x * dr + w.y * di, xoi = w.x * di - w.y * dr;
-      T2 zk = mkc<T2>(xer - xoi, xei + xor_);
-      if (k != mk_) {
-        T2 zmk = mkc<T2>(xer + xoi, -(xei - xor_));
-        X[lp(mk_, f.psh)] = cconj(zmk);
-      }
-      X[lp(k, f.psh)] = cconj(zk);
+    T xer = (xk.x + xmk.x) * (T)0.5, xei = (xk.y - xmk.y) * (T)0.5;
+    T dr = (xk.x - xmk.x) * (T)0.5, di = (xk.y + xmk.y) * (T)0.5;
+    T xor_ = w.x * dr + w.y * di, xoi = w.x * di - w.y * dr;
+    T2 zk = mkc<T2>(xer - xoi, xei + xor_);
+    if (k != mk_) {
+      T2 zmk = mkc<T2>(xer + xoi, -(xei - xor_));
+      X[lp(mk_, sh)] = cconj(zmk);
     }
+    X[lp(k, sh)] = cconj(zk);
   }
   __syncthreads();
   lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
-  T* out = reinterpret_cast<T*>(a.out);
+  if (!live) return;
+  // out_stride is even (n = 2R), so each sample pair is one aligned T2
+  T2* out = reinterpret_cast<T2*>(reinterpret_cast<T*>(a.out) + l * a.out_stride);
   const T sc = (T)a.scale;
-  for (int fi = tid; fi < total; fi += nthr) {
-    int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-    int r = fi - c * R;
-    long long l = l0 + c;
-    if (l >= a.nlines) continue;
-    T2 z = lds[c * Rp + lp(r, f.psh)];
-    // out_stride is even (n = 2R), so each sample pair is one aligned T2
-    *reinterpret_cast<T2*>(out + l * a.out_stride + 2 * (long long)r) = mkc<T2>(z.x * sc, -z.y * sc);
+  for (int r = lt; r < R; r += tpc) {
+    const T2 z = X[lp(r, sh)];
+    out[r] = mkc<T2>(z.x * sc, -z.y * sc);
   }
 }
 
