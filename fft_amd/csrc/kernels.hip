@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   const int in_hook = EXT ? a.in_hook : 0;
   const int out_hook = EXT ? a.out_hook : 0;
   const void* const mid_spec = EXT ? a.mid_spec : nullptr;
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int tid = threadIdx.x;
   const int C = f.C, R = f.R, Rp = f.Rp;
   const long long g0 = (long long)blockIdx.x * C;
   if (tid < C) {
@@ -484,115 +484,90 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   }
   __syncthreads();
   const T2* in = reinterpret_cast<const T2*>(a.in);
-  const int total = C * R;  // <= MAXP * blockDim.x (guaranteed by the planner)
-  // Walking across columns, or along a single column, a thread keeps one
-  // column c = tid mod C and visits r = tid / C + u * (nthr / C): its
-  // addresses are one base plus a fixed step.
-  const int fc = tid & (C - 1);
-  const int fr0 = tid >> f.log2C, frs = nthr >> f.log2C;
-  const bool in_fixed = !a.im.rowmajor || C == 1;
-  const bool out_fixed = !a.om.rowmajor || C == 1;
+  // Every thread keeps one column and visits its points r0 + u * tpc
+  // (R <= MAXP * tpc, guaranteed by the planner), so its addresses are one
+  // base plus a fixed step. Walking across columns (adjacent columns
+  // contiguous) the column is tid mod C and r0 = tid / C; walking along
+  // columns (a column's points contiguous) the column is tid / tpc and
+  // r0 = tid mod tpc. Loads and stores each pick the mapping of their layout.
+  const int frs = f.tpc;
+  const int ic = a.im.rowmajor ? tid >> f.log2tpc : tid & (C - 1);
+  const int ir0 = a.im.rowmajor ? tid & (frs - 1) : tid >> f.log2C;
+  const int oc = a.om.rowmajor ? tid >> f.log2tpc : tid & (C - 1);
+  const int or0 = a.om.rowmajor ? tid & (frs - 1) : tid >> f.log2C;
   // Issue every global load of this thread before using any of them, so that
   // MAXP loads per lane are in flight at once.
   T2 buf[MAXP];
-  if (in_fixed) {
-    const long long base = s_ib[fc];
-    const long long il = s_il[fc];
+  {
+    const long long base = s_ib[ic];
+    const long long il = s_il[ic];
     if (base >= 0 && in_hook == 0) {
-      const T2* p = in + base + il + (long long)fr0 * a.im.g;
+      const T2* p = in + base + il + (long long)ir0 * a.im.g;
       const long long st = (long long)frs * a.im.g;
 #pragma unroll
-      for (int u = 0; u < MAXP; ++u) buf[u] = fr0 + u * frs < R ? p[u * st] : mkc<T2>(0, 0);
+      for (int u = 0; u < MAXP; ++u) buf[u] = ir0 + u * frs < R ? p[u * st] : mkc<T2>(0, 0);
     } else {
 #pragma unroll
       for (int u = 0; u < MAXP; ++u) {
-        const int r = fr0 + u * frs;
+        const int r = ir0 + u * frs;
         buf[u] = mkc<T2>(0, 0);
         const long long i = il + (long long)r * a.im.g;
         if (base >= 0 && r < R && i < a.nvalid)
           buf[u] = in[base + (in_hook == 2 ? (long long)a.hook_perm[i] : i)];
       }
     }
-  } else {
-#pragma unroll
-    for (int u = 0; u < MAXP; ++u) {
-      const int fi = tid + u * nthr;
-      buf[u] = mkc<T2>(0, 0);
-      if (fi < total) {
-        const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-        const int r = fi - c * R;
-        const long long base = s_ib[c];
-        if (base >= 0) {
-          const long long i = s_il[c] + (long long)r * a.im.g;
-          if (in_hook == 0) {
-            buf[u] = in[base + i];
-          } else if (i < a.nvalid) {
-            buf[u] = in[base + (in_hook == 2 ? (long long)a.hook_perm[i] : i)];
-          }
-        }
-      }
-    }
   }
-  // Pass twiddles W_M^(r*k): twiddled passes walk across columns, so this
-  // thread's column is fixed (c = tid mod C) and r steps by tpc; one base and
-  // one step lookup, then an FP64 running product.
+  // Pass twiddles W_M^(r*k) for this thread's column k digit and points
+  // r = ir0 + u * tpc: one base and one step lookup, then an FP64 running
+  // product.
   double2 tw_w = make_double2(1.0, 0.0), tw_step = make_double2(1.0, 0.0);
   if (a.tw_M) {
-    const long long k = s_tk[tid & (C - 1)];
-    const long long r0 = tid >> f.log2C;
+    const long long k = s_tk[ic];
     if (k) {
-      tw_w = pass_twiddle(a.ptw_hi, a.ptw_lo, r0 * k, a.tw_shift);
-      tw_step = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)f.tpc * k, a.tw_shift);
+      tw_w = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)ir0 * k, a.tw_shift);
+      tw_step = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)frs * k, a.tw_shift);
     }
   }
-  // The LDS slots of a fixed column: lp(fr0 + u * frs) is affine in u when
-  // frs is a whole number of pad blocks.
+  // The LDS slots of a column's points r0 + u * tpc are affine in u when tpc
+  // is a whole number of pad blocks.
   const bool lds_aff = (frs & ((1 << f.psh) - 1)) == 0;
-  T2* const lcol = lds + fc * Rp;
   const int lstep = frs + (frs >> f.psh);
-  // When a fixed-column thread's R / frs loaded points are exactly the inputs
-  // j + i * nb (nb = frs) of first-stage butterfly j = fr0, that stage runs
-  // from the load registers; likewise a last stage whose outputs k + i * ns
-  // (ns = frs) are the thread's store points writes straight to memory.
-  // (radix-16 stages only, so that the register arrays stay static)
-  const bool first_reg = !EXT && in_fixed && f.st[0].radix == 16 && R == 16 * frs;
-  const bool last_reg = !EXT && out_fixed && f.nstages >= 2 &&
-                        f.st[f.nstages - 1].radix == 16 && R == 16 * frs;
+  T2* const icol = lds + ic * Rp;
+  T2* const ocol = lds + oc * Rp;
+  // When a thread's loaded points are exactly the inputs j + i * nb (nb = tpc)
+  // of first-stage butterfly j = r0, that stage runs from the load registers;
+  // likewise a last stage whose outputs k + i * ns (ns = tpc) are the
+  // thread's store points writes straight to memory (radix-16 stages only,
+  // so that the register arrays stay static).
+  const bool first_reg = !EXT && f.st[0].radix == 16 && R == 16 * frs;
+  const bool last_reg = !EXT && f.nstages >= 2 && f.st[f.nstages - 1].radix == 16 && R == 16 * frs;
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
-    const int fi = tid + u * nthr;
-    if (fi < total) {
-      int c, r;
-      if (in_fixed) {
-        c = fc;
-        r = fr0 + u * frs;
-      } else {
-        c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-        r = fi - c * R;
-      }
+    const int r = ir0 + u * frs;
+    if (r < R) {
       T2 v = buf[u];
       if (a.tw_M) {
         v = cmul(v, from_d2<T2>(tw_w));
         tw_w = dmul2(tw_w, tw_step);
       }
       if (a.conj_in) v.y = -v.y;
-      if (in_hook == 1 && s_ib[c] >= 0) {
-        const long long i = s_il[c] + (long long)r * a.im.g;
+      if (in_hook == 1 && s_ib[ic] >= 0) {
+        const long long i = s_il[ic] + (long long)r * a.im.g;
         if (i < a.nvalid) v = cmul(v, reinterpret_cast<const T2*>(a.hook_w)[i]);
       }
       if (first_reg)
         buf[u] = v;
-      else if (in_fixed && lds_aff)
-        lcol[lp(fr0, f.psh) + u * lstep] = v;
+      else if (lds_aff)
+        icol[lp(ir0, f.psh) + u * lstep] = v;
       else
-        lds[c * Rp + lp(r, f.psh)] = v;
+        icol[lp(r, f.psh)] = v;
     }
   }
   if (first_reg) {
-    // stage 0 (ns = 1, no twiddles): outputs of butterfly fr0 at fr0 * 16 + i,
+    // stage 0 (ns = 1, no twiddles): outputs of butterfly ir0 at ir0 * 16 + i,
     // inside one pad block
     dft<16>(buf);
-    T2* p = lcol + lp(fr0 * 16, f.psh);
+    T2* p = icol + lp(ir0 * 16, f.psh);
 #pragma unroll
     for (int i = 0; i < 16; ++i) p[i] = buf[i];
   }
@@ -618,20 +593,20 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   T2* out = reinterpret_cast<T2*>(a.out);
   const T sc = (T)a.scale;
   const bool do_scale = a.scale != 1.0;
+  const long long obase = s_ob[oc];
   if (last_reg) {
     // last stage from LDS into registers, then straight to memory: butterfly
-    // j = fr0 (< ns), inputs fr0 + i * frs, outputs fr0 + i * frs
+    // j = or0 (< ns), inputs and outputs or0 + i * tpc
     const Stage& sl = f.st[f.nstages - 1];
     T2 v[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = lds_aff ? lcol[lp(fr0, f.psh) + i * lstep] : lcol[lp(fr0 + i * frs, f.psh)];
-    const T2* tw = reinterpret_cast<const T2*>(a.stw) + sl.tw_off + fr0 * sl.kmul;
+    for (int i = 0; i < 16; ++i) v[i] = lds_aff ? ocol[lp(or0, f.psh) + i * lstep] : ocol[lp(or0 + i * frs, f.psh)];
+    const T2* tw = reinterpret_cast<const T2*>(a.stw) + sl.tw_off + or0 * sl.kmul;
 #pragma unroll
     for (int i = 1; i < 16; ++i) v[i] = cmul(v[i], tw[i - 1]);
     dft<16>(v);
-    const long long base = s_ob[fc];
-    if (base >= 0) {
-      T2* po = out + base + s_ol[fc] + (long long)fr0 * a.om.g;
+    if (obase >= 0) {
+      T2* po = out + obase + s_ol[oc] + (long long)or0 * a.om.g;
       const long long st = (long long)frs * a.om.g;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
@@ -643,63 +618,46 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
     }
     return;
   }
-  if (out_fixed && out_hook == 0) {
-    const long long base = s_ob[fc];
-    if (base >= 0) {
-      T2* po = out + base + s_ol[fc] + (long long)fr0 * a.om.g;
-      const long long st = (long long)frs * a.om.g;
+  if (obase < 0) return;
+  if (out_hook == 0) {
+    T2* po = out + obase + s_ol[oc] + (long long)or0 * a.om.g;
+    const long long st = (long long)frs * a.om.g;
 #pragma unroll
-      for (int u = 0; u < MAXP; ++u) {
-        const int r = fr0 + u * frs;
-        if (r < R) {
-          T2 v = lds_aff ? lcol[lp(fr0, f.psh) + u * lstep] : lcol[lp(r, f.psh)];
-          if (a.conj_out) v.y = -v.y;
-          if (do_scale) v = cscale(v, sc);
-          po[u * st] = v;
-        }
+    for (int u = 0; u < MAXP; ++u) {
+      const int r = or0 + u * frs;
+      if (r < R) {
+        T2 v = lds_aff ? ocol[lp(or0, f.psh) + u * lstep] : ocol[lp(r, f.psh)];
+        if (a.conj_out) v.y = -v.y;
+        if (do_scale) v = cscale(v, sc);
+        po[u * st] = v;
       }
     }
     return;
   }
+  const T2 x0 = out_hook == 2 ? s_x0[oc] : mkc<T2>(0, 0);
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
-    const int fi = tid + u * nthr;
-    if (fi < total) {
-      int c, r;
-      if (out_fixed) {
-        c = fc;
-        r = fr0 + u * frs;
-      } else {
-        c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-        r = fi - c * R;
-      }
-      const long long base = s_ob[c];
-      if (base >= 0) {
-        T2 v = lds[c * Rp + lp(r, f.psh)];
-        const long long p = s_ol[c] + (long long)r * a.om.g;
-        if (out_hook == 0) {
+    const int r = or0 + u * frs;
+    if (r < R) {
+      T2 v = ocol[lp(r, f.psh)];
+      const long long p = s_ol[oc] + (long long)r * a.om.g;
+      if (out_hook == 1) {
+        if (p < a.nvalid) {
+          v = cmul(cconj(v), reinterpret_cast<const T2*>(a.hook_w)[p]);
           if (a.conj_out) v.y = -v.y;
-          if (do_scale) v = cscale(v, sc);
-          out[base + p] = v;
-        } else if (out_hook == 1) {
-          if (p < a.nvalid) {
-            v = cmul(cconj(v), reinterpret_cast<const T2*>(a.hook_w)[p]);
-            if (a.conj_out) v.y = -v.y;
-            out[base + p] = cscale(v, sc);
-          }
-        } else {
-          const T2 x0 = s_x0[c];
-          const long long idx = p - a.conv_off;
-          if (idx >= 0 && idx < a.nvalid) {
-            T2 y = cadd(x0, cconj(v));
-            if (a.conj_out) y.y = -y.y;
-            out[base + a.hook_perm[idx]] = cscale(y, sc);
-          }
-          if (p == 0) {
-            T2 y = cadd(x0, reinterpret_cast<const T2*>(a.dc)[s_ln[c]]);
-            if (a.conj_out) y.y = -y.y;
-            out[base] = cscale(y, sc);
-          }
+          out[obase + p] = cscale(v, sc);
+        }
+      } else {
+        const long long idx = p - a.conv_off;
+        if (idx >= 0 && idx < a.nvalid) {
+          T2 y = cadd(x0, cconj(v));
+          if (a.conj_out) y.y = -y.y;
+          out[obase + a.hook_perm[idx]] = cscale(y, sc);
+        }
+        if (p == 0) {
+          T2 y = cadd(x0, reinterpret_cast<const T2*>(a.dc)[s_ln[oc]]);
+          if (a.conj_out) y.y = -y.y;
+          out[obase] = cscale(y, sc);
         }
       }
     }
