@@ -829,31 +829,33 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
   const SubFFT& f = a.f;
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int tid = threadIdx.x;
   const int C = f.C, R = f.R, Rp = f.Rp;
   const long long l0 = (long long)blockIdx.x * C;
   const T2* in = reinterpret_cast<const T2*>(a.in);
   const T2* chirp = reinterpret_cast<const T2*>(a.chirp);
   const int n = a.n;
   const int q = n - 1;
-  const int total = C * R;
-  // Rounds of LB buffered loads per thread (one round unless C * R exceeds
-  // 16 * blockDim.x, which only the 24-point form allows).
-  constexpr int LB = 16;
-  const int rounds = MAXP == LB ? 1 : (total + LB * nthr - 1) / (LB * nthr);
-  for (int rd = 0; rd < rounds; ++rd) {
-    const int base = rd * LB * nthr;
-    T2 buf[LB];
+  // Each thread keeps one line, c = tid / tpc, and loads its points
+  // r = lt + u * tpc in rounds of LB buffered loads (one round unless R
+  // exceeds 16 * tpc, which only the 24-point form allows).
+  const int c = tid >> f.log2tpc;
+  const int lt = tid & (f.tpc - 1);
+  const int tpc = f.tpc;
+  {
+    constexpr int LB = 16;
+    const long long l = l0 + c;
+    const bool live = l < a.nlines;
+    const T2* x = in + (live ? (a.in_off ? a.in_off[l] : l * a.in_stride) : 0);
+    const int rounds = MAXP == LB ? 1 : (R + LB * tpc - 1) / (LB * tpc);
+    for (int rd = 0; rd < rounds; ++rd) {
+      const int r0 = lt + rd * LB * tpc;
+      T2 buf[LB];
 #pragma unroll
-    for (int u = 0; u < LB; ++u) {
-      const int fi = base + tid + u * nthr;
-      buf[u] = mkc<T2>(0, 0);
-      if (fi < total) {
-        const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-        const int r = fi - c * R;
-        const long long l = l0 + c;
-        if (l < a.nlines) {
-          const T2* x = in + (a.in_off ? a.in_off[l] : l * a.in_stride);
+      for (int u = 0; u < LB; ++u) {
+        const int r = r0 + u * tpc;
+        buf[u] = mkc<T2>(0, 0);
+        if (live && r < R) {
           if (a.kind == 2) {
             if (r < n) buf[u] = x[r];
           } else if (r < q) {
@@ -861,23 +863,19 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
           }
         }
       }
-    }
 #pragma unroll
-    for (int u = 0; u < LB; ++u) {
-      const int fi = base + tid + u * nthr;
-      if (fi < total) {
-        const int c = (int)fastdiv_q((uint32_t)fi, f.Rd);
-        const int r = fi - c * R;
-        T2 v = buf[u];
-        if (a.conj_in) v.y = -v.y;
-        if (a.kind == 2 && r < n) v = cmul(v, chirp[r]);
-        lds[c * Rp + lp(r, f.psh)] = v;
+      for (int u = 0; u < LB; ++u) {
+        const int r = r0 + u * tpc;
+        if (r < R) {
+          T2 v = buf[u];
+          if (a.conj_in) v.y = -v.y;
+          if (a.kind == 2 && r < n) v = cmul(v, chirp[r]);
+          lds[c * Rp + lp(r, f.psh)] = v;
+        }
       }
     }
   }
   __syncthreads();
-  const int c = tid >> f.log2tpc;
-  const int lt = tid & (f.tpc - 1);
   const T2* stw = reinterpret_cast<const T2*>(a.stw);
   lds_fft<T2, MAXP>(lds, f, stw);
   // Rader DC: X[0] = x[0] + sum of the gathered samples, and that sum is the
