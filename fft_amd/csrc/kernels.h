@@ -125,6 +125,7 @@ struct PassArgs {
   const void* mid_spec;
   ColMap mm;
   SubFFT f2;
+  const void* stw2;         // stage twiddles of f2
 };
 
 // Real-input forward (RFFT / PSD / spectrogram), single pass: each column is

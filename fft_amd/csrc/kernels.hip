@@ -308,10 +308,11 @@ constexpr int clog2(int x) { return x <= 1 ? 0 : 1 + clog2(x / 2); }
 
 // TT = 2^t consecutive radix-2 layers of the reference's iterative transform
 // (kernels.Radix2), done in registers with the reference's per-layer twiddle
-// tables. Layer u pairs registers (i, i + TT/2^(u+1)) in place, using the
-// table of layer length L = Nsg*ns*2^(u+1) at entry (k + v*ns)*Nsg + cdig,
+// tables. Layer u pairs registers (i, i + TT/2^(u+1)) in place, using entry
+// (k + v*ns)*Nsg + cdig of the table of layer length L = Nsg*ns*2^(u+1),
 // v = bitrev_u(i >> (t-u)); register i ends as output bitrev_t(i). This is
 // the same set of butterflies, operands and twiddles as t separate layers.
+// The entry is read from a copy laid out per column digit (make_sub_r2).
 template <typename T2, int TT, int MAXP>
 __device__ __forceinline__ void lds_stage_rec(T2* lds, const SubFFT& f, const Stage& s, const T2* stw,
                                               int c, int lt, int cdig) {
@@ -331,12 +332,13 @@ __device__ __forceinline__ void lds_stage_rec(T2* lds, const SubFFT& f, const St
 #pragma unroll
       for (int u = 0; u < t; ++u) {
         const int h = TT >> (u + 1);
-        const int off = ((s.tw_off + 1) << u) - 1;
+        // layer u's table (see make_sub_r2): column digit block, then i
+        const int off = s.tw_off + Nsg * ns * ((1 << u) - 1) + cdig * (ns << u);
 #pragma unroll
         for (int i = 0; i < TT; ++i) {
           if (i & h) continue;
           const int vv = cbrev(i >> (t - u), u);
-          const T2 w = stw[off + (k + vv * ns) * Nsg + cdig];
+          const T2 w = stw[off + k + vv * ns];
           const T2 b = cmul(v[m][i + h], w);
           v[m][i + h] = csub(v[m][i], b);
           v[m][i] = cadd(v[m][i], b);
@@ -588,7 +590,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
       }
     }
     __syncthreads();
-    lds_fft<T2, MAXP>(lds, a.f2, reinterpret_cast<const T2*>(a.stw), 0);
+    lds_fft<T2, MAXP>(lds, a.f2, reinterpret_cast<const T2*>(a.stw2), 0);
   }
   T2* out = reinterpret_cast<T2*>(a.out);
   const T sc = (T)a.scale;

@@ -516,20 +516,29 @@ bool make_sub_r2(int R, int C, i64 Ns, SubPlan& sp) {
   f.log2tpc = ilog2(f.tpc);
   f.nstages = (int)ts.size();
   f.Rd = fastdiv_make((uint32_t)R);
+  // Twiddles of layer u of a stage (local size ns, layer length
+  // L = Ns*ns*2^(u+1)): entry i*Ns + cdig of the reference's table for L,
+  // read at tw_off + Ns*ns*(2^u - 1) + cdig*ns*2^u + i (kernels.hip
+  // lds_stage_rec). With Ns = 1 that is the shared table of every layer
+  // (layer L at L/2 - 1); a pass with Ns > 1 reads its own copy laid out so
+  // (rec_pass_table), adjacent butterflies reading adjacent entries.
   int ns = 1;
+  i64 base = 0;
   for (size_t i = 0; i < ts.size(); i++) {
     Stage& g = f.st[i];
     g.radix = 100 + (1 << ts[i]);
     g.ns = ns;
     g.nb = R >> ts[i];
-    g.tw_off = (int)(Ns * ns - 1);
+    g.tw_off = (int)(Ns == 1 ? ns - 1 : base);
     g.kmul = (int)Ns;
     g.cmul = 1;
     g.nsd = fastdiv_make((uint32_t)ns);
+    base += Ns * ns * (((i64)1 << ts[i]) - 1);
     ns <<= ts[i];
   }
   return true;
 }
+
 
 // Go's math.Sin / math.Cos (Go standard library math/sin.go, the Cephes
 // sin.c algorithm of S. L. Moshier) for |x| < 2^29. The reference's Bluestein
@@ -787,6 +796,22 @@ class TrivialEngine : public Engine {
   }
 };
 
+// The twiddle copy of a recurrence pass with Ns > 1 (make_sub_r2): for each
+// stage and layer, entry cdig*ns*2^u + i is the reference's w_L[i*Ns + cdig]
+// (`full` holds layer L at offset L/2 - 1).
+void rec_pass_table(const std::vector<dcx>& full, const SubPlan& sp, i64 Ns, std::vector<dcx>& out) {
+  for (int si = 0; si < sp.f.nstages; si++) {
+    const Stage& g = sp.f.st[si];
+    const int t = ilog2(g.radix - 100);
+    for (int u = 0; u < t; u++) {
+      const i64 S = (i64)g.ns << u;  // entries per column digit
+      const i64 L = Ns * S * 2;
+      for (i64 cd = 0; cd < Ns; cd++)
+        for (i64 i = 0; i < S; i++) out.push_back(full[(size_t)(L / 2 - 1 + i * Ns + cd)]);
+    }
+  }
+}
+
 // One pass of a global Stockham decomposition (or the single pass).
 struct Pass {
   SubPlan sp;
@@ -896,7 +921,16 @@ class StockhamEngine : public Engine {
       p.tw_M = 0;
       p.tw_digit = 1;
     }
-    return upload_dcx(stw, recurrence_table(n), prec);
+    std::vector<dcx> tab = recurrence_table(n);
+    const size_t nfull = tab.size();
+    std::vector<dcx> extra;
+    for (auto& p : passes) {
+      if (p.Ns == 1) continue;
+      p.stw_off = nfull + extra.size();
+      rec_pass_table(tab, p.sp, p.Ns, extra);
+    }
+    tab.insert(tab.end(), extra.begin(), extra.end());
+    return upload_dcx(stw, tab, prec);
   }
 
   int init(unsigned flags) {
@@ -1219,6 +1253,7 @@ struct ConvChain {
         a.om = pi.om;
         a.mid_spec = h.spec;
         a.f2 = pi.sp.f;
+        a.stw2 = (const char*)inv.stw.p + pi.stw_off * elt(prec);
         if (h.out_hook == 2) a.dc = dc;
         threads = pf.sp.threads;
       } else {
