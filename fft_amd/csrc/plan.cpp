@@ -2767,6 +2767,28 @@ int gofft_exec_c2r_host(const gofft_plan* p, const void* src, int64_t bins, void
                         [&](const void* i, void* o) { return gofft_exec_c2r(p, i, bins, o, nullptr); });
 }
 
+int gofft_vjp_c2c_host(const gofft_plan* p, const void* gy, void* gx, int direction) {
+  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  size_t b = (size_t)gofft_plan_in_elems(p, direction) * elt(p->prec);
+  return host_roundtrip(b, b, gy, gx, [&](const void* i, void* o) {
+    return gofft_vjp_c2c(p, i, o, direction, nullptr);
+  });
+}
+
+int gofft_vjp_r2c_host(const gofft_plan* p, const void* gy, void* gx) {
+  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  size_t ib = p->n > 0 ? (size_t)(p->n / 2 + 1) * p->batch * elt(p->prec) : 0;
+  size_t ob = (size_t)p->n * p->batch * relt(p->prec);
+  return host_roundtrip(ib, ob, gy, gx, [&](const void* i, void* o) { return gofft_vjp_r2c(p, i, o, nullptr); });
+}
+
+int gofft_vjp_c2r_host(const gofft_plan* p, const void* gx, void* gX) {
+  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  size_t ib = (size_t)p->n * p->batch * relt(p->prec);
+  size_t ob = p->n > 0 ? (size_t)(p->n / 2 + 1) * p->batch * elt(p->prec) : 0;
+  return host_roundtrip(ib, ob, gx, gX, [&](const void* i, void* o) { return gofft_vjp_c2r(p, i, o, nullptr); });
+}
+
 int gofft_sync(void) {
   HIP_TRY(hipDeviceSynchronize());
   return GOFFT_OK;

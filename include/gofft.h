@@ -192,6 +192,11 @@ int gofft_vjp_nd_r2c(const gofft_plan* plan, const void* g_out, void* g_in, int 
 int gofft_exec_c2c_host(const gofft_plan* plan, const void* src, void* dst, int direction);
 int gofft_exec_r2c_host(const gofft_plan* plan, const void* src, void* dst);
 int gofft_exec_c2r_host(const gofft_plan* plan, const void* src, int64_t bins, void* dst);
+/* Host-staged VJPs (no reference counterpart): as gofft_vjp_c2c, _r2c and
+   _c2r, with host buffers. */
+int gofft_vjp_c2c_host(const gofft_plan* plan, const void* gy, void* gx, int direction);
+int gofft_vjp_r2c_host(const gofft_plan* plan, const void* gy, void* gx);
+int gofft_vjp_c2r_host(const gofft_plan* plan, const void* gx, void* gX);
 
 /* Synchronise the device (for callers without their own stream handling). */
 int gofft_sync(void);

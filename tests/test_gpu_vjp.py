@@ -213,3 +213,29 @@ def test_vjp_fftn_adjoint_identity(dev):
             lhs = np.vdot(gy, fx)
             rhs = np.vdot(gx, x)
             assert abs(lhs - rhs) <= 1e-12 * abs(lhs) * math.log2(n), (shape, inv)
+
+
+def test_host_staged_vjps(dev):
+    # The host-staged VJP entry points agree with the device-resident ones.
+    import ctypes
+    from fft_amd import _lib as L
+    D, torch = dev
+    n, batch = 1000, 3
+    nb = n // 2 + 1
+    gy = cgauss(batch * nb, seed=4)
+    gx = rgauss(batch * n, seed=5)
+    gc = cgauss(batch * n, seed=6)
+    p = D.R2CPlan(n, batch, torch.float64)
+    out = np.zeros(batch * n)
+    L.check(L.lib().gofft_vjp_r2c_host(p.handle, gy.ctypes.data_as(ctypes.c_void_p),
+                                       out.ctypes.data_as(ctypes.c_void_p)))
+    assert np.array_equal(out, p.vjp_forward(torch.from_numpy(gy).to("cuda")).cpu().numpy())
+    outc = np.zeros(batch * nb, dtype=np.complex128)
+    L.check(L.lib().gofft_vjp_c2r_host(p.handle, gx.ctypes.data_as(ctypes.c_void_p),
+                                       outc.ctypes.data_as(ctypes.c_void_p)))
+    assert np.array_equal(outc, p.vjp_inverse(torch.from_numpy(gx).to("cuda")).cpu().numpy())
+    pc = D.C2CPlan(n, batch, torch.complex128)
+    outg = np.zeros(batch * n, dtype=np.complex128)
+    L.check(L.lib().gofft_vjp_c2c_host(pc.handle, gc.ctypes.data_as(ctypes.c_void_p),
+                                       outg.ctypes.data_as(ctypes.c_void_p), L.FORWARD))
+    assert np.array_equal(outg, pc.vjp(torch.from_numpy(gc).to("cuda")).cpu().numpy())
