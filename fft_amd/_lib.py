@@ -73,6 +73,9 @@ SIGNATURES = {
     "gofft_vjp_c2c_host": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "gofft_vjp_r2c_host": (ctypes.c_int, [_vp, _vp, _vp]),
     "gofft_vjp_c2r_host": (ctypes.c_int, [_vp, _vp, _vp]),
+    "gofft_graph_c2c": (ctypes.c_int, [_pp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "gofft_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "gofft_graph_destroy": (ctypes.c_int, [_vp]),
     "gofft_sync": (ctypes.c_int, []),
 }
 

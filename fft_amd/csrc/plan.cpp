@@ -2789,6 +2789,55 @@ int gofft_vjp_c2r_host(const gofft_plan* p, const void* gx, void* gX) {
   return host_roundtrip(ib, ob, gx, gX, [&](const void* i, void* o) { return gofft_vjp_c2r(p, i, o, nullptr); });
 }
 
+// A captured exec: the launches (and, for mixed-length plans, the stream
+// fork and join) of one gofft_exec_c2c on fixed buffers, replayed as one
+// hipGraph. The graph uses the plan's workspace of the capture stream.
+struct gofft_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~gofft_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+int gofft_graph_c2c(gofft_graph** out, const gofft_plan* p, const void* src, void* dst, int direction,
+                    void* stream) {
+  if (!out || !p) return fail(GOFFT_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (p->kind != K_C2C && p->kind != K_MANY && p->kind != K_ND_C2C)
+    return fail(GOFFT_ERR_INVALID_ARG, "plan is not a complex-to-complex plan");
+  hipStream_t s = (hipStream_t)stream;
+  if (!s) return fail(GOFFT_ERR_INVALID_ARG, "capture needs a stream other than the null stream");
+  char* ws = nullptr;
+  RC_TRY(p->ws(s, &ws));  // allocate before capture: no allocation is captured
+  auto g = std::make_unique<gofft_graph>();
+  HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = gofft_exec_c2c(p, src, dst, direction, stream);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &graph);
+  if (rc != GOFFT_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  HIP_TRY(ec);
+  g->graph = graph;
+  HIP_TRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *out = g.release();
+  return GOFFT_OK;
+}
+
+int gofft_graph_launch(const gofft_graph* g, void* stream) {
+  if (!g) return fail(GOFFT_ERR_INVALID_ARG, "null graph");
+  HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return GOFFT_OK;
+}
+
+int gofft_graph_destroy(gofft_graph* g) {
+  delete g;
+  return GOFFT_OK;
+}
+
 int gofft_sync(void) {
   HIP_TRY(hipDeviceSynchronize());
   return GOFFT_OK;

@@ -78,6 +78,26 @@ class _Plan:
         return L.lib().gofft_plan_out_elems(self._h, direction)
 
 
+class Graph:
+    """A captured exec of a complex plan on fixed tensors (gofft_graph_c2c);
+    launch() replays it on a stream. Keeps the plan and tensors alive."""
+
+    def __init__(self, plan, x, out, inverse=False, stream=None):
+        self._g = ctypes.c_void_p()
+        self._keep = (plan, x, out)
+        L.check(L.lib().gofft_graph_c2c(ctypes.byref(self._g), plan.handle, _ptr(x), _ptr(out),
+                                        int(bool(inverse)), _stream(stream)))
+
+    def launch(self, stream=None):
+        L.check(L.lib().gofft_graph_launch(self._g, _stream(stream)))
+
+    def __del__(self):
+        g = getattr(self, "_g", None)
+        if g and L is not None and L._lib is not None:
+            L._lib.gofft_graph_destroy(g)
+            self._g = ctypes.c_void_p()
+
+
 class C2CPlan(_Plan):
     """`batch` contiguous complex lines of length n (NewPlan, plan.go:42-73)."""
 

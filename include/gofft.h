@@ -198,6 +198,20 @@ int gofft_vjp_c2c_host(const gofft_plan* plan, const void* gy, void* gx, int dir
 int gofft_vjp_r2c_host(const gofft_plan* plan, const void* gy, void* gx);
 int gofft_vjp_c2r_host(const gofft_plan* plan, const void* gx, void* gX);
 
+/* --- graphs (no reference counterpart) ------------------------------------ */
+
+/* One gofft_exec_c2c of a c2c, mixed-length or nd-c2c plan on fixed device
+   buffers, captured on `stream` (not the null stream) as a hipGraph. A
+   launch replays every kernel of that exec (and a mixed-length plan's stream
+   fork and join) at once. The graph uses the plan's workspace of the capture
+   stream, so its replays must not overlap each other or execs of the same
+   plan on that stream. The plan must outlive the graph. */
+typedef struct gofft_graph gofft_graph;
+int gofft_graph_c2c(gofft_graph** graph, const gofft_plan* plan, const void* src, void* dst, int direction,
+                    void* stream);
+int gofft_graph_launch(const gofft_graph* graph, void* stream);
+int gofft_graph_destroy(gofft_graph* graph);
+
 /* Synchronise the device (for callers without their own stream handling). */
 int gofft_sync(void);
 
