@@ -5,9 +5,11 @@ Workload (one "step"): the mixed-length batched complex FFT — 4096 transforms
 with lengths drawn uniformly (numpy default_rng(0)) from {1024, 4096, 65536,
 6720 = 3*5*7*64, 1009 (Rader prime), 10007 (Rader prime), 20014 = 2*10007
 (Bluestein)}, re/im iid N(0,1) from seed 0, packed back to back in HBM, one
-forward transform of the whole batch per step through one plan. The headline
-value is the complex64 (FP32) form's GFLOP/s with the reference's flop count
-5*N*log2(N) per transform; the plain complex128 (FP64) form is timed beside it.
+forward transform of the whole batch per step through one plan, replayed as
+the plan's captured graph (the same exec launched directly is timed too). The
+headline value is the complex64 (FP32) form's GFLOP/s with the reference's
+flop count 5*N*log2(N) per transform; the plain complex128 (FP64) form is
+timed beside it.
 
 Prints ONE JSON line (the bench contract), with a `roofline` object for the
 dominant kernel (the Stockham pass kernel k_pass<float,16>) and a
@@ -99,13 +101,19 @@ def run_dtype(D, torch, dev, name, lengths, x64, steps, warmup):
     L = D.L
     total = sum(lengths)
     plan = D.ManyPlan(lengths, tdtype)
-    xin = torch.from_numpy(x64.astype(ndtype)).to(dev)
-    out = torch.empty_like(xin)
-    ms = time_events(lambda: plan.exec(xin, out=out), steps, warmup, torch)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        xin = torch.from_numpy(x64.astype(ndtype)).to(dev)
+        out = torch.empty_like(xin)
+        # the step as plain execs, then as one replayed graph of the same exec
+        ms_eager = time_events(lambda: plan.exec(xin, out=out, stream=stream), steps, warmup, torch)
+        graph = D.Graph(plan, xin, out, stream=stream)
+        ms = time_events(lambda: graph.launch(stream=stream), steps, warmup, torch)
+    torch.cuda.synchronize()
     work = {k: plan.kernel_work(k) for k in (L.KERNEL_PASS, L.KERNEL_CONV, L.KERNEL_REAL, L.KERNEL_AUX)}
-    res = {"ms": ms, "gflops": flops_of(lengths) / (ms * 1e6), "gbs": 2.0 * total * elt / (ms * 1e6),
-           "launches": plan.num_launches(), "work": work, "groups": {}}
-    del plan, xin, out
+    res = {"ms": ms, "ms_eager": ms_eager, "gflops": flops_of(lengths) / (ms * 1e6),
+           "gbs": 2.0 * total * elt / (ms * 1e6), "launches": plan.num_launches(), "work": work, "groups": {}}
+    del graph, plan, xin, out
     counts = {}
     for n in lengths:
         counts[n] = counts.get(n, 0) + 1
@@ -148,8 +156,9 @@ def pass_roofline(r, L, steps, warmup):
         "kernel_ms_per_step": round(ms, 4),
         "measured_on": "each length group alone on one stream (the step overlaps them)",
         # the run's k_pass<float, ...> dispatches: (warmup + steps) * launches
-        # in the step, then the same count in the per-group runs
-        "trace_window": {"skip": (steps + warmup) * step_launches, "take": (steps + warmup) * launches},
+        # in the eager step, the same in its graph replays (plus the capture,
+        # which launches nothing), then the per-group runs
+        "trace_window": {"skip": 2 * (steps + warmup) * step_launches, "take": (steps + warmup) * launches},
     }
 
 
@@ -440,6 +449,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(r32["ms"], 4),
+        "ms_per_step_eager": round(r32["ms_eager"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -447,13 +457,15 @@ def main():
         "data": "synthetic: re/im iid N(0,1) from numpy default_rng(0)",
         "config": {
             "workload": "mixed-length batched complex FFT: 4096 transforms, lengths uniform from "
-                        "{1024,4096,65536,6720,1009,10007,20014}, one plan, inputs resident in HBM",
+                        "{1024,4096,65536,6720,1009,10007,20014}, one plan, inputs resident in HBM; "
+                        "a step is one forward exec of the plan, replayed as its captured graph "
+                        "(ms_per_step_eager: the same exec launched directly)",
             "transforms": COUNT, "points": total, "flops_per_step": flops,
             "parallelism": "1 GPU",
         },
         "hbm_gbs": round(r32["gbs"], 1),
         "fp64_plain": {"gflops": round(r64["gflops"], 2), "gbs": round(r64["gbs"], 1),
-                       "ms_per_step": round(r64["ms"], 4)},
+                       "ms_per_step": round(r64["ms"], 4), "ms_per_step_eager": round(r64["ms_eager"], 4)},
         "fp32_over_fp64": round(r32["gflops"] / r64["gflops"], 3),
         "launches_per_step": {"fp32": r32["launches"], "fp64": r64["launches"]},
         "groups_ms": {"fp32": {str(n): round(g["ms"], 4) for n, g in r32["groups"].items()},
