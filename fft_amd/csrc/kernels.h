@@ -126,6 +126,8 @@ struct PassArgs {
   ColMap mm;
   SubFFT f2;
   const void* stw2;         // stage twiddles of f2
+  int mid256;               // f and f2 are both 256 = 16 x 16 with 16 threads per column,
+                            // no hooks, output walked along columns (kernels.hip mid256)
 };
 
 // Real-input forward (RFFT / PSD / spectrogram), single pass: each column is

@@ -437,11 +437,118 @@ extern __shared__ unsigned char g_dyn_lds[];
 // ---------------------------------------------------------------------------
 // Generic pass: load C columns (optional pass twiddle W_M^(r*k), optional
 // conjugation), R-point FFT in LDS, store (optional conjugation and scale).
-// EXT: the convolution-chain features (load/store hooks, the middle launch's
-// spectrum multiply and second FFT). Plain passes use the form without them,
-// which compiles to fewer registers.
-template <typename T, int MAXP, bool EXT>
+// Middle launch of a convolution chain whose forward pass and inverse first
+// pass are both 256 = 16 x 16 points, 16 threads per column, no hooks. Thread
+// (c, lt) holds the inputs lt + 16 i of forward stage 2's butterfly lt, whose
+// outputs lt + 16 i are the spectrum points to multiply and, after it, the
+// inputs of the inverse's stage-1 butterfly lt. So forward stage 2, the
+// multiply and inverse stage 1 stay in registers: two LDS round trips and
+// three barriers instead of six and seven. Same arithmetic as the general
+// middle launch.
+template <typename T>
+__device__ __forceinline__ void mid256(const PassArgs& a, typename Vec2<T>::type* lds, const long long* s_ib,
+                                       const long long* s_il, const long long* s_ob, const long long* s_ol,
+                                       const long long* s_tk, const long long* s_mk, const long long* s_ln) {
+  typedef typename Vec2<T>::type T2;
+  const SubFFT& f = a.f;
+  const int tid = threadIdx.x, C = f.C, Rp = f.Rp, sh = f.psh;
+  T2 v[16];
+  // load: fixed column per thread (either layout), points r0 + 16 u
+  {
+    const int ic = a.im.rowmajor ? tid >> 4 : tid & (C - 1);
+    const int ir0 = a.im.rowmajor ? tid & 15 : tid >> f.log2C;
+    const long long base = s_ib[ic];
+    const T2* in = reinterpret_cast<const T2*>(a.in);
+    const T2* p = in + (base >= 0 ? base + s_il[ic] : 0) + (long long)ir0 * a.im.g;
+    const long long st = 16 * a.im.g;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = base >= 0 ? p[u * st] : mkc<T2>(0, 0);
+    double2 tw_w = make_double2(1.0, 0.0), tw_step = make_double2(1.0, 0.0);
+    const long long k = a.tw_M ? s_tk[ic] : 0;
+    if (k) {
+      tw_w = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)ir0 * k, a.tw_shift);
+      tw_step = pass_twiddle(a.ptw_hi, a.ptw_lo, 16 * k, a.tw_shift);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      if (k) {
+        v[u] = cmul(v[u], from_d2<T2>(tw_w));
+        tw_w = dmul2(tw_w, tw_step);
+      }
+      if (a.conj_in) v[u].y = -v[u].y;
+    }
+    // forward stage 1 (ns = 1): butterfly ir0, outputs ir0 * 16 + i
+    dft<16>(v);
+    T2* q = lds + ic * Rp + lp(ir0 * 16, sh);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) q[i] = v[i];
+  }
+  __syncthreads();
+  const int c = tid >> 4, lt = tid & 15;
+  T2* const col = lds + c * Rp;
+  const int step = 16 + (16 >> sh);  // lp(lt + 16 i) = lp(lt) + i * step (sh <= 4)
+  // forward stage 2 (ns = 16): butterfly lt, inputs and outputs lt + 16 i
+  {
+    const Stage& s2 = f.st[1];
+    const T2* tw = reinterpret_cast<const T2*>(a.stw) + s2.tw_off + lt * s2.kmul;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = col[lp(lt, sh) + i * step];
+#pragma unroll
+    for (int i = 1; i < 16; ++i) v[i] = cmul(v[i], tw[i - 1]);
+    dft<16>(v);
+  }
+  // spectrum multiply at forward output positions r = lt + 16 i
+  const bool live = s_ob[c] >= 0;
+  {
+    const T2* spec = reinterpret_cast<const T2*>(a.mid_spec);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const long long k = s_mk[c] + (long long)(lt + 16 * i) * a.mm.g;
+      if (live && a.dc && k == 0) reinterpret_cast<T2*>(a.dc)[s_ln[c]] = v[i];  // Rader: A[0]
+      v[i] = live ? cconj(cmul(v[i], spec[k])) : mkc<T2>(0, 0);
+    }
+  }
+  // inverse stage 1 (ns = 1): butterfly lt has inputs lt + 16 i, the values
+  // just computed; outputs lt * 16 + i
+  dft<16>(v);
+  __syncthreads();  // every thread has read its forward stage-2 inputs
+  {
+    T2* q = col + lp(lt * 16, sh);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) q[i] = v[i];
+  }
+  __syncthreads();
+  // inverse stage 2 (ns = 16): butterfly lt, outputs lt + 16 i, stored
+  {
+    const Stage& s2 = a.f2.st[1];
+    const T2* tw = reinterpret_cast<const T2*>(a.stw2) + s2.tw_off + lt * s2.kmul;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = col[lp(lt, sh) + i * step];
+#pragma unroll
+    for (int i = 1; i < 16; ++i) v[i] = cmul(v[i], tw[i - 1]);
+    dft<16>(v);
+  }
+  if (!live) return;
+  T2* out = reinterpret_cast<T2*>(a.out) + s_ob[c] + s_ol[c] + (long long)lt * a.om.g;
+  const long long st = 16 * a.om.g;
+  const T sc = (T)a.scale;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    T2 w = v[i];
+    if (a.conj_out) w.y = -w.y;
+    if (a.scale != 1.0) w = cscale(w, sc);
+    out[i * st] = w;
+  }
+}
+
+// MODE 0: plain passes. MODE 1 (EXT): the convolution-chain features
+// (load/store hooks, the middle launch's spectrum multiply and second FFT);
+// plain passes use the form without them, which compiles to fewer registers.
+// MODE 2: a middle launch whose two FFTs are both 256 = 16 x 16 points with
+// 16 threads per column and no hooks (mid256 below).
+template <typename T, int MAXP, int MODE>
 __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
+  constexpr bool EXT = MODE >= 1;
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
   __shared__ long long s_ib[kMaxCols], s_il[kMaxCols], s_ob[kMaxCols], s_ol[kMaxCols], s_tk[kMaxCols];
@@ -485,6 +592,10 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
     }
   }
   __syncthreads();
+  if constexpr (MODE == 2) {
+    mid256<T>(a, lds, s_ib, s_il, s_ob, s_ol, s_tk, s_mk, s_ln);
+    return;
+  }
   const T2* in = reinterpret_cast<const T2*>(a.in);
   // Every thread keeps one column and visits its points r0 + u * tpc
   // (R <= MAXP * tpc, guaranteed by the planner), so its addresses are one
@@ -1259,18 +1370,23 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
   (void)maxp;
   long long groups = (a.ncols + a.f.C - 1) / a.f.C;
   const bool ext = a.in_hook || a.out_hook || a.mid_spec;
+  const int mode = a.mid256 ? 2 : ext ? 1 : 0;
   if (prec == 0) {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
-    if (ext)
-      hipLaunchKernelGGL((k_pass<double, 16, true>), dim3(groups), dim3(threads), bytes, s, a);
+    if (mode == 2)
+      hipLaunchKernelGGL((k_pass<double, 16, 2>), dim3(groups), dim3(threads), bytes, s, a);
+    else if (mode == 1)
+      hipLaunchKernelGGL((k_pass<double, 16, 1>), dim3(groups), dim3(threads), bytes, s, a);
     else
-      hipLaunchKernelGGL((k_pass<double, 16, false>), dim3(groups), dim3(threads), bytes, s, a);
+      hipLaunchKernelGGL((k_pass<double, 16, 0>), dim3(groups), dim3(threads), bytes, s, a);
   } else {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
-    if (ext)
-      hipLaunchKernelGGL((k_pass<float, 16, true>), dim3(groups), dim3(threads), bytes, s, a);
+    if (mode == 2)
+      hipLaunchKernelGGL((k_pass<float, 16, 2>), dim3(groups), dim3(threads), bytes, s, a);
+    else if (mode == 1)
+      hipLaunchKernelGGL((k_pass<float, 16, 1>), dim3(groups), dim3(threads), bytes, s, a);
     else
-      hipLaunchKernelGGL((k_pass<float, 16, false>), dim3(groups), dim3(threads), bytes, s, a);
+      hipLaunchKernelGGL((k_pass<float, 16, 0>), dim3(groups), dim3(threads), bytes, s, a);
   }
   return hipGetLastError();
 }

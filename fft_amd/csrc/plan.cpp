@@ -1254,6 +1254,13 @@ struct ConvChain {
         a.mid_spec = h.spec;
         a.f2 = pi.sp.f;
         a.stw2 = (const char*)inv.stw.p + pi.stw_off * elt(prec);
+        auto is16x16 = [](const SubPlan& sp) {
+          return sp.f.R == 256 && sp.f.nstages == 2 && sp.f.st[0].radix == 16 && sp.f.st[1].radix == 16 &&
+                 sp.f.tpc == 16 && sp.f.psh <= 4;
+        };
+        // (complex64 only: the complex128 form would spill registers)
+        a.mid256 = prec == GOFFT_C64 && K == 3 && is16x16(pf.sp) && is16x16(pi.sp) &&
+                   pf.sp.f.C == pi.sp.f.C && a.om.rowmajor;
         if (h.out_hook == 2) a.dc = dc;
         threads = pf.sp.threads;
       } else {
