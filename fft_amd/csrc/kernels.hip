@@ -541,11 +541,118 @@ __device__ __forceinline__ void mid256(const PassArgs& a, typename Vec2<T>::type
   }
 }
 
+// Two-stage register form of a pass whose sub-FFT is R = A x 16 points, radix
+// A first, then radix 16, with 16 threads per column. Thread (c, r0) loads
+// the points r0 + 16 u (u < A): exactly the inputs of stage-1 butterfly r0
+// (nb = 16), so stage 1 runs from the load registers. Stage 2 (ns = A) gives
+// butterfly j < A of a column the outputs j + A i, which go straight to
+// memory. One LDS round trip and one barrier instead of three and five.
+// Handles the pass twiddles, conjugation, scale, and the Bluestein chirp
+// hooks (multiply on load / conjugate-multiply on store, zero past nvalid).
+template <typename T, int A>
+__device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* lds, const long long* s_ib,
+                                     const long long* s_il, const long long* s_ob, const long long* s_ol,
+                                     const long long* s_tk) {
+  typedef typename Vec2<T>::type T2;
+  const SubFFT& f = a.f;
+  const int tid = threadIdx.x, C = f.C, Rp = f.Rp, sh = f.psh;
+  T2 v[16];
+  {
+    const int ic = a.im.rowmajor ? tid >> 4 : tid & (C - 1);
+    const int ir0 = a.im.rowmajor ? tid & 15 : tid >> f.log2C;
+    const long long base = s_ib[ic];
+    const long long il = s_il[ic];
+    const long long g = a.im.g;
+    const T2* in = reinterpret_cast<const T2*>(a.in);
+    T2 w[A];
+    if (a.in_hook == 0) {
+      const T2* p = in + (base >= 0 ? base + il : 0) + (long long)ir0 * g;
+#pragma unroll
+      for (int u = 0; u < A; ++u) v[u] = base >= 0 ? p[u * 16 * g] : mkc<T2>(0, 0);
+    } else {
+      const T2* hw = reinterpret_cast<const T2*>(a.hook_w);
+#pragma unroll
+      for (int u = 0; u < A; ++u) {
+        const long long i = il + (long long)(ir0 + 16 * u) * g;
+        const bool live = base >= 0 && i < a.nvalid;
+        v[u] = live ? in[base + i] : mkc<T2>(0, 0);
+        w[u] = live ? hw[i] : mkc<T2>(0, 0);
+      }
+    }
+    double2 tw_w = make_double2(1.0, 0.0), tw_step = make_double2(1.0, 0.0);
+    const long long k = a.tw_M ? s_tk[ic] : 0;
+    if (k) {
+      tw_w = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)ir0 * k, a.tw_shift);
+      tw_step = pass_twiddle(a.ptw_hi, a.ptw_lo, 16 * k, a.tw_shift);
+    }
+    // same order as the general form: pass twiddle, conjugation, hook
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+      if (k) {
+        v[u] = cmul(v[u], from_d2<T2>(tw_w));
+        tw_w = dmul2(tw_w, tw_step);
+      }
+      if (a.conj_in) v[u].y = -v[u].y;
+      if (a.in_hook) v[u] = cmul(v[u], w[u]);
+    }
+    // stage 1 (ns = 1): butterfly ir0, outputs ir0 * A + i
+    dft<A>(v);
+    T2* col = lds + ic * Rp;
+#pragma unroll
+    for (int i = 0; i < A; ++i) col[lp(ir0 * A + i, sh)] = v[i];
+  }
+  __syncthreads();
+  // stage 2 (radix 16, ns = nb = A): butterfly j of column oc
+  const int oc = a.om.rowmajor ? tid >> 4 : tid & (C - 1);
+  const int j = a.om.rowmajor ? tid & 15 : tid >> f.log2C;
+  const long long obase = s_ob[oc];
+  if (j >= A || obase < 0) return;
+  {
+    const T2* col = lds + oc * Rp;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = col[lp(j + A * i, sh)];
+    const Stage& s2 = f.st[1];
+    const T2* tw = reinterpret_cast<const T2*>(a.stw) + s2.tw_off + j * s2.kmul;
+#pragma unroll
+    for (int i = 1; i < 16; ++i) v[i] = cmul(v[i], tw[i - 1]);
+    dft<16>(v);
+  }
+  T2* out = reinterpret_cast<T2*>(a.out) + obase;
+  const long long ol = s_ol[oc];
+  const long long g = a.om.g;
+  const T sc = (T)a.scale;
+  if (a.out_hook == 0) {
+    const bool do_scale = a.scale != 1.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      T2 x = v[i];
+      if (a.conj_out) x.y = -x.y;
+      if (do_scale) x = cscale(x, sc);
+      out[ol + (long long)(j + A * i) * g] = x;
+    }
+  } else {
+    const T2* hw = reinterpret_cast<const T2*>(a.hook_w);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const long long p = ol + (long long)(j + A * i) * g;
+      if (p < a.nvalid) {
+        T2 x = cmul(cconj(v[i]), hw[p]);
+        if (a.conj_out) x.y = -x.y;
+        out[p] = cscale(x, sc);
+      }
+    }
+  }
+}
+
+// First radix of the two-stage register form of each MODE (0: not that form).
+constexpr int reg2_radix(int mode) { return mode == 3 ? 10 : mode == 4 ? 12 : mode == 5 ? 16 : 0; }
+
 // MODE 0: plain passes. MODE 1 (EXT): the convolution-chain features
 // (load/store hooks, the middle launch's spectrum multiply and second FFT);
 // plain passes use the form without them, which compiles to fewer registers.
 // MODE 2: a middle launch whose two FFTs are both 256 = 16 x 16 points with
-// 16 threads per column and no hooks (mid256 below).
+// 16 threads per column and no hooks (mid256 above). MODES 3, 4, 5: the
+// two-stage register form (reg2 above) with first radix 10, 12, 16.
 template <typename T, int MAXP, int MODE>
 __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   constexpr bool EXT = MODE >= 1;
@@ -594,6 +701,10 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   __syncthreads();
   if constexpr (MODE == 2) {
     mid256<T>(a, lds, s_ib, s_il, s_ob, s_ol, s_tk, s_mk, s_ln);
+    return;
+  }
+  if constexpr (reg2_radix(MODE) > 0) {
+    reg2<T, reg2_radix(MODE)>(a, lds, s_ib, s_il, s_ob, s_ol, s_tk);
     return;
   }
   const T2* in = reinterpret_cast<const T2*>(a.in);
@@ -1370,24 +1481,40 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
   (void)maxp;
   long long groups = (a.ncols + a.f.C - 1) / a.f.C;
   const bool ext = a.in_hook || a.out_hook || a.mid_spec;
-  const int mode = a.mid256 ? 2 : ext ? 1 : 0;
-  if (prec == 0) {
-    size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
-    if (mode == 2)
-      hipLaunchKernelGGL((k_pass<double, 16, 2>), dim3(groups), dim3(threads), bytes, s, a);
-    else if (mode == 1)
-      hipLaunchKernelGGL((k_pass<double, 16, 1>), dim3(groups), dim3(threads), bytes, s, a);
-    else
-      hipLaunchKernelGGL((k_pass<double, 16, 0>), dim3(groups), dim3(threads), bytes, s, a);
-  } else {
-    size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
-    if (mode == 2)
-      hipLaunchKernelGGL((k_pass<float, 16, 2>), dim3(groups), dim3(threads), bytes, s, a);
-    else if (mode == 1)
-      hipLaunchKernelGGL((k_pass<float, 16, 1>), dim3(groups), dim3(threads), bytes, s, a);
-    else
-      hipLaunchKernelGGL((k_pass<float, 16, 0>), dim3(groups), dim3(threads), bytes, s, a);
+  int mode = a.mid256 ? 2 : ext ? 1 : 0;
+  // The two-stage register form: R = A x 16 planned radix A first, 16
+  // threads per column, plain stage twiddles, no spectrum multiply and no
+  // Rader hooks. A plain 16 x 16 pass keeps form 0, which already runs both
+  // of its stages from registers; the complex128 16 x 16 register form would
+  // spill, so it keeps form 1.
+  const SubFFT& f = a.f;
+  const int A = f.st[0].radix;
+  if (mode <= 1 && f.nstages == 2 && f.tpc == 16 && f.st[1].radix == 16 && f.R == 16 * A &&
+      f.st[0].cmul == 0 && f.st[1].cmul == 0 && !a.mid_spec && a.in_hook <= 1 && a.out_hook <= 1) {
+    if (A == 10)
+      mode = 3;
+    else if (A == 12)
+      mode = 4;
+    else if (A == 16 && ext && prec != 0)
+      mode = 5;
   }
+#define GOFFT_PASS_MODES(T)                                                                 \
+  do {                                                                                      \
+    const size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<T>();                                \
+    switch (mode) {                                                                         \
+      case 1: hipLaunchKernelGGL((k_pass<T, 16, 1>), dim3(groups), dim3(threads), bytes, s, a); break; \
+      case 2: hipLaunchKernelGGL((k_pass<T, 16, 2>), dim3(groups), dim3(threads), bytes, s, a); break; \
+      case 3: hipLaunchKernelGGL((k_pass<T, 16, 3>), dim3(groups), dim3(threads), bytes, s, a); break; \
+      case 4: hipLaunchKernelGGL((k_pass<T, 16, 4>), dim3(groups), dim3(threads), bytes, s, a); break; \
+      case 5: hipLaunchKernelGGL((k_pass<T, 16, 5>), dim3(groups), dim3(threads), bytes, s, a); break; \
+      default: hipLaunchKernelGGL((k_pass<T, 16, 0>), dim3(groups), dim3(threads), bytes, s, a); break; \
+    }                                                                                       \
+  } while (0)
+  if (prec == 0)
+    GOFFT_PASS_MODES(double);
+  else
+    GOFFT_PASS_MODES(float);
+#undef GOFFT_PASS_MODES
   return hipGetLastError();
 }
 

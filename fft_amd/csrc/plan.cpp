@@ -444,6 +444,11 @@ bool make_sub(int R, int C, SubPlan& sp, int min_threads = 0, int sh = 4, int ma
     std::vector<int> comp = composite_schedule(R);
     if (!comp.empty() && comp.size() < rads.size() && feasible(comp, T)) rads = comp;
   }
+  // A x 16 points with 16 threads per column: radix A first, the order the
+  // pass kernel's two-stage register form runs (kernels.hip reg2).
+  if (maxp == kMaxP && T / C == 16 && rads.size() == 2 && rads[0] == 16 && rads[1] != 16 && R == 16 * rads[1] &&
+      feasible({rads[1], 16}, T))
+    rads = {rads[1], 16};
   sp = SubPlan();
   sp.threads = T;
   sp.maxp = maxp;
