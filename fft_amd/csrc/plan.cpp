@@ -683,14 +683,26 @@ bool pick_cols(int R, int cap, i64 pts, SubPlan& sp, int* Cout) {
   return false;
 }
 
+// Largest power-of-two column count (at most kMaxCols) that still gives
+// every CU about two workgroups over `lines` lines: packing many lines into
+// few workgroups would leave most of the chip idle on a small batch.
+int spread_cols(i64 lines, int cus) {
+  const i64 want = 2 * (i64)std::max(cus, 1);
+  int c = 1;
+  while (c < kMaxCols && lines / (2 * c) >= want) c *= 2;
+  return c;
+}
+
 // Schedule of a fused single-kernel convolution of length m (k_conv): C
-// columns within the preferred LDS budget when they fit, else one column with
-// the whole LDS. The complex64 kernel also has a 24-points-per-thread form
+// columns within the preferred LDS budget when they fit (fewer when `lines`
+// lines would otherwise fill too few workgroups, spread_cols), else one
+// column with the whole LDS. The complex64 kernel also has a 24-points-per-thread form
 // (radix <= 8 stages), and may pad every 32 or 64 points instead of 16, which
 // takes a single column up to 20480 points.
-bool fused_conv_plan(i64 m, int prec, i64 pts, SubPlan& sp) {
+bool fused_conv_plan(i64 m, int prec, i64 pts, SubPlan& sp, int colcap = 16) {
   int C = 0;
-  if (pitch(m) <= pts && pick_cols((int)m, 16, std::min<i64>(pts, std::max<i64>(pitch(m), soft_pts(prec))), sp, &C))
+  if (pitch(m) <= pts &&
+      pick_cols((int)m, colcap, std::min<i64>(pts, std::max<i64>(pitch(m), soft_pts(prec))), sp, &C))
     return true;
   if (prec != GOFFT_C64 || m > 24 * 1024) return false;
   for (int sh : {4, 5, 6})
@@ -947,7 +959,8 @@ class StockhamEngine : public Engine {
     // Single pass: all n points of a line in one workgroup. Strided lines need
     // several adjacent lines per workgroup for coalesced access.
     const int cwant = st == 1 ? 1 : (int)std::min<i64>(prec == 0 ? 8 : 16, st);
-    const int ccap = (int)std::min<i64>(kMaxCols, std::max<i64>(cwant, lines));
+    const int ccap = (int)std::min<i64>(std::min<i64>(kMaxCols, std::max<i64>(cwant, lines)),
+                                        std::max(cwant, spread_cols(lines, dev.cus)));
     const i64 budget = std::min<i64>(pts, std::max<i64>(pitch(n) * cwant, soft_pts(prec)));
     SubPlan sp;
     int C = 0;
@@ -1404,7 +1417,7 @@ class RaderEngine : public Engine {
     RC_TRY(upload_complex(spec, b, prec));  // natural order (fused kernel)
     DevInfo dev;
     RC_TRY(current_device(&dev));
-    if (fused_conv_plan(cl, prec, dev.lds_full / (i64)elt(prec), sp)) {
+    if (fused_conv_plan(cl, prec, dev.lds_full / (i64)elt(prec), sp, spread_cols(batch, dev.cus))) {
       fused = true;
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
@@ -1539,7 +1552,8 @@ class BluesteinEngine : public Engine {
     if (faithful) {
       if (pitch(m) <= std::min<i64>(pts, soft_pts(prec))) {
         C = 1;
-        while (C * 2 <= 16 && (i64)C * 2 * pitch(m) <= std::min<i64>(pts, soft_pts(prec))) C *= 2;
+        const int cap = std::min(16, spread_cols(batch, dev.cus));
+        while (C * 2 <= cap && (i64)C * 2 * pitch(m) <= std::min<i64>(pts, soft_pts(prec))) C *= 2;
         while (C >= 1 && !make_sub_r2((int)m, C, 1, sp)) C /= 2;
         if (C >= 1) {
           fused = true;
@@ -1552,7 +1566,7 @@ class BluesteinEngine : public Engine {
       RC_TRY(chain.upload_spec(hF, spec));
       return chain.upload_spec(hI, specI);
     }
-    if (fused_conv_plan(m, prec, pts, sp)) {
+    if (fused_conv_plan(m, prec, pts, sp, spread_cols(batch, dev.cus))) {
       fused = true;
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
