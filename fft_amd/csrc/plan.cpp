@@ -2074,12 +2074,12 @@ struct gofft_plan {
 
 namespace {
 
-// Lanes of a mixed-length plan (GOFFT_LANES streams, default 4, at most one
+// Lanes of a mixed-length plan (GOFFT_LANES streams, default 5, at most one
 // per group): each group, largest data first, goes to the lane with the least
 // data so far. Lane workspaces are allocated here, so that an exec enqueues
 // no allocation.
 int assign_lanes(gofft_plan& p) {
-  int want = 4;
+  int want = 5;
   if (const char* e = std::getenv("GOFFT_LANES")) want = std::max(1, std::min(16, std::atoi(e)));
   const int n = (int)std::min<size_t>((size_t)want, p.groups.size());
   p.lane_of.assign(p.groups.size(), 0);
