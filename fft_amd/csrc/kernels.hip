@@ -547,12 +547,16 @@ __device__ __forceinline__ void mid256(const PassArgs& a, typename Vec2<T>::type
 // (nb = 16), so stage 1 runs from the load registers. Stage 2 (ns = A) gives
 // butterfly j < A of a column the outputs j + A i, which go straight to
 // memory. One LDS round trip and one barrier instead of three and five.
-// Handles the pass twiddles, conjugation, scale, and the Bluestein chirp
-// hooks (multiply on load / conjugate-multiply on store, zero past nvalid).
+// Handles the pass twiddles, conjugation, scale, the Bluestein chirp hooks
+// (multiply on load / conjugate-multiply on store, zero past nvalid) and the
+// Rader hooks (gather by the generator permutation on load; on store, x0 plus
+// the conjugated convolution scattered by the inverse permutation, and
+// X[0] = x0 + A[0]).
 template <typename T, int A>
 __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* lds, const long long* s_ib,
                                      const long long* s_il, const long long* s_ob, const long long* s_ol,
-                                     const long long* s_tk) {
+                                     const long long* s_tk, const long long* s_ln,
+                                     const typename Vec2<T>::type* s_x0) {
   typedef typename Vec2<T>::type T2;
   const SubFFT& f = a.f;
   const int tid = threadIdx.x, C = f.C, Rp = f.Rp, sh = f.psh;
@@ -569,7 +573,7 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
       const T2* p = in + (base >= 0 ? base + il : 0) + (long long)ir0 * g;
 #pragma unroll
       for (int u = 0; u < A; ++u) v[u] = base >= 0 ? p[u * 16 * g] : mkc<T2>(0, 0);
-    } else {
+    } else if (a.in_hook == 1) {
       const T2* hw = reinterpret_cast<const T2*>(a.hook_w);
 #pragma unroll
       for (int u = 0; u < A; ++u) {
@@ -577,6 +581,13 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
         const bool live = base >= 0 && i < a.nvalid;
         v[u] = live ? in[base + i] : mkc<T2>(0, 0);
         w[u] = live ? hw[i] : mkc<T2>(0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < A; ++u) {
+        const long long i = il + (long long)(ir0 + 16 * u) * g;
+        const bool live = base >= 0 && i < a.nvalid;
+        v[u] = live ? in[base + (long long)a.hook_perm[i]] : mkc<T2>(0, 0);
       }
     }
     double2 tw_w = make_double2(1.0, 0.0), tw_step = make_double2(1.0, 0.0);
@@ -593,7 +604,7 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
         tw_w = dmul2(tw_w, tw_step);
       }
       if (a.conj_in) v[u].y = -v[u].y;
-      if (a.in_hook) v[u] = cmul(v[u], w[u]);
+      if (a.in_hook == 1) v[u] = cmul(v[u], w[u]);
     }
     // stage 1 (ns = 1): butterfly ir0, outputs ir0 * A + i
     dft<A>(v);
@@ -630,7 +641,7 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
       if (do_scale) x = cscale(x, sc);
       out[ol + (long long)(j + A * i) * g] = x;
     }
-  } else {
+  } else if (a.out_hook == 1) {
     const T2* hw = reinterpret_cast<const T2*>(a.hook_w);
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
@@ -639,6 +650,23 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
         T2 x = cmul(cconj(v[i]), hw[p]);
         if (a.conj_out) x.y = -x.y;
         out[p] = cscale(x, sc);
+      }
+    }
+  } else {
+    const T2 x0 = s_x0[oc];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const long long p = ol + (long long)(j + A * i) * g;
+      const long long idx = p - a.conv_off;
+      if (idx >= 0 && idx < a.nvalid) {
+        T2 y = cadd(x0, cconj(v[i]));
+        if (a.conj_out) y.y = -y.y;
+        out[a.hook_perm[idx]] = cscale(y, sc);
+      }
+      if (p == 0) {
+        T2 y = cadd(x0, reinterpret_cast<const T2*>(a.dc)[s_ln[oc]]);
+        if (a.conj_out) y.y = -y.y;
+        out[0] = cscale(y, sc);
       }
     }
   }
@@ -704,7 +732,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
     return;
   }
   if constexpr (reg2_radix(MODE) > 0) {
-    reg2<T, reg2_radix(MODE)>(a, lds, s_ib, s_il, s_ob, s_ol, s_tk);
+    reg2<T, reg2_radix(MODE)>(a, lds, s_ib, s_il, s_ob, s_ol, s_tk, s_ln, s_x0);
     return;
   }
   const T2* in = reinterpret_cast<const T2*>(a.in);
@@ -1483,14 +1511,16 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
   const bool ext = a.in_hook || a.out_hook || a.mid_spec;
   int mode = a.mid256 ? 2 : ext ? 1 : 0;
   // The two-stage register form: R = A x 16 planned radix A first, 16
-  // threads per column, plain stage twiddles, no spectrum multiply and no
-  // Rader hooks. A plain 16 x 16 pass keeps form 0, which already runs both
+  // threads per column, plain stage twiddles, no spectrum multiply; Rader
+  // hooks only in complex128 (measured slower than form 1 in complex64). A
+  // plain 16 x 16 pass keeps form 0, which already runs both
   // of its stages from registers; the complex128 16 x 16 register form would
   // spill, so it keeps form 1.
   const SubFFT& f = a.f;
   const int A = f.st[0].radix;
   if (mode <= 1 && f.nstages == 2 && f.tpc == 16 && f.st[1].radix == 16 && f.R == 16 * A &&
-      f.st[0].cmul == 0 && f.st[1].cmul == 0 && !a.mid_spec && a.in_hook <= 1 && a.out_hook <= 1) {
+      f.st[0].cmul == 0 && f.st[1].cmul == 0 && !a.mid_spec &&
+      (prec == 0 || (a.in_hook != 2 && a.out_hook != 2))) {
     if (A == 10)
       mode = 3;
     else if (A == 12)

@@ -40,7 +40,7 @@ def _run(D, torch, n, batch, x, dtype, inverse=False, flags=0):
 POW2 = [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384, 65536, 1 << 18, 1 << 20]
 SMOOTH = [3, 5, 6, 7, 9, 10, 11, 12, 13, 15, 21, 49, 60, 100, 121, 169, 1000, 1296, 6720, 1008, 20580, 9408, 20160,
           3 * 5 * 7 * 11 * 13, 729, 2 * 3 * 5 * 7 * 64 * 3]
-RADER = [769, 1009, 2017, 5003, 9973, 10007]
+RADER = [769, 1009, 2017, 5003, 9973, 10007, 20011]
 BLUE = [17, 19, 23, 34, 46, 101, 289, 641]
 
 
@@ -57,7 +57,8 @@ def test_c2c_fp64_vs_oracle(dev, oracle, n):
     assert rel_l2(goti, wanti) <= bound64(n), (n, rel_l2(goti, wanti))
 
 
-@pytest.mark.parametrize("n", [4096, 65536, 6720, 1009, 10007, 20014, 1 << 20, 641, 1296, 10006, 9973, 5003])
+@pytest.mark.parametrize("n", [4096, 65536, 6720, 1009, 10007, 20014, 1 << 20, 641, 1296, 10006, 9973, 5003,
+                               20011, 39239])
 def test_c2c_fp32_vs_fp64_oracle(dev, oracle, n):
     D, torch = dev
     batch = 2
