@@ -497,16 +497,20 @@ __device__ __forceinline__ void mid256(const PassArgs& a, typename Vec2<T>::type
     for (int i = 1; i < 16; ++i) v[i] = cmul(v[i], tw[i - 1]);
     dft<16>(v);
   }
-  // spectrum multiply at forward output positions r = lt + 16 i
+  // spectrum multiply at forward output positions k = s_mk[c] + r * mm.g,
+  // r = lt + 16 i (mm.g >= 1, so k = 0 only at lt = i = 0 of a column with
+  // s_mk = 0). Every lane loads its 16 entries before using any; a tail
+  // column (s_mk = 0) reads valid entries and discards them.
   const bool live = s_ob[c] >= 0;
   {
-    const T2* spec = reinterpret_cast<const T2*>(a.mid_spec);
+    if (live && a.dc && lt == 0 && s_mk[c] == 0) reinterpret_cast<T2*>(a.dc)[s_ln[c]] = v[0];  // Rader: A[0]
+    const T2* spec = reinterpret_cast<const T2*>(a.mid_spec) + s_mk[c] + (long long)lt * a.mm.g;
+    const long long sst = 16 * a.mm.g;
+    T2 w[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const long long k = s_mk[c] + (long long)(lt + 16 * i) * a.mm.g;
-      if (live && a.dc && k == 0) reinterpret_cast<T2*>(a.dc)[s_ln[c]] = v[i];  // Rader: A[0]
-      v[i] = live ? cconj(cmul(v[i], spec[k])) : mkc<T2>(0, 0);
-    }
+    for (int i = 0; i < 16; ++i) w[i] = spec[i * sst];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = live ? cconj(cmul(v[i], w[i])) : mkc<T2>(0, 0);
   }
   // inverse stage 1 (ns = 1): butterfly lt has inputs lt + 16 i, the values
   // just computed; outputs lt * 16 + i
@@ -548,11 +552,13 @@ __device__ __forceinline__ void mid256(const PassArgs& a, typename Vec2<T>::type
 // butterfly j < A of a column the outputs j + A i, which go straight to
 // memory. One LDS round trip and one barrier instead of three and five.
 // Handles the pass twiddles, conjugation, scale, the Bluestein chirp hooks
-// (multiply on load / conjugate-multiply on store, zero past nvalid) and the
-// Rader hooks (gather by the generator permutation on load; on store, x0 plus
-// the conjugated convolution scattered by the inverse permutation, and
-// X[0] = x0 + A[0]).
-template <typename T, int A>
+// (multiply on load / conjugate-multiply on store, zero past nvalid) and, in
+// form HK = 2, the Rader hooks (gather by the generator permutation on load;
+// on store, x0 plus the conjugated convolution scattered by the inverse
+// permutation, and X[0] = x0 + A[0]). Form HK = 1 takes the chirp hooks only
+// and issues all of a thread's store-side chirp loads before any is used
+// (more registers, fewer round trips).
+template <typename T, int A, int HK>
 __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* lds, const long long* s_ib,
                                      const long long* s_il, const long long* s_ob, const long long* s_ol,
                                      const long long* s_tk, const long long* s_ln,
@@ -582,7 +588,7 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
         v[u] = live ? in[base + i] : mkc<T2>(0, 0);
         w[u] = live ? hw[i] : mkc<T2>(0, 0);
       }
-    } else {
+    } else if constexpr (HK == 2) {
 #pragma unroll
       for (int u = 0; u < A; ++u) {
         const long long i = il + (long long)(ir0 + 16 * u) * g;
@@ -641,6 +647,25 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
       if (do_scale) x = cscale(x, sc);
       out[ol + (long long)(j + A * i) * g] = x;
     }
+  } else if (HK == 1) {
+    // all chirp loads first (index clamped to 0 past nvalid), then the
+    // stores of the points below nvalid
+    const T2* hw = reinterpret_cast<const T2*>(a.hook_w);
+    T2 w[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const long long p = ol + (long long)(j + A * i) * g;
+      w[i] = hw[p < a.nvalid ? p : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const long long p = ol + (long long)(j + A * i) * g;
+      if (p < a.nvalid) {
+        T2 x = cmul(cconj(v[i]), w[i]);
+        if (a.conj_out) x.y = -x.y;
+        out[p] = cscale(x, sc);
+      }
+    }
   } else if (a.out_hook == 1) {
     const T2* hw = reinterpret_cast<const T2*>(a.hook_w);
 #pragma unroll
@@ -672,15 +697,21 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
   }
 }
 
-// First radix of the two-stage register form of each MODE (0: not that form).
-constexpr int reg2_radix(int mode) { return mode == 3 ? 10 : mode == 4 ? 12 : mode == 5 ? 16 : 0; }
+// First radix of the two-stage register form of each MODE (0: not that
+// form), and its hook form (reg2's HK).
+constexpr int reg2_radix(int mode) {
+  return mode == 3 || mode == 6 ? 10 : mode == 4 || mode == 7 ? 12 : mode == 5 ? 16 : 0;
+}
+constexpr int reg2_hk(int mode) { return mode >= 5 ? 1 : 2; }
 
 // MODE 0: plain passes. MODE 1 (EXT): the convolution-chain features
 // (load/store hooks, the middle launch's spectrum multiply and second FFT);
 // plain passes use the form without them, which compiles to fewer registers.
 // MODE 2: a middle launch whose two FFTs are both 256 = 16 x 16 points with
-// 16 threads per column and no hooks (mid256 above). MODES 3, 4, 5: the
-// two-stage register form (reg2 above) with first radix 10, 12, 16.
+// 16 threads per column and no hooks (mid256 above). MODES 3, 4: the
+// two-stage register form (reg2 above) with first radix 10, 12, every hook
+// (HK = 2); MODES 6, 7, 5: first radix 10, 12, 16 with the chirp hooks'
+// loads issued up front (HK = 1), for complex64 Bluestein chains.
 template <typename T, int MAXP, int MODE>
 __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   constexpr bool EXT = MODE >= 1;
@@ -732,7 +763,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
     return;
   }
   if constexpr (reg2_radix(MODE) > 0) {
-    reg2<T, reg2_radix(MODE)>(a, lds, s_ib, s_il, s_ob, s_ol, s_tk, s_ln, s_x0);
+    reg2<T, reg2_radix(MODE), reg2_hk(MODE)>(a, lds, s_ib, s_il, s_ob, s_ol, s_tk, s_ln, s_x0);
     return;
   }
   const T2* in = reinterpret_cast<const T2*>(a.in);
@@ -1509,42 +1540,52 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
   (void)maxp;
   long long groups = (a.ncols + a.f.C - 1) / a.f.C;
   const bool ext = a.in_hook || a.out_hook || a.mid_spec;
-  int mode = a.mid256 ? 2 : ext ? 1 : 0;
+  int mode = a.mid256 && prec != 0 ? 2 : ext ? 1 : 0;  // form 2: complex64 only
   // The two-stage register form: R = A x 16 planned radix A first, 16
-  // threads per column, plain stage twiddles, no spectrum multiply; Rader
-  // hooks only in complex128 (measured slower than form 1 in complex64). A
-  // plain 16 x 16 pass keeps form 0, which already runs both
-  // of its stages from registers; the complex128 16 x 16 register form would
-  // spill, so it keeps form 1.
+  // threads per column, plain stage twiddles, no spectrum multiply. Complex64
+  // Bluestein chains take the HK = 1 forms (they spill in complex128); the
+  // Rader hooks run in the register form in complex128 only (measured slower
+  // than form 1 in complex64). A plain 16 x 16 pass keeps form 0, which
+  // already runs both of its stages from registers.
   const SubFFT& f = a.f;
   const int A = f.st[0].radix;
   if (mode <= 1 && f.nstages == 2 && f.tpc == 16 && f.st[1].radix == 16 && f.R == 16 * A &&
-      f.st[0].cmul == 0 && f.st[1].cmul == 0 && !a.mid_spec &&
-      (prec == 0 || (a.in_hook != 2 && a.out_hook != 2))) {
-    if (A == 10)
-      mode = 3;
-    else if (A == 12)
-      mode = 4;
-    else if (A == 16 && ext && prec != 0)
-      mode = 5;
+      f.st[0].cmul == 0 && f.st[1].cmul == 0 && !a.mid_spec) {
+    const bool hooks = a.in_hook || a.out_hook;
+    const bool rader = a.in_hook == 2 || a.out_hook == 2;
+    if (hooks && prec != 0 && !rader) {
+      if (A == 10) mode = 6;
+      if (A == 12) mode = 7;
+      if (A == 16) mode = 5;
+    } else if (!hooks || prec == 0) {
+      if (A == 10) mode = 3;
+      if (A == 12) mode = 4;
+    }
   }
-#define GOFFT_PASS_MODES(T)                                                                 \
-  do {                                                                                      \
-    const size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<T>();                                \
-    switch (mode) {                                                                         \
-      case 1: hipLaunchKernelGGL((k_pass<T, 16, 1>), dim3(groups), dim3(threads), bytes, s, a); break; \
-      case 2: hipLaunchKernelGGL((k_pass<T, 16, 2>), dim3(groups), dim3(threads), bytes, s, a); break; \
-      case 3: hipLaunchKernelGGL((k_pass<T, 16, 3>), dim3(groups), dim3(threads), bytes, s, a); break; \
-      case 4: hipLaunchKernelGGL((k_pass<T, 16, 4>), dim3(groups), dim3(threads), bytes, s, a); break; \
-      case 5: hipLaunchKernelGGL((k_pass<T, 16, 5>), dim3(groups), dim3(threads), bytes, s, a); break; \
-      default: hipLaunchKernelGGL((k_pass<T, 16, 0>), dim3(groups), dim3(threads), bytes, s, a); break; \
-    }                                                                                       \
-  } while (0)
-  if (prec == 0)
-    GOFFT_PASS_MODES(double);
-  else
-    GOFFT_PASS_MODES(float);
-#undef GOFFT_PASS_MODES
+#define GOFFT_PASS(T, M) hipLaunchKernelGGL((k_pass<T, 16, M>), dim3(groups), dim3(threads), bytes, s, a)
+  if (prec == 0) {
+    // complex128 has forms 0, 1, 3, 4 (the others are complex64 only)
+    const size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
+    switch (mode) {
+      case 1: GOFFT_PASS(double, 1); break;
+      case 3: GOFFT_PASS(double, 3); break;
+      case 4: GOFFT_PASS(double, 4); break;
+      default: GOFFT_PASS(double, 0); break;
+    }
+  } else {
+    const size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
+    switch (mode) {
+      case 1: GOFFT_PASS(float, 1); break;
+      case 2: GOFFT_PASS(float, 2); break;
+      case 3: GOFFT_PASS(float, 3); break;
+      case 4: GOFFT_PASS(float, 4); break;
+      case 5: GOFFT_PASS(float, 5); break;
+      case 6: GOFFT_PASS(float, 6); break;
+      case 7: GOFFT_PASS(float, 7); break;
+      default: GOFFT_PASS(float, 0); break;
+    }
+  }
+#undef GOFFT_PASS
   return hipGetLastError();
 }
 
