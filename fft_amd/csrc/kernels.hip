@@ -863,11 +863,26 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
     const int c = tid >> f.log2tpc;
     const int lt = tid & (f.tpc - 1);
     if (s_ob[c] >= 0) {
-      for (int r = lt; r < R; r += f.tpc) {
-        const long long k = s_mk[c] + (long long)r * a.mm.g;
-        T2 v = lds[c * Rp + lp(r, f.psh)];
-        if (a.dc && k == 0) reinterpret_cast<T2*>(a.dc)[s_ln[c]] = v;  // Rader: A[0]
-        lds[c * Rp + lp(r, f.psh)] = cconj(cmul(v, spec[k]));
+      // batches of SB points per thread, table loads issued first
+      constexpr int SB = sizeof(T) == 8 ? 4 : 8;
+      const int tpc = f.tpc;
+      for (int r0 = lt; r0 < R; r0 += SB * tpc) {
+        T2 w[SB];
+#pragma unroll
+        for (int b = 0; b < SB; ++b) {
+          const int r = r0 + b * tpc;
+          w[b] = r < R ? spec[s_mk[c] + (long long)r * a.mm.g] : mkc<T2>(0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < SB; ++b) {
+          const int r = r0 + b * tpc;
+          if (r < R) {
+            const long long k = s_mk[c] + (long long)r * a.mm.g;
+            T2 v = lds[c * Rp + lp(r, f.psh)];
+            if (a.dc && k == 0) reinterpret_cast<T2*>(a.dc)[s_ln[c]] = v;  // Rader: A[0]
+            lds[c * Rp + lp(r, f.psh)] = cconj(cmul(v, w[b]));
+          }
+        }
       }
     }
     __syncthreads();
@@ -918,29 +933,49 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
     return;
   }
   const T2 x0 = out_hook == 2 ? s_x0[oc] : mkc<T2>(0, 0);
+  // Hooked stores in batches of SB points: the chirp values or permutation
+  // entries of a batch are loaded (index clamped to 0 where unused) before
+  // any is used.
+  constexpr int SB = sizeof(T) == 8 ? 4 : 8;
 #pragma unroll
-  for (int u = 0; u < MAXP; ++u) {
-    const int r = or0 + u * frs;
-    if (r < R) {
-      T2 v = ocol[lp(r, f.psh)];
+  for (int u0 = 0; u0 < MAXP; u0 += SB) {
+    T2 w[SB];
+    int ip[SB];
+#pragma unroll
+    for (int b = 0; b < SB; ++b) {
+      const int r = or0 + (u0 + b) * frs;
       const long long p = s_ol[oc] + (long long)r * a.om.g;
       if (out_hook == 1) {
-        if (p < a.nvalid) {
-          v = cmul(cconj(v), reinterpret_cast<const T2*>(a.hook_w)[p]);
-          if (a.conj_out) v.y = -v.y;
-          out[obase + p] = cscale(v, sc);
-        }
+        w[b] = reinterpret_cast<const T2*>(a.hook_w)[r < R && p < a.nvalid ? p : 0];
       } else {
         const long long idx = p - a.conv_off;
-        if (idx >= 0 && idx < a.nvalid) {
-          T2 y = cadd(x0, cconj(v));
-          if (a.conj_out) y.y = -y.y;
-          out[obase + a.hook_perm[idx]] = cscale(y, sc);
-        }
-        if (p == 0) {
-          T2 y = cadd(x0, reinterpret_cast<const T2*>(a.dc)[s_ln[oc]]);
-          if (a.conj_out) y.y = -y.y;
-          out[obase] = cscale(y, sc);
+        ip[b] = r < R && idx >= 0 && idx < a.nvalid ? a.hook_perm[idx] : 0;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < SB; ++b) {
+      const int r = or0 + (u0 + b) * frs;
+      if (r < R) {
+        T2 v = ocol[lp(r, f.psh)];
+        const long long p = s_ol[oc] + (long long)r * a.om.g;
+        if (out_hook == 1) {
+          if (p < a.nvalid) {
+            v = cmul(cconj(v), w[b]);
+            if (a.conj_out) v.y = -v.y;
+            out[obase + p] = cscale(v, sc);
+          }
+        } else {
+          const long long idx = p - a.conv_off;
+          if (idx >= 0 && idx < a.nvalid) {
+            T2 y = cadd(x0, cconj(v));
+            if (a.conj_out) y.y = -y.y;
+            out[obase + ip[b]] = cscale(y, sc);
+          }
+          if (p == 0) {
+            T2 y = cadd(x0, reinterpret_cast<const T2*>(a.dc)[s_ln[oc]]);
+            if (a.conj_out) y.y = -y.y;
+            out[obase] = cscale(y, sc);
+          }
         }
       }
     }
@@ -1165,29 +1200,73 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   // forward transform's A[0]. The thread with lt == 0 reads it before it
   // multiplies point 0 itself.
   const T2 colsum = a.kind != 2 && lt == 0 ? lds[c * Rp] : mkc<T2>(0, 0);
+  // Spectrum multiply in batches of SB points per thread: every table load
+  // of a batch is issued before any is used.
+  constexpr int SB = 8;
   const T2* spec = reinterpret_cast<const T2*>(a.spec);
-  for (int r = lt; r < R; r += f.tpc) lds[c * Rp + lp(r, f.psh)] = cconj(cmul(lds[c * Rp + lp(r, f.psh)], spec[r]));
+  for (int r0 = lt; r0 < R; r0 += SB * tpc) {
+    T2 w[SB];
+#pragma unroll
+    for (int b = 0; b < SB; ++b) {
+      const int r = r0 + b * tpc;
+      w[b] = r < R ? spec[r] : mkc<T2>(0, 0);
+    }
+#pragma unroll
+    for (int b = 0; b < SB; ++b) {
+      const int r = r0 + b * tpc;
+      if (r < R) {
+        T2* q = lds + c * Rp + lp(r, f.psh);
+        *q = cconj(cmul(*q, w[b]));
+      }
+    }
+  }
   __syncthreads();
   lds_fft<T2, MAXP>(lds, f, stw);
   const long long l = l0 + c;
   const bool live = l < a.nlines;
   T2* out = reinterpret_cast<T2*>(a.out) + (live ? (a.out_off ? a.out_off[l] : l * a.out_stride) : 0);
   const T sc = (T)a.scale;
+  // Output in batches of SB points per thread, table loads (chirp or
+  // inverse permutation) issued first.
   if (a.kind == 2) {
-    for (int k = lt; k < n && live; k += f.tpc) {
-      T2 v = cmul(cconj(lds[c * Rp + lp(k, f.psh)]), chirp[k]);
-      if (a.conj_out) v.y = -v.y;
-      out[k] = cscale(v, sc);
+    for (int k0 = lt; k0 < n && live; k0 += SB * tpc) {
+      T2 w[SB];
+#pragma unroll
+      for (int b = 0; b < SB; ++b) {
+        const int k = k0 + b * tpc;
+        w[b] = k < n ? chirp[k] : mkc<T2>(0, 0);
+      }
+#pragma unroll
+      for (int b = 0; b < SB; ++b) {
+        const int k = k0 + b * tpc;
+        if (k < n) {
+          T2 v = cmul(cconj(lds[c * Rp + lp(k, f.psh)]), w[b]);
+          if (a.conj_out) v.y = -v.y;
+          out[k] = cscale(v, sc);
+        }
+      }
     }
   } else {
     const T2* x = in + (live ? (a.in_off ? a.in_off[l] : l * a.in_stride) : 0);
     T2 x0 = live ? x[0] : mkc<T2>(0, 0);
     if (a.conj_in) x0.y = -x0.y;
     const int off = a.kind == 1 ? q : 0;
-    for (int k = lt; k < q && live; k += f.tpc) {
-      T2 v = cadd(x0, cconj(lds[c * Rp + lp(k + off, f.psh)]));
-      if (a.conj_out) v.y = -v.y;
-      out[a.iperm[k]] = cscale(v, sc);
+    for (int k0 = lt; k0 < q && live; k0 += SB * tpc) {
+      int ip[SB];
+#pragma unroll
+      for (int b = 0; b < SB; ++b) {
+        const int k = k0 + b * tpc;
+        ip[b] = k < q ? a.iperm[k] : 0;
+      }
+#pragma unroll
+      for (int b = 0; b < SB; ++b) {
+        const int k = k0 + b * tpc;
+        if (k < q) {
+          T2 v = cadd(x0, cconj(lds[c * Rp + lp(k + off, f.psh)]));
+          if (a.conj_out) v.y = -v.y;
+          out[ip[b]] = cscale(v, sc);
+        }
+      }
     }
     if (lt == 0 && live) {
       T2 v = cadd(x0, colsum);
