@@ -1034,7 +1034,11 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
     buf[u] = mkc<T2>(0, 0);
     if (live && r < R) {
       const T* q = p + 2 * (long long)u * tpc;
-      buf[u] = a.vec_in ? *reinterpret_cast<const T2*>(q) : mkc<T2>(q[0], q[1]);
+      // frames that overlap (spectrogram) read shared samples twice: keep
+      // those in the L2
+      buf[u] = !a.vec_in ? mkc<T2>(q[0], q[1])
+               : a.in_stride >= 2 * (long long)R ? ldnt(reinterpret_cast<const T2*>(q))
+                                                  : *reinterpret_cast<const T2*>(q);
     }
   }
   // lp(lt + u * tpc) is affine in u when tpc is a whole number of pad blocks
@@ -1076,15 +1080,15 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
       X = mkc<T2>(xer + tr, xei + ti);
     }
     if (a.mode == 0) {
-      reinterpret_cast<T2*>(a.out)[l * a.out_stride + k] = X;
+      stnt(reinterpret_cast<T2*>(a.out) + l * a.out_stride + k, X);
     } else if (a.mode == 2) {
       const bool edge = k == 0 || k == R;  // imaginary part exactly 0 at DC and Nyquist
-      reinterpret_cast<T2*>(a.out)[l * a.out_stride + k] =
-          edge ? mkc<T2>(X.x * (T)a.w_edge, 0) : cscale(X, (T)a.w_mid);
+      stnt(reinterpret_cast<T2*>(a.out) + l * a.out_stride + k,
+           edge ? mkc<T2>(X.x * (T)a.w_edge, 0) : cscale(X, (T)a.w_mid));
     } else {
       T pw = (X.x * X.x + X.y * X.y) * (T)a.psd_scale;
       if (k != 0 && k != R) pw *= (T)2;
-      reinterpret_cast<T*>(a.out)[l * a.out_stride + k] = pw;
+      __builtin_nontemporal_store(pw, reinterpret_cast<T*>(a.out) + l * a.out_stride + k);
     }
   }
 }
