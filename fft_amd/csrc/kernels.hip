@@ -988,7 +988,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
           if (p < a.nvalid) {
             v = cmul(cconj(v), w[b]);
             if (a.conj_out) v.y = -v.y;
-            out[obase + p] = cscale(v, sc);
+            stnt(out + obase + p, cscale(v, sc));  // written once, full lines
           }
         } else {
           const long long idx = p - a.conv_off;
