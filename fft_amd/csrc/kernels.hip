@@ -26,32 +26,6 @@ template <typename T2> __device__ __forceinline__ T2 mkc(decltype(T2{}.x) x, dec
   r.y = y;
   return r;
 }
-
-// Streaming (nontemporal) access for line data, which a launch reads or
-// writes once: the L2 then keeps the reused twiddle, spectrum and chirp
-// tables instead of lines that will not be read again.
-typedef float gofft_v2f __attribute__((ext_vector_type(2)));
-typedef double gofft_v2d __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float2 ldnt(const float2* p) {
-  const gofft_v2f v = __builtin_nontemporal_load(reinterpret_cast<const gofft_v2f*>(p));
-  return mkc<float2>(v.x, v.y);
-}
-__device__ __forceinline__ double2 ldnt(const double2* p) {
-  const gofft_v2d v = __builtin_nontemporal_load(reinterpret_cast<const gofft_v2d*>(p));
-  return mkc<double2>(v.x, v.y);
-}
-__device__ __forceinline__ void stnt(float2* p, float2 v) {
-  gofft_v2f w;
-  w.x = v.x;
-  w.y = v.y;
-  __builtin_nontemporal_store(w, reinterpret_cast<gofft_v2f*>(p));
-}
-__device__ __forceinline__ void stnt(double2* p, double2 v) {
-  gofft_v2d w;
-  w.x = v.x;
-  w.y = v.y;
-  __builtin_nontemporal_store(w, reinterpret_cast<gofft_v2d*>(p));
-}
 template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { return mkc<T2>(a.x + b.x, a.y + b.y); }
 template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { return mkc<T2>(a.x - b.x, a.y - b.y); }
 template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
@@ -488,7 +462,7 @@ __device__ __forceinline__ void mid256(const PassArgs& a, typename Vec2<T>::type
     const T2* p = in + (base >= 0 ? base + s_il[ic] : 0) + (long long)ir0 * a.im.g;
     const long long st = 16 * a.im.g;
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = base >= 0 ? ldnt(p + u * st) : mkc<T2>(0, 0);
+    for (int u = 0; u < 16; ++u) v[u] = base >= 0 ? p[u * st] : mkc<T2>(0, 0);
     double2 tw_w = make_double2(1.0, 0.0), tw_step = make_double2(1.0, 0.0);
     const long long k = a.tw_M ? s_tk[ic] : 0;
     if (k) {
@@ -567,7 +541,7 @@ __device__ __forceinline__ void mid256(const PassArgs& a, typename Vec2<T>::type
     T2 w = v[i];
     if (a.conj_out) w.y = -w.y;
     if (a.scale != 1.0) w = cscale(w, sc);
-    stnt(out + i * st, w);
+    out[i * st] = w;
   }
 }
 
@@ -604,7 +578,7 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
     if (a.in_hook == 0) {
       const T2* p = in + (base >= 0 ? base + il : 0) + (long long)ir0 * g;
 #pragma unroll
-      for (int u = 0; u < A; ++u) v[u] = base >= 0 ? ldnt(p + u * 16 * g) : mkc<T2>(0, 0);
+      for (int u = 0; u < A; ++u) v[u] = base >= 0 ? p[u * 16 * g] : mkc<T2>(0, 0);
     } else if (a.in_hook == 1) {
       const T2* hw = reinterpret_cast<const T2*>(a.hook_w);
 #pragma unroll
@@ -671,7 +645,7 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
       T2 x = v[i];
       if (a.conj_out) x.y = -x.y;
       if (do_scale) x = cscale(x, sc);
-      stnt(out + ol + (long long)(j + A * i) * g, x);
+      out[ol + (long long)(j + A * i) * g] = x;
     }
   } else if (HK == 1) {
     // all chirp loads first (index clamped to 0 past nvalid), then the
@@ -814,7 +788,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
       const T2* p = in + base + il + (long long)ir0 * a.im.g;
       const long long st = (long long)frs * a.im.g;
 #pragma unroll
-      for (int u = 0; u < MAXP; ++u) buf[u] = ir0 + u * frs < R ? ldnt(p + u * st) : mkc<T2>(0, 0);
+      for (int u = 0; u < MAXP; ++u) buf[u] = ir0 + u * frs < R ? p[u * st] : mkc<T2>(0, 0);
     } else {
 #pragma unroll
       for (int u = 0; u < MAXP; ++u) {
@@ -937,7 +911,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
         T2 w = v[i];
         if (a.conj_out) w.y = -w.y;
         if (do_scale) w = cscale(w, sc);
-        stnt(po + i * st, w);
+        po[i * st] = w;
       }
     }
     return;
@@ -953,7 +927,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
         T2 v = lds_aff ? ocol[lp(or0, f.psh) + u * lstep] : ocol[lp(r, f.psh)];
         if (a.conj_out) v.y = -v.y;
         if (do_scale) v = cscale(v, sc);
-        stnt(po + u * st, v);
+        po[u * st] = v;
       }
     }
     return;
@@ -988,7 +962,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
           if (p < a.nvalid) {
             v = cmul(cconj(v), w[b]);
             if (a.conj_out) v.y = -v.y;
-            stnt(out + obase + p, cscale(v, sc));  // written once, full lines
+            out[obase + p] = cscale(v, sc);
           }
         } else {
           const long long idx = p - a.conv_off;
@@ -1034,11 +1008,7 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
     buf[u] = mkc<T2>(0, 0);
     if (live && r < R) {
       const T* q = p + 2 * (long long)u * tpc;
-      // frames that overlap (spectrogram) read shared samples twice: keep
-      // those in the L2
-      buf[u] = !a.vec_in ? mkc<T2>(q[0], q[1])
-               : a.in_stride >= 2 * (long long)R ? ldnt(reinterpret_cast<const T2*>(q))
-                                                  : *reinterpret_cast<const T2*>(q);
+      buf[u] = a.vec_in ? *reinterpret_cast<const T2*>(q) : mkc<T2>(q[0], q[1]);
     }
   }
   // lp(lt + u * tpc) is affine in u when tpc is a whole number of pad blocks
@@ -1080,15 +1050,15 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
       X = mkc<T2>(xer + tr, xei + ti);
     }
     if (a.mode == 0) {
-      stnt(reinterpret_cast<T2*>(a.out) + l * a.out_stride + k, X);
+      reinterpret_cast<T2*>(a.out)[l * a.out_stride + k] = X;
     } else if (a.mode == 2) {
       const bool edge = k == 0 || k == R;  // imaginary part exactly 0 at DC and Nyquist
-      stnt(reinterpret_cast<T2*>(a.out) + l * a.out_stride + k,
-           edge ? mkc<T2>(X.x * (T)a.w_edge, 0) : cscale(X, (T)a.w_mid));
+      reinterpret_cast<T2*>(a.out)[l * a.out_stride + k] =
+          edge ? mkc<T2>(X.x * (T)a.w_edge, 0) : cscale(X, (T)a.w_mid);
     } else {
       T pw = (X.x * X.x + X.y * X.y) * (T)a.psd_scale;
       if (k != 0 && k != R) pw *= (T)2;
-      __builtin_nontemporal_store(pw, reinterpret_cast<T*>(a.out) + l * a.out_stride + k);
+      reinterpret_cast<T*>(a.out)[l * a.out_stride + k] = pw;
     }
   }
 }
