@@ -1609,11 +1609,10 @@ static inline unsigned grid_for(long long total, int block) {
 
 template <typename T> static size_t elt2() { return 2 * sizeof(T); }
 
-#define GOFFT_DISPATCH_MAXP(KERNEL, T, ARGS, BYTES, THREADS, GRID, STREAM)                      \
-  do {                                                                                         \
-    (void)maxp;                                                                                \
-    hipLaunchKernelGGL((KERNEL<T, 16>), dim3(GRID), dim3(THREADS), BYTES, STREAM, ARGS);       \
-  } while (0)
+// The 16-points-per-thread form of a kernel (the only one except k_conv's
+// complex64 24-point form).
+#define GOFFT_LAUNCH16(KERNEL, T, ARGS, BYTES, THREADS, GRID, STREAM) \
+  hipLaunchKernelGGL((KERNEL<T, 16>), dim3(GRID), dim3(THREADS), BYTES, STREAM, ARGS)
 
 hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s) {
   (void)maxp;
@@ -1669,25 +1668,27 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
 }
 
 hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads, hipStream_t s) {
+  (void)maxp;
   long long groups = (a.nlines + a.f.C - 1) / a.f.C;
   if (prec == 0) {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
-    GOFFT_DISPATCH_MAXP(k_real_fwd, double, a, bytes, threads, groups, s);
+    GOFFT_LAUNCH16(k_real_fwd, double, a, bytes, threads, groups, s);
   } else {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
-    GOFFT_DISPATCH_MAXP(k_real_fwd, float, a, bytes, threads, groups, s);
+    GOFFT_LAUNCH16(k_real_fwd, float, a, bytes, threads, groups, s);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads, hipStream_t s) {
+  (void)maxp;
   long long groups = (a.nlines + a.f.C - 1) / a.f.C;
   if (prec == 0) {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
-    GOFFT_DISPATCH_MAXP(k_real_inv, double, a, bytes, threads, groups, s);
+    GOFFT_LAUNCH16(k_real_inv, double, a, bytes, threads, groups, s);
   } else {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
-    GOFFT_DISPATCH_MAXP(k_real_inv, float, a, bytes, threads, groups, s);
+    GOFFT_LAUNCH16(k_real_inv, float, a, bytes, threads, groups, s);
   }
   return hipGetLastError();
 }
@@ -1696,13 +1697,13 @@ hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipSt
   long long groups = (a.nlines + a.f.C - 1) / a.f.C;
   if (prec == 0) {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
-    GOFFT_DISPATCH_MAXP(k_conv, double, a, bytes, threads, groups, s);
+    GOFFT_LAUNCH16(k_conv, double, a, bytes, threads, groups, s);
   } else {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
     if (maxp == 24)
       hipLaunchKernelGGL((k_conv<float, 24>), dim3(groups), dim3(threads), bytes, s, a);
     else
-      GOFFT_DISPATCH_MAXP(k_conv, float, a, bytes, threads, groups, s);
+      GOFFT_LAUNCH16(k_conv, float, a, bytes, threads, groups, s);
   }
   return hipGetLastError();
 }
