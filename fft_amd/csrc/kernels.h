@@ -128,6 +128,18 @@ struct PassArgs {
   const void* stw2;         // stage twiddles of f2
   int mid256;               // f and f2 are both 256 = 16 x 16 with 16 threads per column,
                             // no hooks, output walked along columns (kernels.hip mid256)
+  // Real forward untangle fused into the last pass of the half-length
+  // transform (kernels.hip untangle_store): workgroup w of a line takes
+  // columns j = w*C/2 .. and their mirrors Ns - j, whose outputs hold the
+  // pairs (k, m - k).
+  int real_last;            // 1: this launch is that form
+  long long ru_m;           // m = n/2 complex points per line
+  long long ru_lines;       // lines of this launch
+  long long ru_wpl;         // workgroups per line
+  const void* ru_utw;       // W_n^k, k = 0..m
+  long long ru_ostride;     // output elements between lines
+  int ru_mode;              // 0 bins, 1 one-sided PSD, 2 bins weighted (DC/Nyquist, others)
+  double ru_psd, ru_w_edge, ru_w_mid;
 };
 
 // Real-input forward (RFFT / PSD / spectrogram), single pass: each column is

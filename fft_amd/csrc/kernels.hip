@@ -697,6 +697,70 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
   }
 }
 
+// The real forward untangle (realplan.go:138-169) of the half-length
+// transform's last pass, from LDS to the bins. Workgroup slot c < C/2 holds
+// column j = s_mk[c] (0 <= j <= Ns/2) and slot c + C/2 its mirror Ns - j, so
+// output k = j + r*Ns has its partner m - k at point R-1-r of the mirror
+// (columns 0 and Ns/2 are their own mirrors: points R-r and R-1-r). Each pair
+// is untangled once and both bins are stored; k = 0 gives DC and Nyquist.
+// Threads walk the columns fastest, so stores form contiguous runs.
+template <typename T>
+__device__ __forceinline__ void untangle_store(const PassArgs& a, const typename Vec2<T>::type* lds,
+                                               const long long* s_mk, const long long* s_ln) {
+  typedef typename Vec2<T>::type T2;
+  const SubFFT& f = a.f;
+  const int H = f.C >> 1, log2H = f.log2C - 1, R = f.R, Rp = f.Rp, sh = f.psh;
+  const long long m = a.ru_m, Ns = m / R;
+  const T2* utw = reinterpret_cast<const T2*>(a.ru_utw);
+  const T half = (T)0.5;
+  auto untangle = [&](T2 za, T2 zb, T2 w) {
+    const T xer = (za.x + zb.x) * half, xei = (za.y - zb.y) * half;
+    const T xor_ = (za.y + zb.y) * half, xoi = -(za.x - zb.x) * half;
+    return mkc<T2>(xer + (w.x * xor_ - w.y * xoi), xei + (w.x * xoi + w.y * xor_));
+  };
+  auto emit = [&](long long line, long long k, T2 X) {
+    const bool edge = k == 0 || k == m;
+    if (a.ru_mode == 1) {
+      T p = (X.x * X.x + X.y * X.y) * (T)a.ru_psd;
+      if (!edge) p *= (T)2;
+      reinterpret_cast<T*>(a.out)[line * a.ru_ostride + k] = p;
+    } else {
+      if (a.ru_mode == 2) X = edge ? mkc<T2>(X.x * (T)a.ru_w_edge, 0) : cscale(X, (T)a.ru_w_mid);
+      reinterpret_cast<T2*>(a.out)[line * a.ru_ostride + k] = X;
+    }
+  };
+  for (int idx = threadIdx.x; idx < H * R; idx += blockDim.x) {
+    const int c = idx & (H - 1), r = idx >> log2H;
+    const long long j = s_mk[c];
+    if (j < 0) continue;
+    int rp, cp;
+    if (j == 0) {
+      if (2 * r > R) continue;
+      rp = r == 0 ? 0 : R - r;
+      cp = c;
+    } else if (2 * j == Ns) {
+      if (r > R - 1 - r) continue;
+      rp = R - 1 - r;
+      cp = c;
+    } else {
+      rp = R - 1 - r;
+      cp = c + H;
+    }
+    const T2 zk = lds[c * Rp + lp(r, sh)];
+    const long long line = s_ln[c];
+    const long long k = j + (long long)r * Ns;
+    if (k == 0) {
+      emit(line, 0, mkc<T2>(zk.x + zk.y, 0));
+      emit(line, m, mkc<T2>(zk.x - zk.y, 0));
+      continue;
+    }
+    const T2 zp = lds[cp * Rp + lp(rp, sh)];
+    const long long kp = m - k;
+    emit(line, k, untangle(zk, zp, utw[k]));
+    if (kp != k) emit(line, kp, untangle(zp, zk, utw[kp]));
+  }
+}
+
 // First radix of the two-stage register form of each MODE (0: not that
 // form), and its hook form (reg2's HK).
 constexpr int reg2_radix(int mode) {
@@ -711,7 +775,9 @@ constexpr int reg2_hk(int mode) { return mode >= 5 ? 1 : 2; }
 // 16 threads per column and no hooks (mid256 above). MODES 3, 4: the
 // two-stage register form (reg2 above) with first radix 10, 12, every hook
 // (HK = 2); MODES 6, 7, 5: first radix 10, 12, 16 with the chirp hooks'
-// loads issued up front (HK = 1), for complex64 Bluestein chains.
+// loads issued up front (HK = 1), for complex64 Bluestein chains. MODE 11:
+// the last pass of a real forward transform's half-length FFT with the
+// untangle fused into its store (untangle_store above).
 template <typename T, int MAXP, int MODE>
 __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   constexpr bool EXT = MODE >= 1;
@@ -727,7 +793,38 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   const int tid = threadIdx.x;
   const int C = f.C, R = f.R, Rp = f.Rp;
   const long long g0 = (long long)blockIdx.x * C;
-  if (tid < C) {
+  if constexpr (MODE == 11) {
+    // paired columns: slot c < C/2 takes j = w*C/2 + c (j <= Ns/2), slot
+    // c + C/2 its mirror Ns - j (0 < j < Ns/2)
+    if (tid < C) {
+      const int H = C >> 1;
+      const long long line = blockIdx.x / a.ru_wpl, w = blockIdx.x - line * a.ru_wpl;
+      const long long Ns = a.ru_m / R;
+      const long long jl = w * H + (tid & (H - 1));
+      long long j = -1;
+      if (tid < H) {
+        if (2 * jl <= Ns) j = jl;
+      } else if (jl >= 1 && 2 * jl < Ns) {
+        j = Ns - jl;
+      }
+      if (j >= 0) {
+        long long k = 0, lb = 0, d3 = 0;
+        s_il[tid] = col_digits(a.im, line * Ns + j, &k, a.tw_digit, a.in_off, &lb, &d3);
+        s_ib[tid] = lb;
+        s_tk[tid] = k;
+        s_ln[tid] = d3;
+      } else {
+        s_ib[tid] = -1;
+        s_il[tid] = 0;
+        s_tk[tid] = 0;
+        s_ln[tid] = 0;
+      }
+      s_mk[tid] = tid < H ? j : -1;
+      s_ob[tid] = -1;
+      s_ol[tid] = 0;
+      s_x0[tid] = mkc<T2>(0, 0);
+    }
+  } else if (tid < C) {
     long long g = g0 + tid;
     if (g < a.ncols) {
       long long k = 0, lb = 0, d3 = 0;
@@ -823,7 +920,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   // thread's store points writes straight to memory (radix-16 stages only,
   // so that the register arrays stay static).
   const bool first_reg = !EXT && f.st[0].radix == 16 && R == 16 * frs;
-  const bool last_reg = !EXT && f.nstages >= 2 && f.st[f.nstages - 1].radix == 16 && R == 16 * frs;
+  const bool last_reg = !EXT && MODE != 11 && f.nstages >= 2 && f.st[f.nstages - 1].radix == 16 && R == 16 * frs;
 #pragma unroll
   for (int u = 0; u < MAXP; ++u) {
     const int r = ir0 + u * frs;
@@ -857,6 +954,10 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   __syncthreads();
   lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw), (int)s_tk[tid >> f.log2tpc], first_reg ? 1 : 0,
                     last_reg ? f.nstages - 1 : f.nstages);
+  if constexpr (MODE == 11) {
+    untangle_store<T>(a, lds, s_mk, s_ln);  // every stage above ends with a barrier
+    return;
+  }
   if (mid_spec) {
     // forward output position k of point r of column c: s_mk[c] + r * mm.g
     const T2* spec = reinterpret_cast<const T2*>(mid_spec);
@@ -1619,6 +1720,10 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
   long long groups = (a.ncols + a.f.C - 1) / a.f.C;
   const bool ext = a.in_hook || a.out_hook || a.mid_spec;
   int mode = a.mid256 && prec != 0 ? 2 : ext ? 1 : 0;  // form 2: complex64 only
+  if (a.real_last) {
+    mode = 11;
+    groups = a.ru_lines * a.ru_wpl;
+  }
   // The two-stage register form: R = A x 16 planned radix A first, 16
   // threads per column, plain stage twiddles, no spectrum multiply. Complex64
   // Bluestein chains take the HK = 1 forms (they spill in complex128); the
@@ -1627,7 +1732,7 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
   // already runs both of its stages from registers.
   const SubFFT& f = a.f;
   const int A = f.st[0].radix;
-  if (mode <= 1 && f.nstages == 2 && f.tpc == 16 && f.st[1].radix == 16 && f.R == 16 * A &&
+  if (mode <= 1 && !a.real_last && f.nstages == 2 && f.tpc == 16 && f.st[1].radix == 16 && f.R == 16 * A &&
       f.st[0].cmul == 0 && f.st[1].cmul == 0 && !a.mid_spec) {
     const bool hooks = a.in_hook || a.out_hook;
     const bool rader = a.in_hook == 2 || a.out_hook == 2;
@@ -1642,12 +1747,13 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
   }
 #define GOFFT_PASS(T, M) hipLaunchKernelGGL((k_pass<T, 16, M>), dim3(groups), dim3(threads), bytes, s, a)
   if (prec == 0) {
-    // complex128 has forms 0, 1, 3, 4 (the others are complex64 only)
+    // complex128 has forms 0, 1, 3, 4, 11 (the others are complex64 only)
     const size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
     switch (mode) {
       case 1: GOFFT_PASS(double, 1); break;
       case 3: GOFFT_PASS(double, 3); break;
       case 4: GOFFT_PASS(double, 4); break;
+      case 11: GOFFT_PASS(double, 11); break;
       default: GOFFT_PASS(double, 0); break;
     }
   } else {
@@ -1660,6 +1766,7 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
       case 5: GOFFT_PASS(float, 5); break;
       case 6: GOFFT_PASS(float, 6); break;
       case 7: GOFFT_PASS(float, 7); break;
+      case 11: GOFFT_PASS(float, 11); break;
       default: GOFFT_PASS(float, 0); break;
     }
   }

@@ -726,6 +726,15 @@ struct Ctx {
   char* ws;  // workspace for this engine (ws_bytes() bytes)
 };
 
+// A real forward transform's untangle, fused into the last pass of its
+// half-length multi-pass transform (kernels.hip untangle_store).
+struct RealLast {
+  const void* utw = nullptr;  // W_n^k, k = 0..m
+  i64 ostride = 0;            // output elements between lines
+  int mode = 0;               // 0 bins, 1 one-sided PSD, 2 weighted bins
+  double psd = 1.0, w_edge = 1.0, w_mid = 1.0;
+};
+
 // Launches of one exec per kernel family (GOFFT_KERNEL_*) and the data bytes
 // they move: each launch's input and output lines once, tables excluded.
 struct Work {
@@ -1111,7 +1120,7 @@ class StockhamEngine : public Engine {
   }
 
   int run_chunk(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
-                const Ctx& c, i64 lines) const {
+                const Ctx& c, i64 lines, const RealLast* rl = nullptr) const {
     const size_t one = (size_t)n * (size_t)inner * (size_t)chunk() * elt(prec);
     char* bufs[2] = {c.ws, c.ws + one};
     for (int i = 0; i < P(); i++) {
@@ -1140,7 +1149,60 @@ class StockhamEngine : public Engine {
         if (inner == 1) a.om.a3 = lo.stride;
       }
       a.f = p.sp.f;
-      HIP_TRY(launch_pass(prec, kMaxP, a, p.sp.threads, c.s));
+      int threads = p.sp.threads;
+      if (rl && i == P() - 1) {
+        a.f = pair_sp.f;
+        threads = pair_sp.threads;
+        const i64 H = pair_sp.f.C / 2, Ns = n / p.R;
+        a.real_last = 1;
+        a.ru_m = n;
+        a.ru_lines = lines;
+        a.ru_wpl = (Ns / 2 + 1 + H - 1) / H;
+        a.ru_utw = rl->utw;
+        a.ru_ostride = rl->ostride;
+        a.ru_mode = rl->mode;
+        a.ru_psd = rl->psd;
+        a.ru_w_edge = rl->w_edge;
+        a.ru_w_mid = rl->w_mid;
+        a.out_off = nullptr;
+      }
+      HIP_TRY(launch_pass(prec, kMaxP, a, threads, c.s));
+    }
+    return GOFFT_OK;
+  }
+
+  // Whether run_real can fuse a real forward untangle into the last pass:
+  // multi-pass, 1-D lines, and a schedule of the last pass with twice its
+  // columns (each workgroup takes C columns and their C mirrors, so both
+  // halves still read and write whole 128-byte runs) that fits LDS and keeps
+  // the same stage twiddles.
+  SubPlan pair_sp;
+  bool prepare_untangle() {
+    if (P() < 2 || inner != 1) return false;
+    const SubPlan& last = passes.back().sp;
+    DevInfo dev;
+    if (current_device(&dev) != GOFFT_OK) return false;
+    const int C2 = 2 * last.f.C;
+    if (C2 > kMaxCols || (i64)C2 * pitch(last.f.R) * (i64)elt(prec) > dev.lds_bytes) return false;
+    if (!make_sub(last.f.R, C2, pair_sp) || pair_sp.f.nstages != last.f.nstages || pair_sp.tw != last.tw)
+      return false;
+    for (int i = 0; i < last.f.nstages; i++)
+      if (pair_sp.f.st[i].radix != last.f.st[i].radix) return false;
+    return true;
+  }
+
+  // The m-point transform of `lines` real lines read as complex pairs (istride
+  // complex elements between lines), with the untangle to bins (or PSD) done
+  // by the last pass: out holds ostride output elements per line (complex
+  // bins, or reals for the PSD).
+  int run_real(const void* in, i64 istride, void* out, const RealLast& rl, const Ctx& c, i64 lines) const {
+    const i64 cb = chunk();
+    const size_t eb = elt(prec), ob = rl.mode == 1 ? eb / 2 : eb;
+    for (i64 b0 = 0; b0 < lines; b0 += cb) {
+      const i64 nb = std::min<i64>(cb, lines - b0);
+      const char* ib = (const char*)in + (size_t)(b0 * istride) * eb;
+      char* o = (char*)out + (size_t)(b0 * rl.ostride) * ob;
+      RC_TRY(run_chunk(ib, LineIO{nullptr, istride}, o, LineIO{nullptr, 0}, 0, 0, 1.0, c, nb, &rl));
     }
     return GOFFT_OK;
   }
@@ -1859,26 +1921,37 @@ class RealEngine {
       }
       int rc = 0;
       half = make_engine(m, batch, prec, flags, &rc);
-      return half ? GOFFT_OK : rc;
+      if (!half) return rc;
+      StockhamEngine* se = dynamic_cast<StockhamEngine*>(half.get());
+      fused_last = se && se->prepare_untangle();
+      return GOFFT_OK;
     }
     int rc = 0;
     half = make_engine(n, batch, prec, flags, &rc);
     return half ? GOFFT_OK : rc;
   }
+  bool fused_last = false;  // forward untangle done by the half transform's last pass
 
   size_t ws_bytes() const {
     if (single) return 0;
     size_t hw = half ? half->ws_bytes() : 0;
+    // (the inverse still packs into m points, so the workspace keeps them)
     if (even) return (size_t)m * batch * elt(prec) + hw;
     return 2 * (size_t)n * batch * elt(prec) + hw;
   }
+  // Launches of a forward run (the inverse always adds its pack launch).
   int launches() const {
     if (single) return 1;
-    return half->launches() + (even ? 1 : 2);
+    return half->launches() + (even ? (fused_last ? 0 : 1) : 2);
   }
   std::string describe() const {
     std::string s = "real n=" + std::to_string(n) + " lines=" + std::to_string(batch);
     if (single) return s + " fused " + describe_sub(m, sp);
+    if (even && fused_last) {
+      const StockhamEngine* se = static_cast<const StockhamEngine*>(half.get());
+      return s + " packed: " + half->describe() + "; untangle in the last pass, paired " +
+             describe_sub(se->passes.back().R, se->pair_sp);
+    }
     return s + (even ? " packed: " : " promoted: ") + half->describe();
   }
   // One forward (mode 0 bins, 1 power) or inverse (bins read per line) run.
@@ -1894,6 +1967,10 @@ class RealEngine {
     }
     half->work(w, batch);
     if (even) {
+      if (fused_last && !inverse) {
+        w.add(GOFFT_KERNEL_PASS, 0, batch * (sb - m * e));  // the last pass writes the bins
+        return;
+      }
       w.add(GOFFT_KERNEL_AUX, 1, batch * (inverse ? ib + m * e : m * e + sb));
       return;
     }
@@ -1931,6 +2008,16 @@ class RealEngine {
       if (istride % 2) return fail(GOFFT_ERR_UNSUPPORTED, "odd real line stride");
       char* Z = c.ws;
       Ctx ic{c.s, Z + (size_t)m * batch * elt(prec)};
+      if (fused_last) {
+        RealLast rl;
+        rl.utw = utw.p;
+        rl.ostride = ostride;
+        rl.mode = mode;
+        rl.psd = psd_scale;
+        rl.w_edge = w_edge;
+        rl.w_mid = w_mid;
+        return static_cast<const StockhamEngine*>(half.get())->run_real(in, istride / 2, out, rl, ic, batch);
+      }
       RC_TRY(half->run(in, LineIO{nullptr, istride / 2}, Z, LineIO{nullptr, m}, 0, 0, 1.0, ic));
       HIP_TRY(launch_untangle(prec, Z, out, utw.p, m, batch, m, ostride, mode, psd_scale, w_edge, w_mid, c.s));
       return GOFFT_OK;

@@ -21,8 +21,10 @@ def dev():
     return D, torch
 
 
+# multi-pass half-lengths (untangle fused into the last pass): 1 << 17,
+# 26880 (m = 2^7*3*5*7), 49152 (m = 2^13*3), 62500 (m = 2*5^6), 1 << 20
 RLEN = [1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 15, 16, 17, 31, 32, 1000, 1024, 2046, 4096, 1 << 14, 1 << 17,
-        1 << 20, 20014, 2 * 1009]
+        1 << 20, 20014, 2 * 1009, 26880, 49152, 62500]
 
 
 @pytest.mark.parametrize("n", RLEN)
@@ -41,7 +43,7 @@ def test_rfft_irfft_fp64(dev, oracle, n):
     assert rel_l2(back, wantb) <= bound64(n), (n, rel_l2(back, wantb))
 
 
-@pytest.mark.parametrize("n", [1024, 1 << 20, 1000, 20014])
+@pytest.mark.parametrize("n", [1024, 1 << 20, 1000, 20014, 1 << 17, 49152, 62500])
 def test_rfft_fp32(dev, oracle, n):
     D, torch = dev
     x = rgauss(n, seed=n)
@@ -67,7 +69,7 @@ def test_irfft_short_spectrum_and_dc_nyquist(dev, oracle):
             assert np.abs(got - want).max() <= 1e-12 * max(1, np.abs(want).max()), (n, bins)
 
 
-@pytest.mark.parametrize("n", [32, 15, 1024, 1 << 20, 1000])
+@pytest.mark.parametrize("n", [32, 15, 1024, 1 << 20, 1000, 62500, 26880])
 def test_psd(dev, oracle, n):
     D, torch = dev
     x = rgauss(n, seed=3)
