@@ -108,6 +108,7 @@ struct PassArgs {
   // --- convolution hooks (Rader / Bluestein as three fused passes) -------
   // in_hook 1: element i of a line loads x[i] * w[i] (i < nvalid, else 0)
   // in_hook 2: element i loads x[perm[i]]          (i < nvalid, else 0)
+  // in_hook 3: element i loads x[i]                 (i < nvalid, else 0)
   // out_hook 1: output p = conj(v) * w[p] for p < nvalid (others dropped)
   // out_hook 2: output p -> x0 + conj(v) at iperm[p - conv_off] when
   //             0 <= p - conv_off < nvalid, plus X[0] = x0 + dc[line] at p = 0
@@ -133,13 +134,19 @@ struct PassArgs {
   // columns j = w*C/2 .. and their mirrors Ns - j, whose outputs hold the
   // pairs (k, m - k).
   int real_last;            // 1: this launch is that form
+  // Real inverse pack (realplan.go:221-272) fused into the first pass of the
+  // half-length transform (kernels.hip pack_step): same column pairing, the
+  // bins read through load hook 3 (zero past nvalid = bins), Z conjugated
+  // for the inverse when ru_conj.
+  int real_first;           // 1: this launch is that form
+  int ru_conj;
   long long ru_m;           // m = n/2 complex points per line
   long long ru_lines;       // lines of this launch
   long long ru_wpl;         // workgroups per line
   const void* ru_utw;       // W_n^k, k = 0..m
   long long ru_ostride;     // output elements between lines
   int ru_mode;              // 0 bins, 1 one-sided PSD, 2 bins weighted (DC/Nyquist, others)
-  double ru_psd, ru_w_edge, ru_w_mid;
+  double ru_psd, ru_w_edge, ru_w_mid;  // (real_first: w_mid scales bins 1..m-1)
 };
 
 // Real-input forward (RFFT / PSD / spectrogram), single pass: each column is

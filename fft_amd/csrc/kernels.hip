@@ -761,6 +761,65 @@ __device__ __forceinline__ void untangle_store(const PassArgs& a, const typename
   }
 }
 
+// The real inverse pack (realplan.go:221-272) of the half-length transform's
+// first pass, in LDS: slots hold raw bins X[j + r*M'] (M' = m/R columns per
+// line) with the same column pairing as untangle_store, and each pair
+// (k, m - k) becomes Z[k], Z[m - k] in place (Z[0] from X[0] and the Nyquist
+// bin X[m], loaded into s_x0). Bins 1..m-1 are scaled by w_mid; Z is
+// conjugated when ru_conj (the inverse as conj(FFT(conj))).
+template <typename T>
+__device__ __forceinline__ void pack_step(const PassArgs& a, typename Vec2<T>::type* lds, const long long* s_mk,
+                                          const typename Vec2<T>::type* s_x0) {
+  typedef typename Vec2<T>::type T2;
+  const SubFFT& f = a.f;
+  const int H = f.C >> 1, log2H = f.log2C - 1, R = f.R, Rp = f.Rp, sh = f.psh;
+  const long long m = a.ru_m, Mc = m / R;
+  const T2* utw = reinterpret_cast<const T2*>(a.ru_utw);
+  const T half = (T)0.5, wm = (T)a.ru_w_mid;
+  auto pack = [&](T2 xk, T2 xmk, T2 w) {
+    xk = cscale(xk, wm);
+    xmk = cscale(xmk, wm);
+    const T xer = (xk.x + xmk.x) * half, xei = (xk.y - xmk.y) * half;
+    const T dr = (xk.x - xmk.x) * half, di = (xk.y + xmk.y) * half;
+    const T xor_ = w.x * dr + w.y * di, xoi = w.x * di - w.y * dr;
+    T2 z = mkc<T2>(xer - xoi, xei + xor_);
+    if (a.ru_conj) z.y = -z.y;
+    return z;
+  };
+  for (int idx = threadIdx.x; idx < H * R; idx += blockDim.x) {
+    const int c = idx & (H - 1), r = idx >> log2H;
+    const long long j = s_mk[c];
+    if (j < 0) continue;
+    int rp, cp;
+    if (j == 0) {
+      if (2 * r > R) continue;
+      rp = r == 0 ? 0 : R - r;
+      cp = c;
+    } else if (2 * j == Mc) {
+      if (r > R - 1 - r) continue;
+      rp = R - 1 - r;
+      cp = c;
+    } else {
+      rp = R - 1 - r;
+      cp = c + H;
+    }
+    T2* const pk = lds + c * Rp + lp(r, sh);
+    const long long k = j + (long long)r * Mc;
+    if (k == 0) {
+      const T x0 = (*pk).x, xm = s_x0[c].x;
+      T2 z = mkc<T2>((x0 + xm) * half, (x0 - xm) * half);
+      if (a.ru_conj) z.y = -z.y;
+      *pk = z;
+      continue;
+    }
+    T2* const pp = lds + cp * Rp + lp(rp, sh);
+    const T2 xk = *pk, xmk = *pp;
+    const long long kp = m - k;
+    *pk = pack(xk, xmk, utw[k]);
+    if (kp != k) *pp = pack(xmk, xk, utw[kp]);
+  }
+}
+
 // First radix of the two-stage register form of each MODE (0: not that
 // form), and its hook form (reg2's HK).
 constexpr int reg2_radix(int mode) {
@@ -777,7 +836,9 @@ constexpr int reg2_hk(int mode) { return mode >= 5 ? 1 : 2; }
 // (HK = 2); MODES 6, 7, 5: first radix 10, 12, 16 with the chirp hooks'
 // loads issued up front (HK = 1), for complex64 Bluestein chains. MODE 11:
 // the last pass of a real forward transform's half-length FFT with the
-// untangle fused into its store (untangle_store above).
+// untangle fused into its store (untangle_store above). MODE 12: the first
+// pass of a real inverse's half-length FFT with the pack fused into its load
+// (pack_step above).
 template <typename T, int MAXP, int MODE>
 __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   constexpr bool EXT = MODE >= 1;
@@ -793,9 +854,9 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   const int tid = threadIdx.x;
   const int C = f.C, R = f.R, Rp = f.Rp;
   const long long g0 = (long long)blockIdx.x * C;
-  if constexpr (MODE == 11) {
+  if constexpr (MODE == 11 || MODE == 12) {
     // paired columns: slot c < C/2 takes j = w*C/2 + c (j <= Ns/2), slot
-    // c + C/2 its mirror Ns - j (0 < j < Ns/2)
+    // c + C/2 its mirror Ns - j (0 < j < Ns/2); Ns = m/R columns per line
     if (tid < C) {
       const int H = C >> 1;
       const long long line = blockIdx.x / a.ru_wpl, w = blockIdx.x - line * a.ru_wpl;
@@ -820,9 +881,18 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
         s_ln[tid] = 0;
       }
       s_mk[tid] = tid < H ? j : -1;
-      s_ob[tid] = -1;
-      s_ol[tid] = 0;
       s_x0[tid] = mkc<T2>(0, 0);
+      if constexpr (MODE == 12) {
+        // the first pass stores like any pass; column 0 also needs X[m]
+        long long lb = 0;
+        s_ol[tid] = j >= 0 ? col_digits(a.om, line * Ns + j, nullptr, 0, nullptr, &lb, nullptr) : 0;
+        s_ob[tid] = j >= 0 ? lb : -1;
+        if (j == 0 && a.ru_m < a.nvalid)
+          s_x0[tid] = reinterpret_cast<const T2*>(a.in)[s_ib[tid] + s_il[tid] + a.ru_m];
+      } else {
+        s_ob[tid] = -1;
+        s_ol[tid] = 0;
+      }
     }
   } else if (tid < C) {
     long long g = g0 + tid;
@@ -893,7 +963,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
         buf[u] = mkc<T2>(0, 0);
         const long long i = il + (long long)r * a.im.g;
         if (base >= 0 && r < R && i < a.nvalid)
-          buf[u] = in[base + (in_hook == 2 ? (long long)a.hook_perm[i] : i)];
+          buf[u] = in[base + (in_hook == 2 ? (long long)a.hook_perm[i] : i)];  // hooks 1, 2, 3
       }
     }
   }
@@ -952,6 +1022,10 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
     for (int i = 0; i < 16; ++i) p[i] = buf[i];
   }
   __syncthreads();
+  if constexpr (MODE == 12) {
+    pack_step<T>(a, lds, s_mk, s_x0);
+    __syncthreads();
+  }
   lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw), (int)s_tk[tid >> f.log2tpc], first_reg ? 1 : 0,
                     last_reg ? f.nstages - 1 : f.nstages);
   if constexpr (MODE == 11) {
@@ -1720,8 +1794,8 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
   long long groups = (a.ncols + a.f.C - 1) / a.f.C;
   const bool ext = a.in_hook || a.out_hook || a.mid_spec;
   int mode = a.mid256 && prec != 0 ? 2 : ext ? 1 : 0;  // form 2: complex64 only
-  if (a.real_last) {
-    mode = 11;
+  if (a.real_last || a.real_first) {
+    mode = a.real_last ? 11 : 12;
     groups = a.ru_lines * a.ru_wpl;
   }
   // The two-stage register form: R = A x 16 planned radix A first, 16
@@ -1732,7 +1806,7 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
   // already runs both of its stages from registers.
   const SubFFT& f = a.f;
   const int A = f.st[0].radix;
-  if (mode <= 1 && !a.real_last && f.nstages == 2 && f.tpc == 16 && f.st[1].radix == 16 && f.R == 16 * A &&
+  if (mode <= 1 && f.nstages == 2 && f.tpc == 16 && f.st[1].radix == 16 && f.R == 16 * A &&
       f.st[0].cmul == 0 && f.st[1].cmul == 0 && !a.mid_spec) {
     const bool hooks = a.in_hook || a.out_hook;
     const bool rader = a.in_hook == 2 || a.out_hook == 2;
@@ -1747,13 +1821,14 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
   }
 #define GOFFT_PASS(T, M) hipLaunchKernelGGL((k_pass<T, 16, M>), dim3(groups), dim3(threads), bytes, s, a)
   if (prec == 0) {
-    // complex128 has forms 0, 1, 3, 4, 11 (the others are complex64 only)
+    // complex128 has forms 0, 1, 3, 4, 11, 12 (the others are complex64 only)
     const size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
     switch (mode) {
       case 1: GOFFT_PASS(double, 1); break;
       case 3: GOFFT_PASS(double, 3); break;
       case 4: GOFFT_PASS(double, 4); break;
       case 11: GOFFT_PASS(double, 11); break;
+      case 12: GOFFT_PASS(double, 12); break;
       default: GOFFT_PASS(double, 0); break;
     }
   } else {
@@ -1767,6 +1842,7 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
       case 6: GOFFT_PASS(float, 6); break;
       case 7: GOFFT_PASS(float, 7); break;
       case 11: GOFFT_PASS(float, 11); break;
+      case 12: GOFFT_PASS(float, 12); break;
       default: GOFFT_PASS(float, 0); break;
     }
   }

@@ -726,6 +726,14 @@ struct Ctx {
   char* ws;  // workspace for this engine (ws_bytes() bytes)
 };
 
+// A real inverse transform's pack, fused into the first pass of its
+// half-length multi-pass transform (kernels.hip pack_step).
+struct RealFirst {
+  const void* utw = nullptr;  // W_n^k, k = 0..m
+  i64 bins = 0;               // bins per line present in the input
+  double w_mid = 1.0;         // scale of bins 1..m-1
+};
+
 // A real forward transform's untangle, fused into the last pass of its
 // half-length multi-pass transform (kernels.hip untangle_store).
 struct RealLast {
@@ -1120,7 +1128,7 @@ class StockhamEngine : public Engine {
   }
 
   int run_chunk(const void* in, LineIO li, void* out, LineIO lo, int ci, int co, double scale,
-                const Ctx& c, i64 lines, const RealLast* rl = nullptr) const {
+                const Ctx& c, i64 lines, const RealLast* rl = nullptr, const RealFirst* rf = nullptr) const {
     const size_t one = (size_t)n * (size_t)inner * (size_t)chunk() * elt(prec);
     char* bufs[2] = {c.ws, c.ws + one};
     for (int i = 0; i < P(); i++) {
@@ -1150,6 +1158,22 @@ class StockhamEngine : public Engine {
       }
       a.f = p.sp.f;
       int threads = p.sp.threads;
+      if (rf && i == 0) {
+        a.f = pair_first_sp.f;
+        threads = pair_first_sp.threads;
+        const i64 H = pair_first_sp.f.C / 2, Mc = n / p.R;
+        a.real_first = 1;
+        a.in_hook = 3;  // bins read up to rf->bins, zero beyond
+        a.nvalid = rf->bins;
+        a.ru_conj = a.conj_in;  // conjugation after the pack
+        a.conj_in = 0;
+        a.ru_m = n;
+        a.ru_lines = lines;
+        a.ru_wpl = (Mc / 2 + 1 + H - 1) / H;
+        a.ru_utw = rf->utw;
+        a.ru_w_mid = rf->w_mid;
+        a.in_off = nullptr;
+      }
       if (rl && i == P() - 1) {
         a.f = pair_sp.f;
         threads = pair_sp.threads;
@@ -1176,20 +1200,20 @@ class StockhamEngine : public Engine {
   // columns (each workgroup takes C columns and their C mirrors, so both
   // halves still read and write whole 128-byte runs) that fits LDS and keeps
   // the same stage twiddles.
-  SubPlan pair_sp;
-  bool prepare_untangle() {
-    if (P() < 2 || inner != 1) return false;
-    const SubPlan& last = passes.back().sp;
+  SubPlan pair_sp, pair_first_sp;
+  // A schedule of pass sp with twice its columns and the same stages.
+  bool paired(const SubPlan& sp, SubPlan& out) const {
     DevInfo dev;
     if (current_device(&dev) != GOFFT_OK) return false;
-    const int C2 = 2 * last.f.C;
-    if (C2 > kMaxCols || (i64)C2 * pitch(last.f.R) * (i64)elt(prec) > dev.lds_bytes) return false;
-    if (!make_sub(last.f.R, C2, pair_sp) || pair_sp.f.nstages != last.f.nstages || pair_sp.tw != last.tw)
-      return false;
-    for (int i = 0; i < last.f.nstages; i++)
-      if (pair_sp.f.st[i].radix != last.f.st[i].radix) return false;
+    const int C2 = 2 * sp.f.C;
+    if (C2 > kMaxCols || (i64)C2 * pitch(sp.f.R) * (i64)elt(prec) > dev.lds_bytes) return false;
+    if (!make_sub(sp.f.R, C2, out) || out.f.nstages != sp.f.nstages || out.tw != sp.tw) return false;
+    for (int i = 0; i < sp.f.nstages; i++)
+      if (out.f.st[i].radix != sp.f.st[i].radix) return false;
     return true;
   }
+  bool prepare_untangle() { return P() >= 2 && inner == 1 && paired(passes.back().sp, pair_sp); }
+  bool prepare_pack() { return P() >= 2 && inner == 1 && paired(passes.front().sp, pair_first_sp); }
 
   // The m-point transform of `lines` real lines read as complex pairs (istride
   // complex elements between lines), with the untangle to bins (or PSD) done
@@ -1203,6 +1227,22 @@ class StockhamEngine : public Engine {
       const char* ib = (const char*)in + (size_t)(b0 * istride) * eb;
       char* o = (char*)out + (size_t)(b0 * rl.ostride) * ob;
       RC_TRY(run_chunk(ib, LineIO{nullptr, istride}, o, LineIO{nullptr, 0}, 0, 0, 1.0, c, nb, &rl));
+    }
+    return GOFFT_OK;
+  }
+
+  // The m-point inverse (conj(FFT(conj)) scaled) of `lines` bin lines
+  // (istride complex elements between lines, rf.bins present), packed by the
+  // first pass; out gets ostride complex elements (real pairs) per line.
+  int run_real_inv(const void* in, i64 istride, void* out, i64 ostride, double scale, const RealFirst& rf,
+                   const Ctx& c, i64 lines) const {
+    const i64 cb = chunk();
+    const size_t eb = elt(prec);
+    for (i64 b0 = 0; b0 < lines; b0 += cb) {
+      const i64 nb = std::min<i64>(cb, lines - b0);
+      const char* ib = (const char*)in + (size_t)(b0 * istride) * eb;
+      char* o = (char*)out + (size_t)(b0 * ostride) * eb;
+      RC_TRY(run_chunk(ib, LineIO{nullptr, istride}, o, LineIO{nullptr, ostride}, 1, 1, scale, c, nb, nullptr, &rf));
     }
     return GOFFT_OK;
   }
@@ -1924,13 +1964,15 @@ class RealEngine {
       if (!half) return rc;
       StockhamEngine* se = dynamic_cast<StockhamEngine*>(half.get());
       fused_last = se && se->prepare_untangle();
+      fused_first = se && se->prepare_pack();
       return GOFFT_OK;
     }
     int rc = 0;
     half = make_engine(n, batch, prec, flags, &rc);
     return half ? GOFFT_OK : rc;
   }
-  bool fused_last = false;  // forward untangle done by the half transform's last pass
+  bool fused_last = false;   // forward untangle done by the half transform's last pass
+  bool fused_first = false;  // inverse pack done by the half transform's first pass
 
   size_t ws_bytes() const {
     if (single) return 0;
@@ -1969,6 +2011,10 @@ class RealEngine {
     if (even) {
       if (fused_last && !inverse) {
         w.add(GOFFT_KERNEL_PASS, 0, batch * (sb - m * e));  // the last pass writes the bins
+        return;
+      }
+      if (fused_first && inverse) {
+        w.add(GOFFT_KERNEL_PASS, 0, batch * (ib - m * e));  // the first pass reads the bins
         return;
       }
       w.add(GOFFT_KERNEL_AUX, 1, batch * (inverse ? ib + m * e : m * e + sb));
@@ -2062,6 +2108,14 @@ class RealEngine {
       if (ostride % 2) return fail(GOFFT_ERR_UNSUPPORTED, "odd real line stride");
       char* Z = c.ws;
       Ctx ic{c.s, Z + (size_t)m * batch * elt(prec)};
+      if (fused_first) {
+        RealFirst rf;
+        rf.utw = utw.p;
+        rf.bins = std::max<i64>(0, std::min<i64>(bins, m + 1));
+        rf.w_mid = w_mid;
+        return static_cast<const StockhamEngine*>(half.get())
+            ->run_real_inv(in, istride, out, ostride / 2, gain / (double)m, rf, ic, batch);
+      }
       HIP_TRY(launch_herm_pack(prec, in, Z, utw.p, m, batch, istride, bins, m, w_mid, c.s));
       RC_TRY(half->run(Z, LineIO{nullptr, m}, out, LineIO{nullptr, ostride / 2}, 1, 1, gain / (double)m, ic));
       return GOFFT_OK;

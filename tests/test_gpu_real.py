@@ -57,7 +57,7 @@ def test_rfft_fp32(dev, oracle, n):
 
 def test_irfft_short_spectrum_and_dc_nyquist(dev, oracle):
     D, torch = dev
-    for n in [4, 8, 1024, 1 << 18, 7]:
+    for n in [4, 8, 1024, 1 << 18, 7, 62500, 26880]:
         p = D.R2CPlan(n, 1, torch.float64)
         spec = np.fft.rfft(real_signal(n)) + 0j
         spec[0] += 3j  # imaginary DC / Nyquist are ignored (realplan.go:243-250)
