@@ -43,6 +43,20 @@ for dt, nd in [(torch.complex128, np.complex128), (torch.complex64, np.complex64
     got = q.exec(torch.from_numpy(xs.astype(nd)).to('cuda')).cpu().numpy()
     e = rel_l2(got, O.fftn(xs, shape))
     assert e <= (1e-14 * 18 if nd == np.complex128 else 1e-6), e
+    # real transforms whose pack / untangle run in the half-length transform's
+    # first / last pass, over several chunks of lines
+    rt = torch.float64 if nd == np.complex128 else torch.float32
+    n, batch = 1 << 17, 7
+    xr = np.random.default_rng(9).standard_normal((batch, n))
+    rp = D.R2CPlan(n, batch, rt)
+    X = rp.forward(torch.from_numpy(xr.ravel()).to('cuda', rt)).cpu().numpy().reshape(batch, n // 2 + 1)
+    xs32 = xr if rt == torch.float64 else xr.astype(np.float32).astype(np.float64)
+    want = np.stack([O.rfft(r) for r in xs32])
+    bar = 1e-14 * 17 if rt == torch.float64 else 1e-6
+    assert rel_l2(X, want) <= bar, ('rfft', rt, rel_l2(X, want))
+    back = rp.inverse(torch.from_numpy(want.ravel().astype(nd)).to('cuda')).cpu().numpy().reshape(batch, n)
+    wantb = np.stack([O.irfft(w, n) for w in want])
+    assert rel_l2(back, wantb) <= bar, ('irfft', rt, rel_l2(back, wantb))
 print('chunks ok')
 """
 
