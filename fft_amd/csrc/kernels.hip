@@ -853,7 +853,16 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   const void* const mid_spec = EXT ? a.mid_spec : nullptr;
   const int tid = threadIdx.x;
   const int C = f.C, R = f.R, Rp = f.Rp;
-  const long long g0 = (long long)blockIdx.x * C;
+  // XCD-aware order: the hardware deals workgroup b to XCD b mod 8. When
+  // xcd_group consecutive column groups split 128-byte runs, workgroup b
+  // takes group (b mod 8) * q + b / 8 (q = G / 8; the last G mod 8 keep
+  // their index), so consecutive groups run on one XCD and share its L2.
+  long long blk = blockIdx.x;
+  if (a.xcd_group > 1) {
+    const long long G = gridDim.x, q = G >> 3;
+    if (blk < (q << 3)) blk = (blk & 7) * q + (blk >> 3);
+  }
+  const long long g0 = blk * C;
   if constexpr (MODE == 11 || MODE == 12) {
     // paired columns: slot c < C/2 takes j = w*C/2 + c (j <= Ns/2), slot
     // c + C/2 its mirror Ns - j (0 < j < Ns/2); Ns = m/R columns per line

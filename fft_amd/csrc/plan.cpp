@@ -856,6 +856,7 @@ struct Pass {
   int tw_digit = 0;
   size_t stw_off = 0;  // element offset into the stage-twiddle buffer
   size_t ptw_hi = 0, ptw_lo = 0;  // double2 offsets into the pass-twiddle buffer
+  int xcd_group = 1;  // workgroups that split one 128-byte run, placed on one XCD
 };
 
 // Factor n into pass lengths each <= rmax (13-smooth n), balancing products.
@@ -987,12 +988,34 @@ class StockhamEngine : public Engine {
       p.R = n;
       p.Ns = 1;
       passes.push_back(p);
+    } else if (int Cn = narrow_cols(cwant, pts, sp)) {
+      // Strided lines too long for cwant columns: one pass of Cn columns,
+      // the cwant / Cn workgroups that split each 128-byte run on one XCD.
+      Pass p;
+      p.sp = sp;
+      p.R = n;
+      p.Ns = 1;
+      p.xcd_group = cwant / Cn;
+      passes.push_back(p);
     } else {
       std::vector<i64> lens;
       RC_TRY(multi_lengths(lens));
       return init_lengths(lens);
     }
     return finish();
+  }
+
+  // Columns for one pass over strided lines that do not fit cwant columns:
+  // a power of two below cwant whose runs are still at least 32 bytes (so at
+  // most four workgroups split a 128-byte run); 0 when none fits or the
+  // lines are contiguous. Measured on MI355X: FFT2 4096^2 complex64 (4
+  // columns) 0.219 -> 0.150 ms and 2048^2 0.067 -> 0.044 ms; 16-byte runs
+  // (RFFT2 8192^2's columns, 2 of them) were 9% slower than two passes.
+  int narrow_cols(int cwant, i64 pts, SubPlan& sp) const {
+    if (st == 1 || force_cols > 0) return 0;
+    for (int c = cwant / 2; c >= 2 && c * (i64)elt(prec) >= 32; c /= 2)
+      if (pitch(n) * c <= pts && make_sub((int)n, c, sp)) return c;
+    return 0;
   }
 
   // Pass lengths of a multi-pass decomposition: each <= rmax so that cmp
@@ -1157,6 +1180,7 @@ class StockhamEngine : public Engine {
         if (inner == 1) a.om.a3 = lo.stride;
       }
       a.f = p.sp.f;
+      a.xcd_group = p.xcd_group;
       int threads = p.sp.threads;
       if (rf && i == 0) {
         a.f = pair_first_sp.f;
