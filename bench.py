@@ -212,6 +212,24 @@ def _line(name, ms, flops, nbytes, dtype, config, extra=None):
     print(json.dumps(d), flush=True)
 
 
+def _cpu_baseline(fn, what, reps=3):
+    """The oracle's multicore form of the reference's CPU path (parChunks of
+    parallel.go over every host core), best of `reps`; a reported baseline."""
+    from oracle import oracle as O
+    cores = os.cpu_count() or 1
+    O.set_workers(cores)
+    fn(cores)  # warm (plans, page faults)
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn(cores)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return {"ms": round(best * 1e3, 3), "cores": cores, "kind": "port",
+            "sample": what + " (C++ restatement of go-fft's CPU path with parallel.go's line fan-out), "
+                      "best of %d; host %s" % (reps, os.uname().machine)}
+
+
 def _sinusoids(torch, dev, n, dtype, seed=0):
     """Sum of 64 sinusoids (f ~ U[0, 0.5) cycles/sample, amplitude ~ U[0, 1],
     phase 0) plus N(0, 0.1^2) noise, built on the GPU in FP64."""
@@ -244,7 +262,7 @@ def workload_roundtrip4096(D, torch, dev, steps, warmup):
           "FFT + IFFT, N=4096, batch 256, complex128 (64 MiB in total, resident in the 256 MB MALL)")
 
 
-def workload_spectral(D, torch, dev, steps, warmup):
+def workload_spectral(D, torch, dev, steps, warmup, cpu=True):
     n, seg, ov = 1 << 26, 1024, 512
     win = [0.5 - 0.5 * math.cos(2 * math.pi * k / (seg - 1)) for k in range(seg)]
     for name, rt in (("f32", torch.float32), ("f64", torch.float64)):
@@ -253,8 +271,16 @@ def workload_spectral(D, torch, dev, steps, warmup):
         sp = D.SpectrogramPlan(n, seg, ov, win, 1.0, rt)
         out = torch.empty(sp.frames * sp.bins, dtype=rt, device=dev)
         ms = time_events(lambda: sp.exec(x, out=out), steps, warmup, torch)
+        extra = None
+        if cpu and rt == torch.float64:
+            from oracle import oracle as O
+            xh = x.cpu().numpy()
+            extra = {"cpu_baseline": _cpu_baseline(
+                lambda c: O.spectrogram(xh, seg, ov, np.asarray(win), 1.0, threads=c),
+                "the same Spectrogram, float64")}
+            del xh
         _line("spectrogram", ms, sp.frames * 2.5 * seg * 10, (n + sp.frames * sp.bins) * eb, name,
-              "Spectrogram of 2^26 samples, Hann(1024), overlap 512: %d frames x 513 bins" % sp.frames)
+              "Spectrogram of 2^26 samples, Hann(1024), overlap 512: %d frames x 513 bins" % sp.frames, extra)
         del sp, out
         pp = D.PSDPlan(n, 1, 1.0, rt)
         out = torch.empty(n // 2 + 1, dtype=rt, device=dev)
@@ -274,7 +300,7 @@ def workload_spectral(D, torch, dev, steps, warmup):
         torch.cuda.empty_cache()
 
 
-def workload_nd(D, torch, dev, steps, warmup):
+def workload_nd(D, torch, dev, steps, warmup, cpu=True):
     L = D.L
     for shape, dts in (((4096, 4096), (torch.complex64, torch.complex128)), ((256, 256, 256), (torch.complex64,))):
         N = int(np.prod(shape))
@@ -286,8 +312,15 @@ def workload_nd(D, torch, dev, steps, warmup):
             y = torch.empty_like(x)
             ms = time_events(lambda: p.exec(x, out=y), steps, warmup, torch)
             eb = 8 if ct == torch.complex64 else 16
+            extra = None
+            if cpu and ct == torch.complex128 and len(shape) == 2:
+                from oracle import oracle as O
+                xh = x.cpu().numpy()
+                extra = {"cpu_baseline": _cpu_baseline(lambda c: O.fftn(xh, list(shape)),
+                                                       "the same FFT2, complex128")}
+                del xh
             _line("fft2" if len(shape) == 2 else "fftn", ms, 5.0 * N * math.log2(N), 2 * N * eb,
-                  "c64" if eb == 8 else "c128", "x".join(map(str, shape)) + " complex, re/im N(0,1)")
+                  "c64" if eb == 8 else "c128", "x".join(map(str, shape)) + " complex, re/im N(0,1)", extra)
             del p, x, y
             torch.cuda.empty_cache()
     rows = cols = 8192
@@ -428,7 +461,11 @@ def main():
     torch.cuda.set_device(dev)
     if args.workload != "mixed":
         for name in (sorted(WORKLOADS) if args.workload == "all" else [args.workload]):
-            WORKLOADS[name](D, torch, dev, args.steps, args.warmup)
+            fn = WORKLOADS[name]
+            if name in ("spectral", "nd"):
+                fn(D, torch, dev, args.steps, args.warmup, cpu=not args.no_cpu)
+            else:
+                fn(D, torch, dev, args.steps, args.warmup)
         return
     lengths = workload_lengths(0)
     total = sum(lengths)
