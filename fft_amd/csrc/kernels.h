@@ -211,6 +211,9 @@ struct ConvArgs {
 hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s);
 hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads, hipStream_t s);
 hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads, hipStream_t s);
+// Vector registers per thread of the single-pass real kernels (the larger of
+// k_real_fwd and k_real_inv) at precision prec, from the code object.
+int real_kernel_vgprs(int prec);
 hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipStream_t s);
 
 // Element-wise helpers for the multi-pass convolution and real paths.

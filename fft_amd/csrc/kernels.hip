@@ -1872,6 +1872,20 @@ hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads
   return hipGetLastError();
 }
 
+int real_kernel_vgprs(int prec) {
+  hipFuncAttributes fa{}, ia{};
+  hipError_t e1, e2;
+  if (prec == 0) {
+    e1 = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_real_fwd<double, 16>));
+    e2 = hipFuncGetAttributes(&ia, reinterpret_cast<const void*>(&k_real_inv<double, 16>));
+  } else {
+    e1 = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_real_fwd<float, 16>));
+    e2 = hipFuncGetAttributes(&ia, reinterpret_cast<const void*>(&k_real_inv<float, 16>));
+  }
+  if (e1 != hipSuccess || e2 != hipSuccess) return 0;
+  return fa.numRegs > ia.numRegs ? fa.numRegs : ia.numRegs;
+}
+
 hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads, hipStream_t s) {
   (void)maxp;
   long long groups = (a.nlines + a.f.C - 1) / a.f.C;

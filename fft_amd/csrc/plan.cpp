@@ -1956,6 +1956,41 @@ std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, 
 // Real-input engine (realplan.go): even n packs pairs into an n/2-point complex
 // transform plus an untangle; odd n promotes to a full complex transform.
 
+// Columns per workgroup of the single-pass real kernels (k_real_fwd,
+// k_real_inv). The choice of pick_cols (about 64 KB of LDS per workgroup)
+// stands unless it leaves a CU under 60% of the waves the kernels' registers
+// allow; then the column count with the most resident waves wins (fewest
+// columns on ties). A SIMD holds 512 / (registers per thread, in granules
+// of 8) waves, at most 8, and a CU four SIMDs; a CU also holds only as many
+// workgroups as fit its LDS. Measured: complex128 frames of 512 points (126
+// registers) ran 11-19% faster at one column (16 waves per CU) than at four
+// (8 waves); complex64 (84 registers, 16 of 20 waves at eight columns) ran
+// 5-13% slower at one column, so it keeps pick_cols' choice.
+bool real_cols(int R, int cap, i64 pts, i64 budget, int prec, const DevInfo& dev, SubPlan& sp, int* Cout) {
+  if (!pick_cols(R, cap, budget, sp, Cout)) return false;
+  const int vg = real_kernel_vgprs(prec);
+  if (vg <= 0) return true;
+  const int wave_cap = 4 * std::min(8, 512 / ((vg + 7) / 8 * 8));
+  auto waves = [&](const SubPlan& t, int C) {
+    const i64 waves_wg = (t.threads + 63) / 64;
+    const i64 wgs = std::max<i64>(1, (i64)dev.lds_full / ((i64)C * pitch(R) * (i64)elt(prec)));
+    return (int)std::min<i64>(wave_cap, wgs * waves_wg);
+  };
+  if (waves(sp, *Cout) * 5 >= wave_cap * 3) return true;
+  int bestW = -1;
+  for (int C = 1; C <= cap && (i64)C * pitch(R) <= pts; C *= 2) {
+    SubPlan t;
+    if (!make_sub(R, C, t)) continue;
+    const int w = waves(t, C);
+    if (w > bestW) {
+      bestW = w;
+      sp = t;
+      *Cout = C;
+    }
+  }
+  return true;
+}
+
 class RealEngine {
  public:
   int prec = 0;
@@ -1977,8 +2012,9 @@ class RealEngine {
       const i64 pts = dev.lds_bytes / (i64)elt(prec);
       const i64 budget = std::min<i64>(pts, std::max<i64>(pitch(m), soft_pts(prec)));
       int C = 0;
+      const int cap = (int)std::min<i64>(kMaxCols, std::max<i64>(1, batch));
       if (pitch(m) <= pts && factors_are_small(m) &&
-          pick_cols((int)m, (int)std::min<i64>(kMaxCols, std::max<i64>(1, batch)), budget, sp, &C)) {
+          real_cols((int)m, cap, pts, budget, prec, dev, sp, &C)) {
         single = true;
         RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
         return GOFFT_OK;
