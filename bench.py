@@ -112,7 +112,8 @@ def run_dtype(D, torch, dev, name, lengths, x64, steps, warmup):
     torch.cuda.synchronize()
     work = {k: plan.kernel_work(k) for k in (L.KERNEL_PASS, L.KERNEL_CONV, L.KERNEL_REAL, L.KERNEL_AUX)}
     res = {"ms": ms, "ms_eager": ms_eager, "gflops": flops_of(lengths) / (ms * 1e6),
-           "gbs": 2.0 * total * elt / (ms * 1e6), "launches": plan.num_launches(), "work": work, "groups": {}}
+           "gbs": 2.0 * total * elt / (ms * 1e6), "launches": plan.num_launches(), "work": work, "groups": {},
+           "out": out.cpu().numpy()}  # the last replay's output, for the accuracy figures
     del graph, plan, xin, out
     counts = {}
     for n in lengths:
@@ -191,12 +192,48 @@ def cpu_leg(lengths, x64):
             O.fft(v)
         dt = time.perf_counter() - t0
         best = dt if best is None else min(best, dt)
+    ys = [O.fft(v) for v in xs]  # untimed: the outputs the accuracy figures compare against
     return {"value": round(flops_of(lengths) / best / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "port",
             "sample": "the whole step: all 4096 mixed-length transforms, complex128, one thread, serial "
                       "Plan.FFT per transform (C++ restatement of go-fft's CPU algorithm, -O2 "
                       "-ffp-contract=off), best of 3; host %s with %d logical cores"
                       % (os.uname().machine, os.cpu_count() or 0),
-            "seconds": round(best, 3)}
+            "seconds": round(best, 3)}, ys
+
+
+def accuracy(lengths, y32, y64, ref=None):
+    """Per length: the worst per-transform relative L2 error and the largest
+    absolute error of the FP32 and FP64 step outputs, against the CPU leg's
+    oracle outputs on the same inputs when `ref` holds them, else (FP32 only)
+    against the FP64 GPU output."""
+    per = {}
+    off = 0
+    for i, n in enumerate(lengths):
+        want = ref[i] if ref is not None else y64[off: off + n]
+        den = max(float(np.linalg.norm(want)), 1e-300)
+        row = per.setdefault(n, {"transforms": 0})
+        row["transforms"] += 1
+        for name, y in (("c64", y32), ("c128", y64 if ref is not None else None)):
+            if y is None:
+                continue
+            d = y[off: off + n].astype(np.complex128) - want
+            e = float(np.linalg.norm(d)) / den
+            m = float(np.abs(d).max())
+            row[name + "_rel_l2"] = max(row.get(name + "_rel_l2", 0.0), e)
+            row[name + "_max_abs"] = max(row.get(name + "_max_abs", 0.0), m)
+        off += n
+    for n, row in per.items():
+        for k in list(row):
+            if k != "transforms":
+                row[k] = float("%.3g" % row[k])
+        row["c64_bar"] = 1e-6
+        if ref is not None:
+            row["c128_bar"] = float("%.3g" % (1e-14 * math.log2(n)))
+    return {"reference": "the oracle's outputs from the CPU leg (FP64 restatement of go-fft, same inputs)"
+                         if ref is not None else "the FP64 GPU path's outputs (same inputs)",
+            "rel_l2": "worst transform of the length", "max_abs": "largest |error| over the length's transforms "
+                                                                  "(inputs re/im N(0,1))",
+            "per_length": {str(n): per[n] for n in sorted(per)}}
 
 
 # ---------------------------------------------------------------------------
@@ -230,6 +267,14 @@ def _cpu_baseline(fn, what, reps=3):
                       "best of %d; host %s" % (reps, os.uname().machine)}
 
 
+def _err(y, ref, what="the FP64 GPU path, same input"):
+    """Relative L2 and largest absolute error of y against ref (both on the
+    device), in FP64."""
+    d = y.reshape(-1).to(ref.dtype) - ref.reshape(-1)
+    return {"rel_l2": float("%.3g" % (d.norm().item() / max(ref.norm().item(), 1e-300))),
+            "max_abs": float("%.3g" % d.abs().max().item()), "reference": what}
+
+
 def _sinusoids(torch, dev, n, dtype, seed=0):
     """Sum of 64 sinusoids (f ~ U[0, 0.5) cycles/sample, amplitude ~ U[0, 1],
     phase 0) plus N(0, 0.1^2) noise, built on the GPU in FP64."""
@@ -259,25 +304,33 @@ def workload_roundtrip4096(D, torch, dev, steps, warmup):
         p.exec(y, out=z, inverse=True)
     ms = time_events(step, steps, warmup, torch)
     _line("roundtrip4096", ms, 2 * batch * 5.0 * n * 12, 4 * n * batch * 16, "c128",
-          "FFT + IFFT, N=4096, batch 256, complex128 (64 MiB in total, resident in the 256 MB MALL)")
+          "FFT + IFFT, N=4096, batch 256, complex128 (64 MiB in total, resident in the 256 MB MALL)",
+          {"roundtrip_err": _err(z, x, "the input (IFFT(FFT(x)) = x)")})
 
 
 def workload_spectral(D, torch, dev, steps, warmup, cpu=True):
     n, seg, ov = 1 << 26, 1024, 512
     win = [0.5 - 0.5 * math.cos(2 * math.pi * k / (seg - 1)) for k in range(seg)]
+    x64 = _sinusoids(torch, dev, n, torch.float64)
+    f64 = torch.float64
     for name, rt in (("f32", torch.float32), ("f64", torch.float64)):
-        x = _sinusoids(torch, dev, n, rt)
+        x = x64 if rt == f64 else x64.to(rt)
+
+        def vs64(y, run64):
+            """FP32 lines: the error against the FP64 plan on the same signal
+            (untimed)."""
+            return {} if rt == f64 else {"err_vs_fp64": _err(y, run64())}
         eb = 4 if rt == torch.float32 else 8
         sp = D.SpectrogramPlan(n, seg, ov, win, 1.0, rt)
         out = torch.empty(sp.frames * sp.bins, dtype=rt, device=dev)
         ms = time_events(lambda: sp.exec(x, out=out), steps, warmup, torch)
-        extra = None
-        if cpu and rt == torch.float64:
+        extra = vs64(out, lambda: D.SpectrogramPlan(n, seg, ov, win, 1.0, f64).exec(x64))
+        if cpu and rt == f64:
             from oracle import oracle as O
             xh = x.cpu().numpy()
-            extra = {"cpu_baseline": _cpu_baseline(
+            extra["cpu_baseline"] = _cpu_baseline(
                 lambda c: O.spectrogram(xh, seg, ov, np.asarray(win), 1.0, threads=c),
-                "the same Spectrogram, float64")}
+                "the same Spectrogram, float64")
             del xh
         _line("spectrogram", ms, sp.frames * 2.5 * seg * 10, (n + sp.frames * sp.bins) * eb, name,
               "Spectrogram of 2^26 samples, Hann(1024), overlap 512: %d frames x 513 bins" % sp.frames, extra)
@@ -285,56 +338,71 @@ def workload_spectral(D, torch, dev, steps, warmup, cpu=True):
         pp = D.PSDPlan(n, 1, 1.0, rt)
         out = torch.empty(n // 2 + 1, dtype=rt, device=dev)
         ms = time_events(lambda: pp.exec(x, out=out), steps, warmup, torch)
-        _line("psd", ms, 2.5 * n * 26, (n + n // 2 + 1) * eb, name, "one-sided PSD of 2^26 samples")
+        extra = vs64(out, lambda: D.PSDPlan(n, 1, 1.0, f64).exec(x64))
+        _line("psd", ms, 2.5 * n * 26, (n + n // 2 + 1) * eb, name, "one-sided PSD of 2^26 samples", extra)
         del pp, out
         m = 1 << 24
         rp = D.R2CPlan(m, 1, rt)
         xs = x[:m].contiguous()
         X = rp.forward(xs)
         ms = time_events(lambda: rp.forward(xs, out=X), steps, warmup, torch)
-        _line("rfft", ms, 2.5 * m * 24, m * eb + (m // 2 + 1) * 2 * eb, name, "RFFT of 2^24 samples")
+        extra = vs64(X, lambda: D.R2CPlan(m, 1, f64).forward(x64[:m].contiguous()))
+        _line("rfft", ms, 2.5 * m * 24, m * eb + (m // 2 + 1) * 2 * eb, name, "RFFT of 2^24 samples", extra)
         back = torch.empty_like(xs)
         ms = time_events(lambda: rp.inverse(X, out=back), steps, warmup, torch)
-        _line("irfft", ms, 2.5 * m * 24, m * eb + (m // 2 + 1) * 2 * eb, name, "IRFFT to 2^24 samples")
+        # the FP32 inverse of the FP32 spectrum against the FP64 inverse of
+        # the same (widened) spectrum
+        extra = vs64(back, lambda: D.R2CPlan(m, 1, f64).inverse(X.to(torch.complex128)))
+        _line("irfft", ms, 2.5 * m * 24, m * eb + (m // 2 + 1) * 2 * eb, name, "IRFFT to 2^24 samples", extra)
         del rp, X, back, xs, x
         torch.cuda.empty_cache()
+    del x64
 
 
 def workload_nd(D, torch, dev, steps, warmup, cpu=True):
     L = D.L
-    for shape, dts in (((4096, 4096), (torch.complex64, torch.complex128)), ((256, 256, 256), (torch.complex64,))):
+    c64, c128 = torch.complex64, torch.complex128
+    for shape, dts in (((4096, 4096), (c64, c128)), ((256, 256, 256), (c64,))):
         N = int(np.prod(shape))
+        g = torch.Generator(device=dev)
+        g.manual_seed(0)
+        x128 = torch.randn(N, dtype=c128, device=dev, generator=g)
         for ct in dts:
-            g = torch.Generator(device=dev)
-            g.manual_seed(0)
-            x = torch.randn(N, dtype=ct, device=dev, generator=g)
+            x = x128 if ct == c128 else x128.to(ct)
             p = D.NDPlan(shape, L.ND_C2C, 1, ct)
             y = torch.empty_like(x)
             ms = time_events(lambda: p.exec(x, out=y), steps, warmup, torch)
-            eb = 8 if ct == torch.complex64 else 16
-            extra = None
-            if cpu and ct == torch.complex128 and len(shape) == 2:
+            eb = 8 if ct == c64 else 16
+            extra = {}
+            if ct == c64:  # against the FP64 plan on the same input (untimed)
+                extra["err_vs_fp64"] = _err(y, D.NDPlan(shape, L.ND_C2C, 1, c128).exec(x128))
+            if cpu and ct == c128 and len(shape) == 2:
                 from oracle import oracle as O
                 xh = x.cpu().numpy()
-                extra = {"cpu_baseline": _cpu_baseline(lambda c: O.fftn(xh, list(shape)),
-                                                       "the same FFT2, complex128")}
+                extra["cpu_baseline"] = _cpu_baseline(lambda c: O.fftn(xh, list(shape)),
+                                                      "the same FFT2, complex128")
                 del xh
             _line("fft2" if len(shape) == 2 else "fftn", ms, 5.0 * N * math.log2(N), 2 * N * eb,
-                  "c64" if eb == 8 else "c128", "x".join(map(str, shape)) + " complex, re/im N(0,1)", extra)
+                  "c64" if eb == 8 else "c128",
+                  "x".join(map(str, shape)) + " complex normal (torch.randn, re/im N(0, 1/2))", extra)
             del p, x, y
             torch.cuda.empty_cache()
+        del x128
     rows = cols = 8192
     g = torch.Generator(device=dev)
     g.manual_seed(0)
     x = torch.rand(rows * cols, dtype=torch.float32, device=dev, generator=g)
-    p = D.NDPlan((rows, cols), L.ND_R2C, 1, torch.complex64)
+    p = D.NDPlan((rows, cols), L.ND_R2C, 1, c64)
     X = p.rfft2(x)
     ms = time_events(lambda: p.rfft2(x, out=X), steps, warmup, torch)
     nb = rows * cols * 4 + rows * (cols // 2 + 1) * 8
-    _line("rfft2", ms, 2.5 * rows * cols * 26, nb, "f32", "8192x8192 float32 U[0,1)")
+    p64 = D.NDPlan((rows, cols), L.ND_R2C, 1, c128)  # FP64 references on the same inputs (untimed)
+    _line("rfft2", ms, 2.5 * rows * cols * 26, nb, "f32", "8192x8192 float32 U[0,1)",
+          {"err_vs_fp64": _err(X, p64.rfft2(x.double()))})
     back = torch.empty_like(x)
     ms = time_events(lambda: p.irfft2(X, out=back), steps, warmup, torch)
-    _line("irfft2", ms, 2.5 * rows * cols * 26, nb, "f32", "8192x8192 -> 8192x4097 bins and back")
+    _line("irfft2", ms, 2.5 * rows * cols * 26, nb, "f32", "8192x8192 -> 8192x4097 bins and back",
+          {"err_vs_fp64": _err(back, p64.irfft2(X.to(c128)))})
 
 
 def workload_blur(D, torch, dev, steps, warmup):
@@ -365,11 +433,25 @@ def workload_blur(D, torch, dev, steps, warmup):
         gS.mul_(H)
         p.vjp(gS, gx, direction=0)
     ms = time_events(step, steps, warmup, torch)
+    # The same gradient through FP64 plans on the same x, H and y (untimed).
+    # The residual subtracts nearly equal values, which scales FP32 rounding
+    # up, so the adjoint transforms are also compared on their own: both
+    # precisions applied to the FP32 step's residual r.
+    p64 = D.NDPlan((rows, cols), L.ND_R2C, batch, torch.complex128)
+    H64 = H.double()
+
+    def grad64(res):
+        gS64 = p64.vjp(res, torch.empty_like(S, dtype=torch.complex128), direction=1)
+        return p64.vjp(gS64 * H64, torch.empty_like(res), direction=0)
+    err = _err(gx, grad64(2.0 * (p64.irfft2(p64.rfft2(x.double()) * H64) - y.double())))
+    err_adj = _err(gx, grad64(r.double()), "the FP64 adjoint transforms on the FP32 step's residual")
+    del p64, H64
     per = 2.5 * rows * cols * 20 * batch
     passb = batch * (rows * cols * 4 + rows * rc * 8)
     _line("blur_grad", ms, 4 * per, 4 * passb, "f32",
           "loss ||IRFFT2(RFFT2(x) H) - y||^2 and its gradient, 64 x 1024x1024 float32, Gaussian H sigma 3 px; "
-          "one step = RFFT2, IRFFT2 and the two adjoint transforms plus elementwise work")
+          "one step = RFFT2, IRFFT2 and the two adjoint transforms plus elementwise work",
+          {"err_vs_fp64": err, "adjoint_err_vs_fp64": err_adj})
 
 
 WORKLOADS = {"roundtrip4096": workload_roundtrip4096, "spectral": workload_spectral, "nd": workload_nd,
@@ -460,12 +542,18 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     if args.workload != "mixed":
+        # Each of these lines times a freshly built plan on freshly allocated
+        # buffers, whose first few dozen calls run up to 7% slower than the
+        # steady state (measured: 20-call means 0.517, 0.488, 0.482, 0.482 ms
+        # for the float64 Spectrogram after 3 warm-up calls), so they warm
+        # up at least 20 calls.
+        warm = max(args.warmup, 20)
         for name in (sorted(WORKLOADS) if args.workload == "all" else [args.workload]):
             fn = WORKLOADS[name]
             if name in ("spectral", "nd"):
-                fn(D, torch, dev, args.steps, args.warmup, cpu=not args.no_cpu)
+                fn(D, torch, dev, args.steps, warm, cpu=not args.no_cpu)
             else:
-                fn(D, torch, dev, args.steps, args.warmup)
+                fn(D, torch, dev, args.steps, warm)
         return
     lengths = workload_lengths(0)
     total = sum(lengths)
@@ -476,7 +564,9 @@ def main():
     r64 = run_dtype(D, torch, dev, "fp64", lengths, x64, args.steps, args.warmup)
     roofline = pass_roofline(r32, D.L, args.steps, args.warmup)
     copy = copy_bandwidth(torch, dev, args.steps, args.warmup)
-    cpu = None if args.no_cpu else cpu_leg(lengths, x64)
+    cpu, ref = (None, None) if args.no_cpu else cpu_leg(lengths, x64)
+    acc = accuracy(lengths, r32.pop("out"), r64.pop("out"), ref)
+    del ref
 
     line = {
         "metric": "FFT GFLOP/s (5N*log2N) & HBM GB/s: FP32 batched vs FP64 plain, 1 GPU",
@@ -510,6 +600,7 @@ def main():
         "copy_1GiB": copy,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "accuracy": acc,
     }
     print(json.dumps(line), flush=True)
 
