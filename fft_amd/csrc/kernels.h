@@ -139,7 +139,7 @@ struct PassArgs {
   // bins read through load hook 3 (zero past nvalid = bins), Z conjugated
   // for the inverse when ru_conj.
   int real_first;           // 1: this launch is that form
-  int xcd_group;            // > 1: workgroups b*xcd_group .. share 128-byte runs; keep them on one XCD
+  int xcd_group;            // > 1: consecutive column groups share cache lines; keep them on one XCD
   int ru_conj;
   long long ru_m;           // m = n/2 complex points per line
   long long ru_lines;       // lines of this launch

@@ -854,9 +854,10 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
   const int tid = threadIdx.x;
   const int C = f.C, R = f.R, Rp = f.Rp;
   // XCD-aware order: the hardware deals workgroup b to XCD b mod 8. When
-  // xcd_group consecutive column groups split 128-byte runs, workgroup b
-  // takes group (b mod 8) * q + b / 8 (q = G / 8; the last G mod 8 keep
-  // their index), so consecutive groups run on one XCD and share its L2.
+  // consecutive column groups share cache lines (xcd_group > 1: they split
+  // 128-byte runs, or read and write unaligned lines), workgroup b takes
+  // group (b mod 8) * q + b / 8 (q = G / 8; the last G mod 8 keep their
+  // index), so consecutive groups run on one XCD and share its L2.
   long long blk = blockIdx.x;
   if (a.xcd_group > 1) {
     const long long G = gridDim.x, q = G >> 3;
