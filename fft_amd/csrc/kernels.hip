@@ -869,7 +869,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
     // c + C/2 its mirror Ns - j (0 < j < Ns/2); Ns = m/R columns per line
     if (tid < C) {
       const int H = C >> 1;
-      const long long line = blockIdx.x / a.ru_wpl, w = blockIdx.x - line * a.ru_wpl;
+      const long long line = blk / a.ru_wpl, w = blk - line * a.ru_wpl;
       const long long Ns = a.ru_m / R;
       const long long jl = w * H + (tid & (H - 1));
       long long j = -1;
