@@ -1194,9 +1194,13 @@ class StockhamEngine : public Engine {
       // Consecutive groups then go to one XCD and meet in its L2; partial
       // lines written from two XCDs cost the most. Measured on the 4096-
       // transform mixed-length batch: FP32 0.850 -> 0.786 ms, FP64 2.173 ->
-      // 2.094 ms.
+      // 2.094 ms. The same holds in any pass whose columns per line are not
+      // a multiple of C, where a group's runs straddle two lines: 40000 x
+      // 256 complex64 0.106 -> 0.100 ms, 20000 x 256 complex128 0.114 ->
+      // 0.111 ms.
       if (P() > 1 && inner == 1 &&
-          ((i == 0 && unaligned_lines(in, li)) || (i == P() - 1 && unaligned_lines(out, lo))))
+          ((i == 0 && unaligned_lines(in, li)) || (i == P() - 1 && unaligned_lines(out, lo)) ||
+           (n / p.R) % p.sp.f.C != 0))
         a.xcd_group = std::max(a.xcd_group, 2);
       int threads = p.sp.threads;
       if (rf && i == 0) {
