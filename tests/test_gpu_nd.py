@@ -49,8 +49,27 @@ def test_fft2_4096_fp32(dev, oracle):
     assert rel_l2(got, want) <= 1e-6
 
 
+# Strided axes long enough for several passes, over rows that are not a
+# whole number of 128-byte lines or of column groups (such passes place
+# consecutive column groups on one XCD).
+@pytest.mark.parametrize("shape", [[40000, 3], [20000, 5], [3, 40000, 2], [65536, 3]])
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_fftn_long_strided_axes(dev, oracle, shape, prec):
+    D, torch = dev
+    total = int(np.prod(shape))
+    x = cgauss(total, seed=total % 997)
+    dt, nd = (torch.complex128, np.complex128) if prec == "c128" else (torch.complex64, np.complex64)
+    p = D.NDPlan(shape, 0, 1, dt)
+    xt = torch.from_numpy(x.astype(nd)).to("cuda")
+    bar = bound64(total) if prec == "c128" else 1e-6
+    for inverse in (False, True):
+        got = p.exec(xt, inverse=inverse).cpu().numpy().astype(np.complex128)
+        e = rel_l2(got, oracle.fftn(x, shape, inverse=inverse))
+        assert e <= bar, (shape, prec, inverse, e, p.describe())
+
+
 @pytest.mark.parametrize("shape", [(4, 6), (3, 4), (5, 8), (2, 2), (4, 5), (1, 6), (6, 1), (7, 3),
-                                   (64, 128), (512, 1024), (1024, 1024)])
+                                   (64, 128), (512, 1024), (1024, 1024), (20000, 6), (16384, 10)])
 def test_rfft2_irfft2_fp64(dev, oracle, shape):
     D, torch = dev
     r, c = shape
