@@ -1,0 +1,69 @@
+"""GPU parity for complex transforms above 2^20 points: schedules of three
+passes, two-level pass twiddles (products formed in FP64 and, for complex64,
+rounded once), and radix-3 / radix-5-only lengths. Against the oracle on the
+same seeded input, forward and inverse: complex128 within rel-L2
+1e-14*log2(n), complex64 within 1e-6. At 2^25 points (complex64), where the
+oracle would take longer than these tests should, the check is the round
+trip IFFT(FFT(x)) = x within 1e-6 plus Parseval's sum within 1e-6.
+"""
+import functools
+import math
+
+import numpy as np
+import pytest
+
+from conftest import cgauss, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CASES = [1 << 22, 1 << 24, 3 ** 13, 1244160, 5 ** 9]
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from fft_amd import device as D
+    return D, torch
+
+
+@functools.lru_cache(maxsize=None)
+def _case(n):
+    from oracle import oracle as O
+    x = cgauss(n, seed=n % 101)
+    return x, O.fft(x), O.fft(x, inverse=True)
+
+
+@pytest.mark.parametrize("n", CASES)
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_large_c2c(dev, n, prec):
+    D, torch = dev
+    x, want_f, want_i = _case(n)
+    dt, nd = (torch.complex128, np.complex128) if prec == "c128" else (torch.complex64, np.complex64)
+    p = D.C2CPlan(n, 1, dt)
+    assert p.num_launches() >= 2, p.describe()
+    bar = 1e-14 * math.log2(n) if prec == "c128" else 1e-6
+    xt = torch.from_numpy(x.astype(nd)).to("cuda")
+    for inverse, want in ((False, want_f), (True, want_i)):
+        got = p.exec(xt, inverse=inverse).cpu().numpy().astype(np.complex128)
+        e = rel_l2(got, want)
+        assert e <= bar, (n, prec, inverse, e, p.describe())
+
+
+def test_round_trip_2_25_fp32(dev):
+    D, torch = dev
+    n = 1 << 25
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    x = torch.randn(n, dtype=torch.complex64, device="cuda", generator=g)
+    p = D.C2CPlan(n, 1, torch.complex64)
+    assert p.num_launches() >= 3, p.describe()
+    X = p.exec(x)
+    back = p.exec(X, inverse=True)
+    x64 = x.to(torch.complex128)
+    e = ((back.to(torch.complex128) - x64).norm() / x64.norm()).item()
+    assert e <= 1e-6, e
+    # Parseval: sum |X|^2 = n sum |x|^2
+    ex = (x64.abs() ** 2).sum().item()
+    eX = (X.to(torch.complex128).abs() ** 2).sum().item()
+    assert abs(eX / (n * ex) - 1.0) <= 1e-6, (eX, n * ex)
