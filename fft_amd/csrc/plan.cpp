@@ -2683,12 +2683,15 @@ int64_t gofft_plan_len(const gofft_plan* p) { return p ? p->n : -1; }
 int64_t gofft_plan_frames(const gofft_plan* p) { return p ? p->frames : -1; }
 int64_t gofft_plan_workspace_bytes(const gofft_plan* p) { return p ? (int64_t)p->ws_need : -1; }
 
+// Bins per line of a real plan: n/2 + 1, none for an empty line.
+static i64 real_bins(const gofft_plan* p) { return p->n > 0 ? p->n / 2 + 1 : 0; }
+
 int64_t gofft_plan_in_elems(const gofft_plan* p, int direction) {
   if (!p) return -1;
   switch (p->kind) {
     case K_C2C: return p->n * p->batch;
     case K_MANY: return p->total_elems;
-    case K_R2C: return direction == 0 ? p->n * p->batch : (p->n / 2 + 1) * p->batch;
+    case K_R2C: return direction == 0 ? p->n * p->batch : real_bins(p) * p->batch;
     case K_PSD: return p->n * p->batch;
     case K_ND_C2C: return p->nd_total * p->batch;
     case K_ND_R2C: return direction == 0 ? p->rows * p->cols * p->batch : p->rows * p->rcols * p->batch;
@@ -2702,8 +2705,8 @@ int64_t gofft_plan_out_elems(const gofft_plan* p, int direction) {
   switch (p->kind) {
     case K_C2C: return p->n * p->batch;
     case K_MANY: return p->total_elems;
-    case K_R2C: return direction == 0 ? (p->n / 2 + 1) * p->batch : p->n * p->batch;
-    case K_PSD: return (p->n / 2 + 1) * p->batch;
+    case K_R2C: return direction == 0 ? real_bins(p) * p->batch : p->n * p->batch;
+    case K_PSD: return real_bins(p) * p->batch;
     case K_ND_C2C: return p->nd_total * p->batch;
     case K_ND_R2C: return direction == 0 ? p->rows * p->rcols * p->batch : p->rows * p->cols * p->batch;
     case K_SPEC: return p->frames * (p->seg / 2 + 1);
@@ -2826,10 +2829,27 @@ int gofft_plan_kernel_work(const gofft_plan* p, int direction, int kernel, int64
   return GOFFT_OK;
 }
 
-int gofft_exec_c2c(const gofft_plan* p, const void* src, void* dst, int direction, void* stream) {
-  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+// A call that reads or writes elements needs that data pointer: a null one
+// would reach a kernel (or the host copy of a host-staged call). `layout` is
+// the direction whose element counts the call's input and output have; `in`
+// may be null when the call reads nothing from it (an inverse given zero
+// bins).
+static int check_data(const gofft_plan* p, int layout, const void* in, const void* out, bool reads_in = true) {
+  if ((reads_in && !in && gofft_plan_in_elems(p, layout) > 0) || (!out && gofft_plan_out_elems(p, layout) > 0))
+    return fail(GOFFT_ERR_INVALID_ARG, "null data pointer");
+  return GOFFT_OK;
+}
+
+static int check_direction(int direction) {
   if (direction != GOFFT_FORWARD && direction != GOFFT_INVERSE)
     return fail(GOFFT_ERR_INVALID_ARG, "direction must be GOFFT_FORWARD or GOFFT_INVERSE");
+  return GOFFT_OK;
+}
+
+int gofft_exec_c2c(const gofft_plan* p, const void* src, void* dst, int direction, void* stream) {
+  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  RC_TRY(check_direction(direction));
+  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
   hipStream_t s = (hipStream_t)stream;
   char* ws = nullptr;
   RC_TRY(p->ws(s, &ws));
@@ -2851,6 +2871,7 @@ int gofft_exec_c2c(const gofft_plan* p, const void* src, void* dst, int directio
 
 int gofft_exec_r2c(const gofft_plan* p, const void* src, void* dst, void* stream) {
   if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
   hipStream_t s = (hipStream_t)stream;
   char* ws = nullptr;
   RC_TRY(p->ws(s, &ws));
@@ -2861,6 +2882,7 @@ int gofft_exec_r2c(const gofft_plan* p, const void* src, void* dst, void* stream
 int gofft_exec_c2r(const gofft_plan* p, const void* src, int64_t bins, void* dst, void* stream) {
   if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
   if (bins < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative bin count");
+  RC_TRY(check_data(p, GOFFT_INVERSE, src, dst, bins > 0));
   hipStream_t s = (hipStream_t)stream;
   char* ws = nullptr;
   RC_TRY(p->ws(s, &ws));
@@ -2871,6 +2893,7 @@ int gofft_exec_c2r(const gofft_plan* p, const void* src, int64_t bins, void* dst
 
 int gofft_exec_psd(const gofft_plan* p, const void* src, void* dst, void* stream) {
   if (!p || p->kind != K_PSD) return fail(GOFFT_ERR_INVALID_ARG, "plan is not a psd plan");
+  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
   hipStream_t s = (hipStream_t)stream;
   char* ws = nullptr;
   RC_TRY(p->ws(s, &ws));
@@ -2880,6 +2903,7 @@ int gofft_exec_psd(const gofft_plan* p, const void* src, void* dst, void* stream
 
 int gofft_exec_spectrogram(const gofft_plan* p, const void* src, void* dst, void* stream) {
   if (!p || p->kind != K_SPEC) return fail(GOFFT_ERR_INVALID_ARG, "plan is not a spectrogram plan");
+  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
   hipStream_t s = (hipStream_t)stream;
   char* ws = nullptr;
   RC_TRY(p->ws(s, &ws));
@@ -2897,6 +2921,9 @@ int gofft_exec_spectrogram(const gofft_plan* p, const void* src, void* dst, void
 int gofft_exec_nd_r2c(const gofft_plan* p, const void* src, int64_t bins, void* dst, int direction,
                       void* stream) {
   if (!p || p->kind != K_ND_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an nd r2c plan");
+  RC_TRY(check_direction(direction));
+  if (direction == GOFFT_INVERSE && bins < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative bin count");
+  RC_TRY(check_data(p, direction, src, dst, direction == GOFFT_FORWARD || bins > 0));
   hipStream_t s = (hipStream_t)stream;
   char* ws = nullptr;
   RC_TRY(p->ws(s, &ws));
@@ -2925,6 +2952,8 @@ int gofft_exec_nd_r2c(const gofft_plan* p, const void* src, int64_t bins, void* 
 
 int gofft_vjp_c2c(const gofft_plan* p, const void* gy, void* gx, int direction, void* stream) {
   if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  RC_TRY(check_direction(direction));
+  RC_TRY(check_data(p, GOFFT_INVERSE, gy, gx));
   // y = F x  =>  gx = F^H gy = conj(F conj(gy)) = n * IFFT(gy).
   // y = IFFT x = F^H x / n  =>  gx = F gy / n.
   if (p->kind == K_C2C || p->kind == K_MANY || p->kind == K_ND_C2C) {
@@ -2954,6 +2983,7 @@ int gofft_vjp_c2c(const gofft_plan* p, const void* gy, void* gx, int direction, 
 // IRFFT counts every interior bin twice through its Hermitian mirror).
 int gofft_vjp_r2c(const gofft_plan* p, const void* gy, void* gx, void* stream) {
   if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  RC_TRY(check_data(p, GOFFT_INVERSE, gy, gx));
   hipStream_t s = (hipStream_t)stream;
   char* ws = nullptr;
   RC_TRY(p->ws(s, &ws));
@@ -2966,6 +2996,7 @@ int gofft_vjp_r2c(const gofft_plan* p, const void* gy, void* gx, void* stream) {
 // gX = RFFT(gx) / n with those bins doubled (Im of DC and Nyquist is 0).
 int gofft_vjp_c2r(const gofft_plan* p, const void* gx, void* gX, void* stream) {
   if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  RC_TRY(check_data(p, GOFFT_FORWARD, gx, gX));
   hipStream_t s = (hipStream_t)stream;
   char* ws = nullptr;
   RC_TRY(p->ws(s, &ws));
@@ -2980,8 +3011,8 @@ int gofft_vjp_c2r(const gofft_plan* p, const void* gx, void* gX, void* stream) {
 // weights w'.
 int gofft_vjp_nd_r2c(const gofft_plan* p, const void* g_out, void* g_in, int direction, void* stream) {
   if (!p || p->kind != K_ND_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an nd r2c plan");
-  if (direction != GOFFT_FORWARD && direction != GOFFT_INVERSE)
-    return fail(GOFFT_ERR_INVALID_ARG, "direction must be GOFFT_FORWARD or GOFFT_INVERSE");
+  RC_TRY(check_direction(direction));
+  RC_TRY(check_data(p, direction == GOFFT_FORWARD ? GOFFT_INVERSE : GOFFT_FORWARD, g_out, g_in));
   hipStream_t s = (hipStream_t)stream;
   char* ws = nullptr;
   RC_TRY(p->ws(s, &ws));
@@ -3016,6 +3047,8 @@ static int host_roundtrip(size_t in_bytes, size_t out_bytes, const void* src, vo
 
 int gofft_exec_c2c_host(const gofft_plan* p, const void* src, void* dst, int direction) {
   if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  RC_TRY(check_direction(direction));
+  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
   size_t b = (size_t)gofft_plan_in_elems(p, direction) * elt(p->prec);
   return host_roundtrip(b, b, src, dst, [&](const void* i, void* o) {
     return gofft_exec_c2c(p, i, o, direction, nullptr);
@@ -3024,6 +3057,7 @@ int gofft_exec_c2c_host(const gofft_plan* p, const void* src, void* dst, int dir
 
 int gofft_exec_r2c_host(const gofft_plan* p, const void* src, void* dst) {
   if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
   size_t ib = (size_t)p->n * p->batch * relt(p->prec);
   size_t ob = (size_t)(p->n / 2 + 1) * p->batch * elt(p->prec);
   if (p->n == 0) ob = 0;
@@ -3032,6 +3066,8 @@ int gofft_exec_r2c_host(const gofft_plan* p, const void* src, void* dst) {
 
 int gofft_exec_c2r_host(const gofft_plan* p, const void* src, int64_t bins, void* dst) {
   if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  if (bins < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative bin count");
+  RC_TRY(check_data(p, GOFFT_INVERSE, src, dst, bins > 0));
   size_t ib = (size_t)bins * p->batch * elt(p->prec);
   size_t ob = (size_t)p->n * p->batch * relt(p->prec);
   return host_roundtrip(ib, ob, src, dst,
@@ -3040,6 +3076,8 @@ int gofft_exec_c2r_host(const gofft_plan* p, const void* src, int64_t bins, void
 
 int gofft_vjp_c2c_host(const gofft_plan* p, const void* gy, void* gx, int direction) {
   if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  RC_TRY(check_direction(direction));
+  RC_TRY(check_data(p, GOFFT_INVERSE, gy, gx));
   size_t b = (size_t)gofft_plan_in_elems(p, direction) * elt(p->prec);
   return host_roundtrip(b, b, gy, gx, [&](const void* i, void* o) {
     return gofft_vjp_c2c(p, i, o, direction, nullptr);
@@ -3048,6 +3086,7 @@ int gofft_vjp_c2c_host(const gofft_plan* p, const void* gy, void* gx, int direct
 
 int gofft_vjp_r2c_host(const gofft_plan* p, const void* gy, void* gx) {
   if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  RC_TRY(check_data(p, GOFFT_INVERSE, gy, gx));
   size_t ib = p->n > 0 ? (size_t)(p->n / 2 + 1) * p->batch * elt(p->prec) : 0;
   size_t ob = (size_t)p->n * p->batch * relt(p->prec);
   return host_roundtrip(ib, ob, gy, gx, [&](const void* i, void* o) { return gofft_vjp_r2c(p, i, o, nullptr); });
@@ -3055,6 +3094,7 @@ int gofft_vjp_r2c_host(const gofft_plan* p, const void* gy, void* gx) {
 
 int gofft_vjp_c2r_host(const gofft_plan* p, const void* gx, void* gX) {
   if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+  RC_TRY(check_data(p, GOFFT_FORWARD, gx, gX));
   size_t ib = (size_t)p->n * p->batch * relt(p->prec);
   size_t ob = p->n > 0 ? (size_t)(p->n / 2 + 1) * p->batch * elt(p->prec) : 0;
   return host_roundtrip(ib, ob, gx, gX, [&](const void* i, void* o) { return gofft_vjp_c2r(p, i, o, nullptr); });

@@ -19,6 +19,10 @@
  *    modified unless it aliases dst.
  *  - Every call returns GOFFT_OK (0) or a negative status; gofft_last_error()
  *    returns a thread-local message for the last failure on this thread.
+ *  - Transform and VJP calls on a plan with data reject a NULL data pointer,
+ *    and a direction other than GOFFT_FORWARD / GOFFT_INVERSE, with
+ *    GOFFT_ERR_INVALID_ARG before any work is enqueued (an inverse given
+ *    zero bins may pass NULL for its input).
  *  - Plans are immutable after creation and safe to execute concurrently from
  *    several threads provided each concurrent call uses its own stream; the
  *    plan keeps one workspace per stream (plan.go:12-14, realplan.go:10-11).
