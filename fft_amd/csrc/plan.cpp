@@ -770,6 +770,9 @@ class Engine {
   virtual void work(Work& w, i64 lines) const = 0;
   // One line of text naming the engine and its launches.
   virtual std::string describe() const = 0;
+  // Whether a workgroup of this engine takes more than half a CU's LDS, so
+  // that it can start only on a CU the other launches have left.
+  virtual bool whole_cu() const { return false; }
   // out = scale * conj?( FFT( conj?(in) ) ) per line, for the first `nl`
   // lines (all `batch` lines when nl < 0).
   virtual int run(const void* in, LineIO li, void* out, LineIO lo, int conj_in, int conj_out,
@@ -1552,6 +1555,7 @@ class RaderEngine : public Engine {
   DevBuf perm, iperm, spec;
   // fused path
   bool fused = false;
+  bool big_lds = false;  // its workgroup takes more than half a CU's LDS
   SubPlan sp;
   DevBuf stw;
   // multi-kernel path
@@ -1589,6 +1593,7 @@ class RaderEngine : public Engine {
     RC_TRY(current_device(&dev));
     if (fused_conv_plan(cl, prec, dev.lds_full / (i64)elt(prec), sp, spread_cols(batch, dev.cus))) {
       fused = true;
+      big_lds = 2 * (i64)sp.f.C * sp.f.Rp * (i64)elt(prec) > dev.lds_full;
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
       cb = chunk_lines(2 * cl * (i64)elt(prec), batch);
@@ -1600,6 +1605,7 @@ class RaderEngine : public Engine {
   i64 cb = 1;  // lines per chunk (multi-kernel path)
 
   size_t ws_bytes() const override { return fused ? 0 : chain.ws_bytes(); }
+  bool whole_cu() const override { return fused && big_lds; }
   int launches() const override {
     return fused ? 1 : (int)(chain.launches() * ((batch + cb - 1) / cb));
   }
@@ -1669,6 +1675,7 @@ class BluesteinEngine : public Engine {
   bool faithful = false;
   DevBuf chirp, spec, chirpI, specI;
   bool fused = false;
+  bool big_lds = false;  // its workgroup takes more than half a CU's LDS
   SubPlan sp;
   DevBuf stw;
   ConvChain chain;
@@ -1738,6 +1745,7 @@ class BluesteinEngine : public Engine {
     }
     if (fused_conv_plan(m, prec, pts, sp, spread_cols(batch, dev.cus))) {
       fused = true;
+      big_lds = 2 * (i64)sp.f.C * sp.f.Rp * (i64)elt(prec) > dev.lds_full;
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
     } else {
       cb = chunk_lines(2 * m * (i64)elt(prec), batch);
@@ -1749,6 +1757,7 @@ class BluesteinEngine : public Engine {
   i64 cb = 1;  // lines per chunk (multi-kernel path)
 
   size_t ws_bytes() const override { return fused ? 0 : chain.ws_bytes(); }
+  bool whole_cu() const override { return fused && big_lds; }
   int launches() const override {
     return fused ? 1 : (int)(chain.launches() * ((batch + cb - 1) / cb));
   }
@@ -2300,6 +2309,7 @@ struct gofft_plan {
   hipEvent_t fork_ev = nullptr;
   std::vector<hipEvent_t> join_ev;
   std::vector<int> lane_of;
+  std::vector<size_t> order;  // enqueue order of the groups
   mutable std::mutex lane_mu;  // one fork/join enqueue at a time
 
   int ws(hipStream_t s, char** out) const {
@@ -2328,6 +2338,14 @@ int assign_lanes(gofft_plan& p) {
   if (const char* e = std::getenv("GOFFT_LANES")) want = std::max(1, std::min(16, std::atoi(e)));
   const int n = (int)std::min<size_t>((size_t)want, p.groups.size());
   p.lane_of.assign(p.groups.size(), 0);
+  // Enqueue order: groups whose workgroups need more than half a CU's LDS
+  // first, while every CU is still empty; the others after, largest first.
+  // Measured on the 4096-transform mixed-length batch (Rader 10007's fused
+  // convolution first): FP32 0.785 -> 0.773 ms.
+  p.order.clear();
+  for (int pass = 0; pass < 2; pass++)
+    for (size_t g = 0; g < p.groups.size(); g++)
+      if (p.groups[g]->eng->whole_cu() == (pass == 0)) p.order.push_back(g);
   if (n <= 1) return GOFFT_OK;
   std::vector<double> load(n, 0.0);
   for (size_t g = 0; g < p.groups.size(); g++) {
@@ -2359,13 +2377,13 @@ int run_groups(const gofft_plan* p, const void* src, void* dst, int ci, int co, 
     return g.eng->run(src, LineIO{o, 0}, dst, LineIO{o, 0}, ci, co, inverse_scale ? 1.0 / (double)g.len : 1.0, c);
   };
   if (p->lanes.empty()) {
-    for (const auto& g : p->groups) RC_TRY(one(*g, s));
+    for (size_t g : p->order) RC_TRY(one(*p->groups[g], s));
     return GOFFT_OK;
   }
   std::lock_guard<std::mutex> lk(p->lane_mu);
   HIP_TRY(hipEventRecord(p->fork_ev, s));
   for (hipStream_t l : p->lanes) HIP_TRY(hipStreamWaitEvent(l, p->fork_ev, 0));
-  for (size_t g = 0; g < p->groups.size(); g++) {
+  for (size_t g : p->order) {
     const int k = p->lane_of[g];
     RC_TRY(one(*p->groups[g], k == 0 ? s : p->lanes[k - 1]));
   }
