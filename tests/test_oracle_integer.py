@@ -68,7 +68,7 @@ def test_conv_length_selection(oracle):
         _, _, cl, cyc = oracle.rader_info(n)
         assert not cyc and cl >= 2 * (n - 1)
         assert oracle.conv_cost(cl)[1]
-    # computed from the code (SURVEY: the plan.go comments are stale)
+    # the lengths plan.go's code computes (its comments quote other values)
     assert oracle.rader_info(10007)[2] == 20580
     assert oracle.rader_info(9973)[2] == 20580
     assert oracle.rader_info(5003)[2] == 10290
