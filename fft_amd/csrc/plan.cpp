@@ -1045,7 +1045,18 @@ class StockhamEngine : public Engine {
     DevInfo dev;
     RC_TRY(current_device(&dev));
     const i64 pts = dev.lds_bytes / (i64)elt(prec);
-    const int cmp = force_cols > 0 ? force_cols : prec == 0 ? 8 : 16;
+    int cmp = force_cols > 0 ? force_cols : prec == 0 ? 8 : 16;
+    // complex128 runs 16 columns (256-byte runs) per pass when 16 columns of
+    // every pass fit 48 KB of LDS (three workgroups per CU). Measured: 16384
+    // x 1024 0.278 -> 0.267 ms; Rader chains 5003 x 1024 0.346 -> 0.314 ms,
+    // 10007 x 585 0.343 -> 0.332 ms (passes of 128 and 160). Passes of 256
+    // to 512 points keep 8: 262144 x 64 was 10% and 40009 x 128 24% slower
+    // with 16.
+    if (prec == GOFFT_C128 && force_cols == 0) {
+      bool fits = true;
+      for (i64 R : lens) fits = fits && pitch(R) * 16 * (i64)elt(prec) <= 48 * 1024;
+      if (fits) cmp = 16;
+    }
     passes.clear();
     i64 Ns = 1;
     for (i64 R : lens) {
@@ -1360,17 +1371,6 @@ struct ConvChain {
     } else {
       std::vector<i64> lens;
       RC_TRY(fwd.multi_lengths(lens));
-      // complex128 chains run 16 columns (256-byte runs) per pass when 16
-      // columns of every pass fit 48 KB of LDS (three workgroups per CU).
-      // Measured: Rader 5003 x 1024 0.346 -> 0.314 ms, 10007 x 585 0.343 ->
-      // 0.332 ms, 9973 x 585 0.342 -> 0.331 ms (passes of 128 and 160);
-      // 40009 x 128 (passes of 320 and 256) was 24% slower with them.
-      if (prec == GOFFT_C128 && cols == 0) {
-        bool fits = true;
-        for (i64 R : lens) fits = fits && pitch(R) * 16 * (i64)elt(prec) <= 48 * 1024;
-        if (fits)
-          for (StockhamEngine* e : {&fwd, &inv}) e->force_cols = 16;
-      }
       // The middle launch runs the forward's last pass length twice (forward
       // and inverse): give it the length with the fewest LDS stages.
       auto stages = [](i64 R) { return radix_schedule((int)R).size(); };
