@@ -28,6 +28,10 @@ FLAG_AUTOTUNE = 4
 ND_C2C = 0
 ND_R2C = 1
 KERNEL_PASS, KERNEL_CONV, KERNEL_REAL, KERNEL_AUX = 0, 1, 2, 3
+(TABLE_RADER_GENERATOR, TABLE_RADER_PERM, TABLE_RADER_IPERM, TABLE_RADER_CONV, TABLE_CHIRP_INDEX,
+ TABLE_BLUESTEIN_CONV) = range(6)
+(OP_C2C, OP_R2C, OP_C2R, OP_ND_R2C, OP_PSD, OP_SPECTROGRAM, OP_VJP_C2C, OP_VJP_R2C, OP_VJP_C2R,
+ OP_VJP_ND_R2C) = range(10)
 
 _i64 = ctypes.c_int64
 _vp = ctypes.c_void_p
@@ -55,6 +59,7 @@ SIGNATURES = {
     "gofft_plan_workspace_bytes": (_i64, [_vp]),
     "gofft_plan_num_launches": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gofft_plan_describe": (_i64, [_vp, ctypes.c_char_p, _i64]),
+    "gofft_plan_index_table": (_i64, [_vp, _i64, ctypes.c_int, ctypes.POINTER(_i64), _i64]),
     "gofft_plan_kernel_work": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_i64),
                                               ctypes.POINTER(_i64)]),
     "gofft_exec_c2c": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
@@ -73,6 +78,11 @@ SIGNATURES = {
     "gofft_vjp_c2c_host": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "gofft_vjp_r2c_host": (ctypes.c_int, [_vp, _vp, _vp]),
     "gofft_vjp_c2r_host": (ctypes.c_int, [_vp, _vp, _vp]),
+    "gofft_exec_nd_r2c_host": (ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.c_int]),
+    "gofft_exec_psd_host": (ctypes.c_int, [_vp, _vp, _vp]),
+    "gofft_exec_spectrogram_host": (ctypes.c_int, [_vp, _vp, _vp]),
+    "gofft_vjp_nd_r2c_host": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "gofft_graph_capture": (ctypes.c_int, [_pp, _vp, ctypes.c_int, _vp, _i64, _vp, ctypes.c_int, _vp]),
     "gofft_graph_c2c": (ctypes.c_int, [_pp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "gofft_graph_launch": (ctypes.c_int, [_vp, _vp]),
     "gofft_graph_destroy": (ctypes.c_int, [_vp]),

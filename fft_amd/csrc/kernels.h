@@ -107,7 +107,8 @@ struct PassArgs {
   SubFFT f;
   // --- convolution hooks (Rader / Bluestein as three fused passes) -------
   // in_hook 1: element i of a line loads x[i] * w[i] (i < nvalid, else 0)
-  // in_hook 2: element i loads x[perm[i]]          (i < nvalid, else 0)
+  // in_hook 2: element i loads x[perm[i]]          (i < nvalid, else 0); x[0]
+  //            goes to x0buf[line]
   // in_hook 3: element i loads x[i]                 (i < nvalid, else 0)
   // out_hook 1: output p = conj(v) * w[p] for p < nvalid (others dropped)
   // out_hook 2: output p -> x0 + conj(v) at iperm[p - conv_off] when
@@ -116,10 +117,11 @@ struct PassArgs {
   long long nvalid, conv_off;
   const void* hook_w;
   const int* hook_perm;
-  const void* rader_x;             // out_hook 2: the transform input (for x0)
+  // Rader: per-line x0 = x[0] of the caller's line, saved by the first launch
+  // (in_hook 2: the column holding line point 0 stores it) and read by the
+  // last (out_hook 2), so that an output that aliases the input is safe.
+  void* x0buf;
   int x0_conj;                     // out_hook 2: x0 enters conjugated (inverse)
-  const long long* rader_x_off;
-  long long rader_x_stride;
   void* dc;                        // Rader: per-line A[0], written by the mid pass
   // mid pass: after the first sub-FFT, multiply by mid_spec at the forward
   // output position (mm map), conjugate, run the second sub-FFT f2.

@@ -915,9 +915,12 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
       s_tk[tid] = k;
       s_ln[tid] = d3;
       if (mid_spec) s_mk[tid] = col_digits(a.mm, g, nullptr, 0, nullptr, &lb, nullptr);
+      // Rader: the column holding point 0 of its line (in-line offset 0)
+      // saves x[0] for the last launch, which may write over the input.
+      if (in_hook == 2 && s_il[tid] == 0)
+        reinterpret_cast<T2*>(a.x0buf)[d3] = reinterpret_cast<const T2*>(a.in)[s_ib[tid]];
       if (out_hook == 2) {
-        const T2* xr = reinterpret_cast<const T2*>(a.rader_x);
-        T2 x0 = xr[a.rader_x_off ? a.rader_x_off[d3] : d3 * a.rader_x_stride];
+        T2 x0 = reinterpret_cast<const T2*>(a.x0buf)[d3];
         if (a.x0_conj) x0.y = -x0.y;
         s_x0[tid] = x0;
       }
@@ -1345,11 +1348,19 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
   const int c = tid >> f.log2tpc;
   const int lt = tid & (f.tpc - 1);
   const int tpc = f.tpc;
+  // Rader's x0 = x[0] of this thread's line, read with the other input loads:
+  // every read of the input ends before the first barrier and every store of
+  // the output starts after the last, so an output aliasing the input is safe.
+  T2 x0 = mkc<T2>(0, 0);
   {
     constexpr int LB = 16;
     const long long l = l0 + c;
     const bool live = l < a.nlines;
     const T2* x = in + (live ? (a.in_off ? a.in_off[l] : l * a.in_stride) : 0);
+    if (live && a.kind != 2) {
+      x0 = x[0];
+      if (a.conj_in) x0.y = -x0.y;
+    }
     const int rounds = MAXP == LB ? 1 : (R + LB * tpc - 1) / (LB * tpc);
     for (int rd = 0; rd < rounds; ++rd) {
       const int r0 = lt + rd * LB * tpc;
@@ -1432,9 +1443,6 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
       }
     }
   } else {
-    const T2* x = in + (live ? (a.in_off ? a.in_off[l] : l * a.in_stride) : 0);
-    T2 x0 = live ? x[0] : mkc<T2>(0, 0);
-    if (a.conj_in) x0.y = -x0.y;
     const int off = a.kind == 1 ? q : 0;
     for (int k0 = lt; k0 < q && live; k0 += SB * tpc) {
       int ip[SB];

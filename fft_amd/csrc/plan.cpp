@@ -58,11 +58,17 @@ int fail(int code, const std::string& msg) {
 // ---------------------------------------------------------------------------
 // Device limits.
 
+// Library version (gofft_version) and the short tag that the autotuner's
+// cache file records with each entry.
+const char* const kVersionText = "gofft-hip 0.2 (gfx950)";
+const char* const kVersionTag = "0.2";
+
 struct DevInfo {
   int id = -1;
   int lds_bytes = 65536;  // dynamic LDS per workgroup beside k_pass's static tables
   int lds_full = 65536;   // dynamic LDS per workgroup of the kernels without static LDS
   int cus = 0;
+  std::string arch;       // e.g. "gfx950:sramecc+:xnack-"
 };
 std::mutex g_dev_mu;
 std::map<int, DevInfo> g_devs;
@@ -88,6 +94,12 @@ int current_device(DevInfo* out) {
       if (v > 0 && v < d.lds_full) d.lds_full = v;
     }
     d.cus = cus;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, id));
+    d.arch = prop.gcnArchName;
+    for (char& ch : d.arch)
+      if (ch == ' ') ch = '_';
+    if (d.arch.empty()) d.arch = "unknown";
     it = g_devs.emplace(id, d).first;
   }
   *out = it->second;
@@ -621,14 +633,13 @@ std::vector<dcx> recurrence_table(i64 n) {
 // Reference Bluestein set-up (bluestein.go:28-63) in float64 with separately
 // rounded products: chirps from Go's sin/cos and kernel spectra by
 // BitReverse + Radix2 with recurrence twiddles.
-void reference_bluestein_tables(i64 n, i64 m, std::vector<dcx>& wF, std::vector<dcx>& wI,
-                                std::vector<dcx>& bF, std::vector<dcx>& bI) {
+void reference_bluestein_tables(i64 n, i64 m, const std::vector<i64>& jj, std::vector<dcx>& wF,
+                                std::vector<dcx>& wI, std::vector<dcx>& bF, std::vector<dcx>& bI) {
   const double kPi = 3.14159265358979323846264338327950288;
   wF.resize(n);
   wI.resize(n);
   for (i64 j = 0; j < n; j++) {
-    i64 jj = (i64)((__int128)j * j % (2 * n));
-    double ang = kPi * (double)jj / (double)n;
+    double ang = kPi * (double)jj[j] / (double)n;
     double s1, c1, s2, c2;
     go_sincos(-ang, &s1, &c1);
     go_sincos(ang, &s2, &c2);
@@ -666,6 +677,14 @@ void reference_bluestein_tables(i64 n, i64 m, std::vector<dcx>& wF, std::vector<
   spectrum(wI, bI);
 }
 #pragma clang fp contract(on)
+
+// Chirp phase indices j^2 mod 2n, j < n (bluestein.go:35-41), exact in
+// 128-bit integers: the chirp is w_j = exp(-pi i (j^2 mod 2n) / n).
+std::vector<i64> chirp_index(i64 n) {
+  std::vector<i64> jj(n);
+  for (i64 j = 0; j < n; j++) jj[j] = (i64)((__int128)j * j % (2 * n));
+  return jj;
+}
 
 int upload_dcx(DevBuf& b, const std::vector<dcx>& v, int prec, double scale = 1.0) {
   std::vector<cld> t(v.size());
@@ -1387,7 +1406,9 @@ struct ConvChain {
     return GOFFT_OK;
   }
 
-  size_t ws_bytes() const { return 2 * (size_t)m * cb * elt(prec) + (size_t)cb * elt(prec); }
+  // two ping-pong buffers of cb lines, then per line the Rader DC term A[0]
+  // and the saved x0
+  size_t ws_bytes() const { return 2 * (size_t)m * cb * elt(prec) + 2 * (size_t)cb * elt(prec); }
   int launches() const { return 2 * fwd.P() - 1; }
   std::string describe() const {
     return "chain m=" + std::to_string(m) + " chunk=" + std::to_string(cb) + " fwd=" + fwd.passes_text() +
@@ -1431,6 +1452,7 @@ struct ConvChain {
           char* ws, hipStream_t s, i64 nl) const {
     char* bufs[2] = {ws, ws + (size_t)m * cb * elt(prec)};
     void* dc = ws + 2 * (size_t)m * cb * elt(prec);
+    void* x0buf = (char*)dc + (size_t)cb * elt(prec);
     const int P = fwd.P();
     const int K = 2 * P - 1;
     for (int k = 0; k < K; k++) {
@@ -1476,6 +1498,7 @@ struct ConvChain {
         a.hook_w = h.w;
         a.hook_perm = h.perm;
         a.conj_in = ci;
+        if (h.in_hook == 2) a.x0buf = x0buf;
       }
       if (k == K - 1) {
         a.om.a3 = lo.stride;
@@ -1488,9 +1511,7 @@ struct ConvChain {
         a.conj_out = co;
         a.scale = scale;
         if (h.out_hook == 2) {
-          a.rader_x = in;
-          a.rader_x_off = li.off;
-          a.rader_x_stride = li.stride;
+          a.x0buf = x0buf;
           a.x0_conj = ci;
           a.dc = dc;
         }
@@ -1566,7 +1587,7 @@ std::vector<i64> conv_candidates(i64 lo, int prec, int k) {
 // scatter by the inverse permutation.
 class RaderEngine : public Engine {
  public:
-  i64 q = 0, cl = 0;
+  i64 q = 0, cl = 0, g = 0;  // g: the primitive root whose powers order the gather
   bool cyclic = true;
   DevBuf perm, iperm, spec;
   // fused path
@@ -1580,13 +1601,16 @@ class RaderEngine : public Engine {
   int init(unsigned flags) {
     (void)flags;
     q = n - 1;
-    const i64 g = primitive_root(n);
+    g = primitive_root(n);
     if (factors_are_small(q)) {
       cyclic = true;
       cl = q;
     } else {
       cyclic = false;
       cl = force_m > 0 ? force_m : choose_conv_len(2 * q, prec);
+      if (cl < 2 * q)
+        return fail(GOFFT_ERR_UNSUPPORTED, "no convolution length >= " + std::to_string(2 * q) + " for Rader n=" +
+                                               std::to_string(n));
     }
     std::vector<int> ph(q), iph(q);
     i64 x = 1;
@@ -1690,6 +1714,7 @@ class BluesteinEngine : public Engine {
   // spectra, so results round the way the reference's do.
   bool faithful = false;
   DevBuf chirp, spec, chirpI, specI;
+  std::vector<i64> jj;  // chirp phase indices j^2 mod 2n (the tables were built from these)
   bool fused = false;
   bool big_lds = false;  // its workgroup takes more than half a CU's LDS
   SubPlan sp;
@@ -1705,11 +1730,15 @@ class BluesteinEngine : public Engine {
       while (m < 2 * n - 1) m <<= 1;
     } else {
       m = force_m > 0 ? force_m : choose_conv_len(2 * n - 1, prec);
+      if (m < 2 * n - 1)
+        return fail(GOFFT_ERR_UNSUPPORTED, "no convolution length >= " + std::to_string(2 * n - 1) +
+                                               " for Bluestein n=" + std::to_string(n));
     }
     std::vector<cld> hF, hI;  // kernel spectra (1/m folded in), natural order
+    jj = chirp_index(n);
     if (faithful) {
       std::vector<dcx> wF, wI, bF, bI;
-      reference_bluestein_tables(n, m, wF, wI, bF, bI);
+      reference_bluestein_tables(n, m, jj, wF, wI, bF, bI);
       const double invm = 1.0 / (double)m;  // exact: m is a power of two
       RC_TRY(upload_dcx(chirp, wF, prec));
       RC_TRY(upload_dcx(chirpI, wI, prec));
@@ -1723,10 +1752,7 @@ class BluesteinEngine : public Engine {
       RC_TRY(upload_complex(specI, hI, prec));
     } else {
       std::vector<cld> w(n), b(m, cld(0, 0));
-      for (i64 j = 0; j < n; j++) {
-        i64 jj = (i64)((__int128)j * j % (2 * n));  // bit-exact chirp phase index
-        w[j] = root(jj, 2 * n);                      // exp(-pi i jj / n)
-      }
+      for (i64 j = 0; j < n; j++) w[j] = root(jj[j], 2 * n);  // exp(-pi i jj / n)
       b[0] = std::conj(w[0]);
       for (i64 j = 1; j < n; j++) {
         b[j] = std::conj(w[j]);
@@ -1893,9 +1919,33 @@ struct TuneChoice {
   i64 m = 0;
   int cols = 0;
 };
+// (device architecture, route, length, lines, precision)
+typedef std::tuple<std::string, int, i64, i64, int> TuneKey;
 std::mutex g_tune_mu;
-std::map<std::tuple<int, i64, i64, int>, TuneChoice> g_tune;
+std::map<TuneKey, TuneChoice> g_tune;
 bool g_tune_loaded = false;
+// Cache file lines carry this tag, the device architecture and the library
+// version; lines with another tag or version are ignored.
+const char* const kTuneTag = "gofft-tune-v2";
+
+// Columns per workgroup that the autotuner tries for multi-pass passes.
+std::vector<int> tune_cols(int prec) {
+  return prec == GOFFT_C64 ? std::vector<int>{16, 8, 32} : std::vector<int>{8, 4, 16};
+}
+
+// Whether a (cached) choice is one the autotuner could have made: columns
+// from the candidate set, and for a convolution a 7-smooth length in the
+// planner's window [lo, lo + ceil(0.4 lo)] (none for other routes).
+bool tune_choice_valid(const TuneChoice& c, bool conv, i64 lo, int prec) {
+  const std::vector<int> cols = tune_cols(prec);
+  if (std::find(cols.begin(), cols.end(), c.cols) == cols.end()) return false;
+  if (!conv) return c.m == 0;
+  if (c.m < lo || c.m > lo + (lo * 4 + 9) / 10) return false;
+  i64 x = c.m;
+  for (int p : {2, 3, 5, 7})
+    while (x % p == 0) x /= p;
+  return x == 1;
+}
 
 bool autotune_on(unsigned flags) {
   if (flags & GOFFT_FLAG_AUTOTUNE) return true;
@@ -1910,19 +1960,28 @@ void tune_load_locked() {
   if (!path) return;
   FILE* f = std::fopen(path, "r");
   if (!f) return;
-  long long r, n, b, pr, m, c;
-  while (std::fscanf(f, "%lld %lld %lld %lld %lld %lld", &r, &n, &b, &pr, &m, &c) == 6)
-    g_tune[std::make_tuple((int)r, (i64)n, (i64)b, (int)pr)] = TuneChoice{(i64)m, (int)c};
+  char line[512];
+  while (std::fgets(line, sizeof line, f)) {
+    char tag[64], arch[64], ver[64];
+    long long r, n, b, pr, m, c;
+    if (std::sscanf(line, "%63s %63s %63s %lld %lld %lld %lld %lld %lld", tag, ver, arch, &r, &n, &b, &pr, &m,
+                    &c) != 9)
+      continue;
+    if (std::strcmp(tag, kTuneTag) != 0 || std::strcmp(ver, kVersionTag) != 0) continue;
+    // entries are checked against the planner's constraints when used
+    g_tune[TuneKey(arch, (int)r, (i64)n, (i64)b, (int)pr)] = TuneChoice{(i64)m, (int)c};
+  }
   std::fclose(f);
 }
 
-void tune_store_locked(const std::tuple<int, i64, i64, int>& key, const TuneChoice& c) {
+void tune_store_locked(const TuneKey& key, const TuneChoice& c) {
   g_tune[key] = c;
   const char* path = std::getenv("GOFFT_TUNE_CACHE");
   if (!path) return;
   if (FILE* f = std::fopen(path, "a")) {
-    std::fprintf(f, "%d %lld %lld %d %lld %d\n", std::get<0>(key), (long long)std::get<1>(key),
-                 (long long)std::get<2>(key), std::get<3>(key), (long long)c.m, c.cols);
+    std::fprintf(f, "%s %s %s %d %lld %lld %d %lld %d\n", kTuneTag, kVersionTag, std::get<0>(key).c_str(),
+                 std::get<1>(key), (long long)std::get<2>(key), (long long)std::get<3>(key), std::get<4>(key),
+                 (long long)c.m, c.cols);
     std::fclose(f);
   }
 }
@@ -1970,27 +2029,39 @@ std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, 
   const bool conv = (r == 3 && !factors_are_small(n - 1)) || (r == 4 && !faithful_blue);
   if (!autotune_on(flags) || batch <= 0 || r == 0 || faithful_blue || (!conv && r != 1 && r != 2))
     return make_engine_fixed(n, batch, prec, flags, rc, 0, 0);
-  const auto key = std::make_tuple(r, n, batch, prec);
-  {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    tune_load_locked();
-    auto it = g_tune.find(key);
-    if (it != g_tune.end()) {
-      auto e = make_engine_fixed(n, batch, prec, flags, rc, it->second.m, it->second.cols);
-      if (e) e->tuned = true;
-      return e;
-    }
-  }
+  DevInfo dev;
+  if ((*rc = current_device(&dev)) != GOFFT_OK) return nullptr;
+  const TuneKey key(dev.arch, r, n, batch, prec);
   // Candidates: (conv length, columns); columns only matter for multi-pass
   // passes and chains, so a single-pass candidate is tried once.
   std::vector<TuneChoice> cand;
-  const std::vector<int> cols = prec == GOFFT_C64 ? std::vector<int>{16, 8, 32} : std::vector<int>{8, 4, 16};
+  const std::vector<int> cols = tune_cols(prec);
+  const i64 lo = r == 3 ? 2 * (n - 1) : 2 * n - 1;
   if (conv) {
-    const i64 lo = r == 3 ? 2 * (n - 1) : 2 * n - 1;
     for (i64 m : conv_candidates(lo, prec, 3))
       for (int c : cols) cand.push_back(TuneChoice{m, c});
   } else {
     for (int c : cols) cand.push_back(TuneChoice{0, c});
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    auto it = g_tune.find(key);
+    // A cached choice is used only if the planner could have made it: a
+    // convolution length of the cost model's window, columns from the
+    // candidate set. Anything else (a stale or edited file) is dropped.
+    if (it != g_tune.end() && !tune_choice_valid(it->second, conv, lo, prec)) {
+      g_tune.erase(it);
+      it = g_tune.end();
+    }
+    if (it != g_tune.end()) {
+      auto e = make_engine_fixed(n, batch, prec, flags, rc, it->second.m, it->second.cols);
+      if (e) {
+        e->tuned = true;
+        return e;
+      }
+      g_tune.erase(key);  // not plannable here: plan afresh below
+    }
   }
   // Time at most 512 MB of lines, or all of them.
   const i64 lines = std::max<i64>(1, std::min<i64>(batch, ((i64)512 << 20) / std::max<i64>(1, n * (i64)elt(prec))));
@@ -2281,6 +2352,78 @@ struct WorkspacePool {
   }
 };
 
+// Pinned host memory (hipHostMalloc), for the host-staged calls.
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() {}
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  int alloc(size_t b) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) return GOFFT_OK;
+    hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return fail(GOFFT_ERR_OOM, "hipHostMalloc of " + std::to_string(b) + " bytes failed");
+    }
+    bytes = b;
+    return GOFFT_OK;
+  }
+};
+
+// One host-staged call in flight: its own stream, device copies of the
+// call's input and output, and two pinned staging buffers through which
+// chunks of the caller's (pageable) memory are copied while the previous
+// chunk's DMA runs. Sessions are made on first use and reused; a plan keeps
+// as many as calls have overlapped, so concurrent host calls on one plan run
+// on separate streams and wait for their own stream only.
+struct HostSession {
+  hipStream_t s = nullptr;
+  DevBuf din, dout;
+  PinnedBuf stage[2];
+  hipEvent_t done[2] = {nullptr, nullptr};  // the DMA of each staging buffer's last chunk
+  ~HostSession() {
+    if (s) (void)hipStreamSynchronize(s);
+    for (hipEvent_t e : done)
+      if (e) (void)hipEventDestroy(e);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+struct HostPool {
+  std::mutex mu;
+  std::vector<std::unique_ptr<HostSession>> all;
+  std::vector<HostSession*> idle;
+  int acquire(HostSession** out) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (!idle.empty()) {
+        *out = idle.back();
+        idle.pop_back();
+        return GOFFT_OK;
+      }
+    }
+    auto h = std::make_unique<HostSession>();
+    HIP_TRY(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    for (hipEvent_t& e : h->done) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    std::lock_guard<std::mutex> lk(mu);
+    *out = h.get();
+    all.push_back(std::move(h));
+    return GOFFT_OK;
+  }
+  void release(HostSession* h) {
+    std::lock_guard<std::mutex> lk(mu);
+    idle.push_back(h);
+  }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2318,6 +2461,7 @@ struct gofft_plan {
   bool spec_gather = false;
   size_t ws_need = 0;
   mutable WorkspacePool pool;
+  mutable HostPool host;  // sessions of the host-staged calls
   // Mixed-length plans: streams (default priority) on which length groups run
   // beside the caller's stream, joined back by events. lane_of[g] is group
   // g's lane, 0 being the caller's stream.
@@ -2337,6 +2481,7 @@ struct gofft_plan {
   }
 
   ~gofft_plan() {
+    host.all.clear();  // drains each session's stream
     for (hipStream_t l : lanes) (void)hipStreamDestroy(l);
     for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
@@ -2381,19 +2526,39 @@ int assign_lanes(gofft_plan& p) {
   return GOFFT_OK;
 }
 
+// Workspace of one exec: the plan's per-stream pool (a null WsSet), or fixed
+// buffers of at least the plan's workspace bytes each: `main` for the
+// caller's stream and lanes[i] for a mixed-length plan's side lane i (a
+// graph's own workspace, or the no-allocation entry point's scratch).
+struct WsSet {
+  char* main = nullptr;
+  std::vector<char*> lanes;
+};
+
+int exec_ws(const gofft_plan* p, hipStream_t s, const WsSet* w, char** out) {
+  if (w) {
+    *out = w->main;
+    return GOFFT_OK;
+  }
+  return p->ws(s, out);
+}
+
 // Runs every group of a mixed-length plan: on the caller's stream alone, or
 // forked over the plan's lanes and joined back.
 int run_groups(const gofft_plan* p, const void* src, void* dst, int ci, int co, bool inverse_scale,
-               hipStream_t s) {
-  auto one = [&](const gofft_plan::Group& g, hipStream_t st) -> int {
+               hipStream_t s, const WsSet* w) {
+  auto one = [&](const gofft_plan::Group& g, hipStream_t st, int lane) -> int {
     char* ws = nullptr;
-    RC_TRY(p->ws(st, &ws));
+    if (w)
+      ws = lane == 0 ? w->main : w->lanes[lane - 1];
+    else
+      RC_TRY(p->ws(st, &ws));
     Ctx c{st, ws};
     const i64* o = (const i64*)g.off.p;
     return g.eng->run(src, LineIO{o, 0}, dst, LineIO{o, 0}, ci, co, inverse_scale ? 1.0 / (double)g.len : 1.0, c);
   };
   if (p->lanes.empty()) {
-    for (size_t g : p->order) RC_TRY(one(*p->groups[g], s));
+    for (size_t g : p->order) RC_TRY(one(*p->groups[g], s, 0));
     return GOFFT_OK;
   }
   std::lock_guard<std::mutex> lk(p->lane_mu);
@@ -2401,7 +2566,7 @@ int run_groups(const gofft_plan* p, const void* src, void* dst, int ci, int co, 
   for (hipStream_t l : p->lanes) HIP_TRY(hipStreamWaitEvent(l, p->fork_ev, 0));
   for (size_t g : p->order) {
     const int k = p->lane_of[g];
-    RC_TRY(one(*p->groups[g], k == 0 ? s : p->lanes[k - 1]));
+    RC_TRY(one(*p->groups[g], k == 0 ? s : p->lanes[k - 1], k));
   }
   for (size_t i = 0; i < p->lanes.size(); i++) {
     HIP_TRY(hipEventRecord(p->join_ev[i], p->lanes[i]));
@@ -2531,7 +2696,7 @@ int gofft_init(void) {
 
 const char* gofft_last_error(void) { return t_err.c_str(); }
 
-const char* gofft_version(void) { return "gofft-hip 0.1 (gfx950)"; }
+const char* gofft_version(void) { return kVersionText; }
 
 int gofft_route(int64_t n) { return route(n); }
 
@@ -2804,6 +2969,77 @@ int64_t gofft_plan_describe(const gofft_plan* p, char* buf, int64_t len) {
   return (int64_t)s.size() + 1;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The plan's engine for transform length n (a length group, an axis, the
+// half-length transform of a real plan), or null.
+const Engine* find_engine(const gofft_plan* p, i64 n) {
+  auto pick = [&](const Engine* e) { return e && e->n == n ? e : nullptr; };
+  const Engine* e = nullptr;
+  switch (p->kind) {
+    case K_C2C: e = pick(p->eng.get()); break;
+    case K_MANY:
+      for (const auto& g : p->groups)
+        if (!e) e = pick(g->eng.get());
+      break;
+    case K_R2C:
+    case K_PSD:
+    case K_SPEC:
+      if (p->real) e = pick(p->real->half.get());
+      break;
+    case K_ND_C2C:
+    case K_ND_R2C:
+      if (p->nd_row) e = pick(p->nd_row->half.get());
+      for (const auto& a : p->axes)
+        if (!e) e = pick(a.eng.get());
+      break;
+  }
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t gofft_plan_index_table(const gofft_plan* p, int64_t n, int table, int64_t* out, int64_t cap) {
+  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  const Engine* e = find_engine(p, n);
+  if (!e) return fail(GOFFT_ERR_INVALID_ARG, "the plan has no transform of length " + std::to_string(n));
+  std::vector<i64> v;
+  const RaderEngine* ra = dynamic_cast<const RaderEngine*>(e);
+  const BluesteinEngine* bl = dynamic_cast<const BluesteinEngine*>(e);
+  switch (table) {
+    case GOFFT_TABLE_RADER_GENERATOR:
+    case GOFFT_TABLE_RADER_CONV:
+      if (!ra) return fail(GOFFT_ERR_INVALID_ARG, "length " + std::to_string(n) + " is not routed to Rader");
+      if (table == GOFFT_TABLE_RADER_GENERATOR) v = {ra->g};
+      else v = {ra->cl, ra->cyclic ? 1 : 0};
+      break;
+    case GOFFT_TABLE_RADER_PERM:
+    case GOFFT_TABLE_RADER_IPERM: {
+      if (!ra) return fail(GOFFT_ERR_INVALID_ARG, "length " + std::to_string(n) + " is not routed to Rader");
+      // read back the device tables the kernels index with
+      const DevBuf& b = table == GOFFT_TABLE_RADER_PERM ? ra->perm : ra->iperm;
+      std::vector<int> h((size_t)ra->q);
+      if (ra->q > 0) HIP_TRY(hipMemcpy(h.data(), b.p, h.size() * sizeof(int), hipMemcpyDeviceToHost));
+      v.assign(h.begin(), h.end());
+      break;
+    }
+    case GOFFT_TABLE_CHIRP_INDEX:
+    case GOFFT_TABLE_BLUESTEIN_CONV:
+      if (!bl) return fail(GOFFT_ERR_INVALID_ARG, "length " + std::to_string(n) + " is not routed to Bluestein");
+      if (table == GOFFT_TABLE_CHIRP_INDEX) v = bl->jj;
+      else v = {bl->m};
+      break;
+    default:
+      return fail(GOFFT_ERR_INVALID_ARG, "unknown table");
+  }
+  if (out && cap > 0) std::memcpy(out, v.data(), sizeof(i64) * (size_t)std::min<i64>(cap, (i64)v.size()));
+  return (int64_t)v.size();
+}
+
 int gofft_plan_kernel_work(const gofft_plan* p, int direction, int kernel, int64_t* launches,
                            int64_t* bytes) {
   if (!p || !launches || !bytes) return fail(GOFFT_ERR_INVALID_ARG, "null argument");
@@ -2863,302 +3099,418 @@ int gofft_plan_kernel_work(const gofft_plan* p, int direction, int kernel, int64
   return GOFFT_OK;
 }
 
-// A call that reads or writes elements needs that data pointer: a null one
-// would reach a kernel (or the host copy of a host-staged call). `layout` is
-// the direction whose element counts the call's input and output have; `in`
-// may be null when the call reads nothing from it (an inverse given zero
-// bins).
-static int check_data(const gofft_plan* p, int layout, const void* in, const void* out, bool reads_in = true) {
-  if ((reads_in && !in && gofft_plan_in_elems(p, layout) > 0) || (!out && gofft_plan_out_elems(p, layout) > 0))
-    return fail(GOFFT_ERR_INVALID_ARG, "null data pointer");
+// ---------------------------------------------------------------------------
+// Execution. Every exec and VJP entry point, device or host, goes through
+// exec_op; graph capture records one exec_op.
+
+}  // extern "C"
+
+namespace {
+
+enum ExecOp {
+  OP_C2C = 0,
+  OP_R2C,
+  OP_C2R,
+  OP_ND_R2C,
+  OP_PSD,
+  OP_SPECTROGRAM,
+  OP_VJP_C2C,
+  OP_VJP_R2C,
+  OP_VJP_C2R,
+  OP_VJP_ND_R2C,
+  OP_COUNT
+};
+
+bool is_complex_plan(const gofft_plan* p) { return p->kind == K_C2C || p->kind == K_MANY || p->kind == K_ND_C2C; }
+
+// The plan kind an op needs, and the error text when the plan is another.
+int check_kind(const gofft_plan* p, int op) {
+  switch (op) {
+    case OP_C2C:
+    case OP_VJP_C2C:
+      if (!is_complex_plan(p)) return fail(GOFFT_ERR_INVALID_ARG, "plan is not a complex-to-complex plan");
+      return GOFFT_OK;
+    case OP_R2C:
+    case OP_C2R:
+    case OP_VJP_R2C:
+    case OP_VJP_C2R:
+      if (p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
+      return GOFFT_OK;
+    case OP_ND_R2C:
+    case OP_VJP_ND_R2C:
+      if (p->kind != K_ND_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an nd r2c plan");
+      return GOFFT_OK;
+    case OP_PSD:
+      if (p->kind != K_PSD) return fail(GOFFT_ERR_INVALID_ARG, "plan is not a psd plan");
+      return GOFFT_OK;
+    case OP_SPECTROGRAM:
+      if (p->kind != K_SPEC) return fail(GOFFT_ERR_INVALID_ARG, "plan is not a spectrogram plan");
+      return GOFFT_OK;
+  }
+  return fail(GOFFT_ERR_INVALID_ARG, "unknown operation");
+}
+
+// Whether the op's direction argument is meaningful (and so checked).
+bool op_has_direction(int op) { return op == OP_C2C || op == OP_VJP_C2C || op == OP_ND_R2C || op == OP_VJP_ND_R2C; }
+
+// Whether the op takes a bin count (the inverse real transforms).
+bool op_takes_bins(int op, int direction) {
+  return op == OP_C2R || (op == OP_ND_R2C && direction == GOFFT_INVERSE);
+}
+
+// Bytes of the op's input and output arrays.
+void io_bytes(const gofft_plan* p, int op, int direction, i64 bins, size_t* ib, size_t* ob) {
+  const size_t e = elt(p->prec), r = relt(p->prec);
+  const i64 nb = p->n > 0 ? p->n / 2 + 1 : 0;
+  const i64 B = p->batch;
+  size_t in = 0, out = 0;
+  switch (op) {
+    case OP_C2C:
+    case OP_VJP_C2C:
+      in = out = (size_t)gofft_plan_in_elems(p, GOFFT_FORWARD) * e;
+      break;
+    case OP_R2C:
+      in = (size_t)(p->n * B) * r;
+      out = (size_t)(nb * B) * e;
+      break;
+    case OP_C2R:
+      in = (size_t)(std::max<i64>(bins, 0) * B) * e;
+      out = (size_t)(p->n * B) * r;
+      break;
+    case OP_VJP_R2C:
+      in = (size_t)(nb * B) * e;
+      out = (size_t)(p->n * B) * r;
+      break;
+    case OP_VJP_C2R:
+      in = (size_t)(p->n * B) * r;
+      out = (size_t)(nb * B) * e;
+      break;
+    case OP_PSD:
+      in = (size_t)(p->n * B) * r;
+      out = (size_t)(nb * B) * r;
+      break;
+    case OP_SPECTROGRAM:
+      in = (size_t)p->sig_len * r;
+      out = (size_t)(p->frames * (p->seg / 2 + 1)) * r;
+      break;
+    case OP_ND_R2C:
+      if (direction == GOFFT_FORWARD) {
+        in = (size_t)(p->rows * p->cols * B) * r;
+        out = (size_t)(p->rows * p->rcols * B) * e;
+      } else {
+        in = (size_t)(std::max<i64>(bins, 0) * B) * e;
+        out = (size_t)(p->rows * p->cols * B) * r;
+      }
+      break;
+    case OP_VJP_ND_R2C:
+      if (direction == GOFFT_FORWARD) {
+        in = (size_t)(p->rows * p->rcols * B) * e;
+        out = (size_t)(p->rows * p->cols * B) * r;
+      } else {
+        in = (size_t)(p->rows * p->cols * B) * r;
+        out = (size_t)(p->rows * p->rcols * B) * e;
+      }
+      break;
+  }
+  *ib = in;
+  *ob = out;
+}
+
+// Argument checks shared by every exec path, before any work is enqueued: the
+// plan kind, the direction, the bin count, data pointers for the elements
+// the call reads and writes, and (real transforms) no overlap between input
+// and output. The kernels of a batched real transform write line l's output
+// over where line l+1's input would lie, and the reference's real calls
+// cannot alias either (their input and output slices have different element
+// types).
+int check_op(const gofft_plan* p, int op, const void* src, i64 bins, const void* dst, int direction) {
+  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
+  RC_TRY(check_kind(p, op));
+  if (op_has_direction(op) && direction != GOFFT_FORWARD && direction != GOFFT_INVERSE)
+    return fail(GOFFT_ERR_INVALID_ARG, "direction must be GOFFT_FORWARD or GOFFT_INVERSE");
+  if (op_takes_bins(op, direction) && bins < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative bin count");
+  size_t ib = 0, ob = 0;
+  io_bytes(p, op, direction, bins, &ib, &ob);
+  if ((ib > 0 && !src) || (ob > 0 && !dst)) return fail(GOFFT_ERR_INVALID_ARG, "null data pointer");
+  if (op != OP_C2C && op != OP_VJP_C2C && ib > 0 && ob > 0) {
+    const uintptr_t a = (uintptr_t)src, b = (uintptr_t)dst;
+    if (a < b + ob && b < a + ib)
+      return fail(GOFFT_ERR_INVALID_ARG, "input and output of a real transform must not overlap");
+  }
   return GOFFT_OK;
 }
 
-static int check_direction(int direction) {
-  if (direction != GOFFT_FORWARD && direction != GOFFT_INVERSE)
-    return fail(GOFFT_ERR_INVALID_ARG, "direction must be GOFFT_FORWARD or GOFFT_INVERSE");
+// Enqueues one checked op on stream s.
+int run_op(const gofft_plan* p, int op, const void* src, i64 bins, void* dst, int direction, hipStream_t s,
+           const WsSet* w) {
+  char* ws = nullptr;
+  if (p->kind != K_MANY) RC_TRY(exec_ws(p, s, w, &ws));
+  const int inv = direction == GOFFT_INVERSE;
+  const i64 B = p->batch;
+  switch (op) {
+    case OP_C2C:
+    case OP_VJP_C2C: {
+      // VJP: y = F x  =>  gx = F^H gy = conj(F conj(gy)) = n * IFFT(gy);
+      // y = IFFT x = F^H x / n  =>  gx = F gy / n.
+      const int c = op == OP_C2C ? inv : !inv;
+      const bool scaled = inv;
+      if (p->kind == K_C2C) {
+        if (p->n == 0 || B == 0) return GOFFT_OK;
+        return p->eng->run(src, LineIO{nullptr, p->n}, dst, LineIO{nullptr, p->n}, c, c,
+                           scaled ? 1.0 / (double)p->n : 1.0, Ctx{s, ws});
+      }
+      if (p->kind == K_MANY) return run_groups(p, src, dst, c, c, scaled, s, w);
+      if (p->nd_total * B == 0) return GOFFT_OK;
+      return run_nd_axes(p->axes, p->prec, p->nd_total * B, src, dst, c, scaled ? 1.0 / (double)p->nd_total : 1.0,
+                         s, ws);
+    }
+    case OP_R2C:
+      if (p->n == 0) return GOFFT_OK;
+      return p->real->forward(src, p->n, dst, p->n / 2 + 1, 0, 1.0, nullptr, Ctx{s, ws});
+    case OP_C2R: {
+      if (p->n <= 0) return GOFFT_OK;
+      const i64 keep = std::min<i64>(bins, p->n / 2 + 1);
+      return p->real->inverse(src, bins, keep, dst, p->n, Ctx{s, ws});
+    }
+    case OP_PSD:
+      if (p->n == 0) return GOFFT_OK;
+      return p->real->forward(src, p->n, dst, p->n / 2 + 1, 1, p->d / (double)p->n, nullptr, Ctx{s, ws});
+    case OP_SPECTROGRAM: {
+      if (p->frames == 0) return GOFFT_OK;
+      const i64 nbins = p->seg / 2 + 1;
+      const double sc = p->d / (double)p->seg;
+      if (!p->spec_gather) return p->real->forward(src, p->step, dst, nbins, 1, sc, p->window.p, Ctx{s, ws});
+      char* F = ws;
+      char* rest = ws + (size_t)p->frames * p->seg * relt(p->prec);
+      HIP_TRY(launch_frames(p->prec, src, p->window.p, F, p->seg, p->step, p->frames, s));
+      return p->real->forward(F, p->seg, dst, nbins, 1, sc, nullptr, Ctx{s, rest});
+    }
+    case OP_VJP_R2C: {
+      // y = RFFT(x): dL/dx_j = Re sum_k gy_k W^{-jk} over the n/2+1 bins,
+      // which is n * IRFFT(w * gy) with w = 1 at DC and Nyquist and 1/2
+      // elsewhere (the IRFFT counts every interior bin twice through its
+      // Hermitian mirror).
+      if (p->n <= 0) return GOFFT_OK;
+      const i64 nb = p->n / 2 + 1;
+      return p->real->inverse(src, nb, nb, dst, p->n, Ctx{s, ws}, (double)p->n, 0.5);
+    }
+    case OP_VJP_C2R: {
+      // x = IRFFT(X) reads Re X_0, Re X_{n/2} once and every other bin
+      // twice: gX = RFFT(gx) / n with those bins doubled (Im of DC and
+      // Nyquist is 0).
+      if (p->n <= 0) return GOFFT_OK;
+      const double in = 1.0 / (double)p->n;
+      return p->real->forward(src, p->n, dst, p->n / 2 + 1, 2, 0.0, nullptr, Ctx{s, ws}, in, 2.0 * in);
+    }
+    case OP_ND_R2C: {
+      if (B == 0) return GOFFT_OK;
+      const i64 half = p->rows * p->rcols;
+      const i64 htotal = half * B;
+      if (!inv) {
+        // Rows: real transform of each row into dst (rows x rcols), then the
+        // column FFT in place (rfft2.go:22-74).
+        RC_TRY(p->nd_row->forward(src, p->cols, dst, p->rcols, 0, 1.0, nullptr, Ctx{s, ws}));
+        return run_nd_axes(p->axes, p->prec, htotal, dst, dst, 0, 1.0, s, ws);
+      }
+      // IRFFT2 (rfft2.go:87-138): copy zero-padded to rows x rcols per
+      // array, column inverse, row IRFFT. `bins` complex values are supplied
+      // per array.
+      char* H = ws;
+      char* rest = ws + (size_t)htotal * elt(p->prec);
+      const i64 keep = std::min<i64>(bins, half);
+      if (keep < half) HIP_TRY(launch_fill(p->prec, H, 2 * htotal, 0.0, s));
+      if (keep > 0) HIP_TRY(launch_copy_lines(p->prec, src, H, keep, B, bins, half, s));
+      RC_TRY(run_nd_axes(p->axes, p->prec, htotal, H, H, 1, 1.0 / (double)p->rows, s, rest));
+      return p->nd_row->inverse(H, p->rcols, p->rcols, dst, p->cols, Ctx{s, rest});
+    }
+    case OP_VJP_ND_R2C: {
+      // direction 0, y = RFFT2(x): gx = rows*cols * IRFFT2(w * gy) with the
+      // r2c weights w over the bin axis (they commute with the column
+      // transforms). direction 1, x = IRFFT2(X): gX = w' * RFFT2(gx) /
+      // (rows*cols) with the c2r weights w'.
+      if (B == 0) return GOFFT_OK;
+      const i64 htotal = p->rows * p->rcols * B;
+      if (!inv) {
+        char* H = ws;
+        char* rest = ws + (size_t)htotal * elt(p->prec);
+        RC_TRY(run_nd_axes(p->axes, p->prec, htotal, src, H, 1, 1.0, s, rest));
+        return p->nd_row->inverse(H, p->rcols, p->rcols, dst, p->cols, Ctx{s, rest}, (double)p->cols, 0.5);
+      }
+      const double in = 1.0 / ((double)p->rows * (double)p->cols);
+      RC_TRY(p->nd_row->forward(src, p->cols, dst, p->rcols, 2, 0.0, nullptr, Ctx{s, ws}, in, 2.0 * in));
+      return run_nd_axes(p->axes, p->prec, htotal, dst, dst, 0, 1.0, s, ws);
+    }
+  }
+  return fail(GOFFT_ERR_INVALID_ARG, "unknown operation");
+}
+
+int exec_op(const gofft_plan* p, int op, const void* src, i64 bins, void* dst, int direction, void* stream,
+            const WsSet* w = nullptr) {
+  RC_TRY(check_op(p, op, src, bins, dst, direction));
+  return run_op(p, op, src, bins, dst, direction, (hipStream_t)stream, w);
+}
+
+// Host-staged op: the caller's input goes through pinned staging to a device
+// copy, the op runs on the session's own stream, and the output comes back
+// the same way; the call waits for that stream only. Chunks of the caller's
+// memory are copied into one staging buffer while the other's DMA runs.
+const size_t kStageChunk = (size_t)4 << 20;
+
+int host_op(const gofft_plan* p, int op, const void* src, i64 bins, void* dst, int direction) {
+  RC_TRY(check_op(p, op, src, bins, dst, direction));
+  size_t ib = 0, ob = 0;
+  io_bytes(p, op, direction, bins, &ib, &ob);
+  HostSession* h = nullptr;
+  RC_TRY(p->host.acquire(&h));
+  struct Release {
+    const gofft_plan* p;
+    HostSession* h;
+    ~Release() {
+      (void)hipStreamSynchronize(h->s);
+      p->host.release(h);
+    }
+  } rel{p, h};
+  if (h->din.bytes < ib) RC_TRY(h->din.alloc(ib));
+  if (h->dout.bytes < ob) RC_TRY(h->dout.alloc(ob));
+  const size_t chunk = std::min(kStageChunk, std::max(ib, ob));
+  for (PinnedBuf& b : h->stage)
+    if (chunk > 0 && b.bytes < chunk) RC_TRY(b.alloc(chunk));
+  // in: fill staging buffer k % 2 once its previous DMA has finished
+  int k = 0;
+  for (size_t off = 0; off < ib; off += chunk, k++) {
+    const size_t b = std::min(chunk, ib - off);
+    const int j = k & 1;
+    if (k >= 2) HIP_TRY(hipEventSynchronize(h->done[j]));
+    std::memcpy(h->stage[j].p, (const char*)src + off, b);
+    HIP_TRY(hipMemcpyAsync((char*)h->din.p + off, h->stage[j].p, b, hipMemcpyHostToDevice, h->s));
+    HIP_TRY(hipEventRecord(h->done[j], h->s));
+  }
+  RC_TRY(run_op(p, op, ib ? h->din.p : nullptr, bins, ob ? h->dout.p : nullptr, direction, h->s, nullptr));
+  // out: the DMA of chunk k + 1 runs while chunk k is copied out
+  const size_t nout = chunk ? (ob + chunk - 1) / chunk : 0;
+  auto issue = [&](size_t c) -> int {
+    const size_t off = c * chunk, b = std::min(chunk, ob - off);
+    HIP_TRY(hipMemcpyAsync(h->stage[c & 1].p, (const char*)h->dout.p + off, b, hipMemcpyDeviceToHost, h->s));
+    HIP_TRY(hipEventRecord(h->done[c & 1], h->s));
+    return GOFFT_OK;
+  };
+  if (nout > 0) RC_TRY(issue(0));
+  for (size_t c = 0; c < nout; c++) {
+    if (c + 1 < nout) RC_TRY(issue(c + 1));
+    HIP_TRY(hipEventSynchronize(h->done[c & 1]));
+    const size_t off = c * chunk, b = std::min(chunk, ob - off);
+    std::memcpy((char*)dst + off, h->stage[c & 1].p, b);
+  }
+  HIP_TRY(hipStreamSynchronize(h->s));
   return GOFFT_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int gofft_exec_c2c(const gofft_plan* p, const void* src, void* dst, int direction, void* stream) {
-  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
-  RC_TRY(check_direction(direction));
-  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = nullptr;
-  RC_TRY(p->ws(s, &ws));
-  const int inv = direction == GOFFT_INVERSE;
-  if (p->kind == K_C2C) {
-    if (p->n == 0 || p->batch == 0) return GOFFT_OK;
-    Ctx c{s, ws};
-    return p->eng->run(src, LineIO{nullptr, p->n}, dst, LineIO{nullptr, p->n}, inv, inv,
-                       inv ? 1.0 / (double)p->n : 1.0, c);
-  }
-  if (p->kind == K_MANY) return run_groups(p, src, dst, inv, inv, inv, s);
-  if (p->kind == K_ND_C2C) {
-    if (p->nd_total * p->batch == 0) return GOFFT_OK;
-    return run_nd_axes(p->axes, p->prec, p->nd_total * p->batch, src, dst, inv,
-                       inv ? 1.0 / (double)p->nd_total : 1.0, s, ws);
-  }
-  return fail(GOFFT_ERR_INVALID_ARG, "plan is not a complex-to-complex plan");
+  return exec_op(p, OP_C2C, src, 0, dst, direction, stream);
 }
-
 int gofft_exec_r2c(const gofft_plan* p, const void* src, void* dst, void* stream) {
-  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
-  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = nullptr;
-  RC_TRY(p->ws(s, &ws));
-  if (p->n == 0) return GOFFT_OK;
-  return p->real->forward(src, p->n, dst, p->n / 2 + 1, 0, 1.0, nullptr, Ctx{s, ws});
+  return exec_op(p, OP_R2C, src, 0, dst, GOFFT_FORWARD, stream);
 }
-
 int gofft_exec_c2r(const gofft_plan* p, const void* src, int64_t bins, void* dst, void* stream) {
-  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
-  if (bins < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative bin count");
-  RC_TRY(check_data(p, GOFFT_INVERSE, src, dst, bins > 0));
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = nullptr;
-  RC_TRY(p->ws(s, &ws));
-  if (p->n <= 0) return GOFFT_OK;
-  i64 keep = std::min<i64>(bins, p->n / 2 + 1);
-  return p->real->inverse(src, bins, keep, dst, p->n, Ctx{s, ws});
+  return exec_op(p, OP_C2R, src, bins, dst, GOFFT_INVERSE, stream);
 }
-
 int gofft_exec_psd(const gofft_plan* p, const void* src, void* dst, void* stream) {
-  if (!p || p->kind != K_PSD) return fail(GOFFT_ERR_INVALID_ARG, "plan is not a psd plan");
-  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = nullptr;
-  RC_TRY(p->ws(s, &ws));
-  if (p->n == 0) return GOFFT_OK;
-  return p->real->forward(src, p->n, dst, p->n / 2 + 1, 1, p->d / (double)p->n, nullptr, Ctx{s, ws});
+  return exec_op(p, OP_PSD, src, 0, dst, GOFFT_FORWARD, stream);
 }
-
 int gofft_exec_spectrogram(const gofft_plan* p, const void* src, void* dst, void* stream) {
-  if (!p || p->kind != K_SPEC) return fail(GOFFT_ERR_INVALID_ARG, "plan is not a spectrogram plan");
-  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = nullptr;
-  RC_TRY(p->ws(s, &ws));
-  if (p->frames == 0) return GOFFT_OK;
-  const i64 bins = p->seg / 2 + 1;
-  const double sc = p->d / (double)p->seg;
-  if (!p->spec_gather)
-    return p->real->forward(src, p->step, dst, bins, 1, sc, p->window.p, Ctx{s, ws});
-  char* F = ws;
-  char* rest = ws + (size_t)p->frames * p->seg * relt(p->prec);
-  HIP_TRY(launch_frames(p->prec, src, p->window.p, F, p->seg, p->step, p->frames, s));
-  return p->real->forward(F, p->seg, dst, bins, 1, sc, nullptr, Ctx{s, rest});
+  return exec_op(p, OP_SPECTROGRAM, src, 0, dst, GOFFT_FORWARD, stream);
 }
-
 int gofft_exec_nd_r2c(const gofft_plan* p, const void* src, int64_t bins, void* dst, int direction,
                       void* stream) {
-  if (!p || p->kind != K_ND_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an nd r2c plan");
-  RC_TRY(check_direction(direction));
-  if (direction == GOFFT_INVERSE && bins < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative bin count");
-  RC_TRY(check_data(p, direction, src, dst, direction == GOFFT_FORWARD || bins > 0));
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = nullptr;
-  RC_TRY(p->ws(s, &ws));
-  const i64 B = p->batch;
-  if (B == 0) return GOFFT_OK;
-  const i64 half = p->rows * p->rcols;
-  const size_t pad_bytes = (size_t)half * B * elt(p->prec);
-  const i64 htotal = half * B;
-  if (direction == GOFFT_FORWARD) {
-    // Rows: real transform of each row into dst (rows x rcols), then the
-    // column FFT in place (rfft2.go:22-74).
-    RC_TRY(p->nd_row->forward(src, p->cols, dst, p->rcols, 0, 1.0, nullptr, Ctx{s, ws}));
-    return run_nd_axes(p->axes, p->prec, htotal, dst, dst, 0, 1.0, s, ws);
-  }
-  // IRFFT2 (rfft2.go:87-138): copy zero-padded to rows x rcols per array,
-  // column inverse, row IRFFT. `bins` complex values are supplied per array.
-  if (bins < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative bin count");
-  char* H = ws;
-  char* rest = ws + pad_bytes;
-  i64 keep = std::min<i64>(bins, half);
-  if (keep < half) HIP_TRY(launch_fill(p->prec, H, 2 * htotal, 0.0, s));
-  if (keep > 0) HIP_TRY(launch_copy_lines(p->prec, src, H, keep, B, bins, half, s));
-  RC_TRY(run_nd_axes(p->axes, p->prec, htotal, H, H, 1, 1.0 / (double)p->rows, s, rest));
-  return p->nd_row->inverse(H, p->rcols, p->rcols, dst, p->cols, Ctx{s, rest});
+  return exec_op(p, OP_ND_R2C, src, bins, dst, direction, stream);
 }
-
 int gofft_vjp_c2c(const gofft_plan* p, const void* gy, void* gx, int direction, void* stream) {
-  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
-  RC_TRY(check_direction(direction));
-  RC_TRY(check_data(p, GOFFT_INVERSE, gy, gx));
-  // y = F x  =>  gx = F^H gy = conj(F conj(gy)) = n * IFFT(gy).
-  // y = IFFT x = F^H x / n  =>  gx = F gy / n.
-  if (p->kind == K_C2C || p->kind == K_MANY || p->kind == K_ND_C2C) {
-    hipStream_t s = (hipStream_t)stream;
-    char* ws = nullptr;
-    RC_TRY(p->ws(s, &ws));
-    if (p->kind == K_C2C) {
-      if (p->n == 0 || p->batch == 0) return GOFFT_OK;
-      Ctx c{s, ws};
-      int adj = direction == GOFFT_FORWARD;
-      return p->eng->run(gy, LineIO{nullptr, p->n}, gx, LineIO{nullptr, p->n}, adj, adj,
-                         adj ? 1.0 : 1.0 / (double)p->n, c);
-    }
-    if (p->kind == K_MANY) {
-      int adj = direction == GOFFT_FORWARD;
-      return run_groups(p, gy, gx, adj, adj, !adj, s);
-    }
-    int adj = direction == GOFFT_FORWARD;
-    return run_nd_axes(p->axes, p->prec, p->nd_total * p->batch, gy, gx, adj,
-                       adj ? 1.0 : 1.0 / (double)p->nd_total, s, ws);
-  }
-  return fail(GOFFT_ERR_INVALID_ARG, "plan is not a complex-to-complex plan");
+  return exec_op(p, OP_VJP_C2C, gy, 0, gx, direction, stream);
 }
-
-// y = RFFT(x): dL/dx_j = Re sum_k gy_k W^{-jk} over the n/2+1 bins, which is
-// n * IRFFT(w * gy) with w = 1 at DC and Nyquist and 1/2 elsewhere (the
-// IRFFT counts every interior bin twice through its Hermitian mirror).
 int gofft_vjp_r2c(const gofft_plan* p, const void* gy, void* gx, void* stream) {
-  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
-  RC_TRY(check_data(p, GOFFT_INVERSE, gy, gx));
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = nullptr;
-  RC_TRY(p->ws(s, &ws));
-  if (p->n <= 0) return GOFFT_OK;
-  const i64 nb = p->n / 2 + 1;
-  return p->real->inverse(gy, nb, nb, gx, p->n, Ctx{s, ws}, (double)p->n, 0.5);
+  return exec_op(p, OP_VJP_R2C, gy, 0, gx, GOFFT_FORWARD, stream);
 }
-
-// x = IRFFT(X) reads Re X_0, Re X_{n/2} once and every other bin twice:
-// gX = RFFT(gx) / n with those bins doubled (Im of DC and Nyquist is 0).
 int gofft_vjp_c2r(const gofft_plan* p, const void* gx, void* gX, void* stream) {
-  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
-  RC_TRY(check_data(p, GOFFT_FORWARD, gx, gX));
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = nullptr;
-  RC_TRY(p->ws(s, &ws));
-  if (p->n <= 0) return GOFFT_OK;
-  const double inv = 1.0 / (double)p->n;
-  return p->real->forward(gx, p->n, gX, p->n / 2 + 1, 2, 0.0, nullptr, Ctx{s, ws}, inv, 2.0 * inv);
+  return exec_op(p, OP_VJP_C2R, gx, 0, gX, GOFFT_FORWARD, stream);
 }
-
-// direction 0, y = RFFT2(x): gx = rows*cols * IRFFT2(w * gy) with the r2c
-// weights w over the bin axis (they commute with the column transforms).
-// direction 1, x = IRFFT2(X): gX = w' * RFFT2(gx) / (rows*cols) with the c2r
-// weights w'.
 int gofft_vjp_nd_r2c(const gofft_plan* p, const void* g_out, void* g_in, int direction, void* stream) {
-  if (!p || p->kind != K_ND_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an nd r2c plan");
-  RC_TRY(check_direction(direction));
-  RC_TRY(check_data(p, direction == GOFFT_FORWARD ? GOFFT_INVERSE : GOFFT_FORWARD, g_out, g_in));
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = nullptr;
-  RC_TRY(p->ws(s, &ws));
-  const i64 B = p->batch;
-  if (B == 0) return GOFFT_OK;
-  const i64 htotal = p->rows * p->rcols * B;
-  if (direction == GOFFT_FORWARD) {
-    char* H = ws;
-    char* rest = ws + (size_t)htotal * elt(p->prec);
-    RC_TRY(run_nd_axes(p->axes, p->prec, htotal, g_out, H, 1, 1.0, s, rest));
-    return p->nd_row->inverse(H, p->rcols, p->rcols, g_in, p->cols, Ctx{s, rest}, (double)p->cols, 0.5);
-  }
-  const double inv = 1.0 / ((double)p->rows * (double)p->cols);
-  RC_TRY(p->nd_row->forward(g_out, p->cols, g_in, p->rcols, 2, 0.0, nullptr, Ctx{s, ws}, inv, 2.0 * inv));
-  return run_nd_axes(p->axes, p->prec, htotal, g_in, g_in, 0, 1.0, s, ws);
+  return exec_op(p, OP_VJP_ND_R2C, g_out, 0, g_in, direction, stream);
 }
 
 // ---------------------------------------------------------------------------
-// Host convenience calls: stage through device memory.
-
-static int host_roundtrip(size_t in_bytes, size_t out_bytes, const void* src, void* dst,
-                          const std::function<int(const void*, void*)>& fn) {
-  DevBuf din, dout;
-  RC_TRY(din.alloc(in_bytes ? in_bytes : 16));
-  RC_TRY(dout.alloc(out_bytes ? out_bytes : 16));
-  if (in_bytes) HIP_TRY(hipMemcpy(din.p, src, in_bytes, hipMemcpyHostToDevice));
-  RC_TRY(fn(din.p, dout.p));
-  HIP_TRY(hipDeviceSynchronize());
-  if (out_bytes) HIP_TRY(hipMemcpy(dst, dout.p, out_bytes, hipMemcpyDeviceToHost));
-  return GOFFT_OK;
-}
+// Host convenience calls (the reference's slice API).
 
 int gofft_exec_c2c_host(const gofft_plan* p, const void* src, void* dst, int direction) {
-  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
-  RC_TRY(check_direction(direction));
-  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
-  size_t b = (size_t)gofft_plan_in_elems(p, direction) * elt(p->prec);
-  return host_roundtrip(b, b, src, dst, [&](const void* i, void* o) {
-    return gofft_exec_c2c(p, i, o, direction, nullptr);
-  });
+  return host_op(p, OP_C2C, src, 0, dst, direction);
 }
-
 int gofft_exec_r2c_host(const gofft_plan* p, const void* src, void* dst) {
-  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
-  RC_TRY(check_data(p, GOFFT_FORWARD, src, dst));
-  size_t ib = (size_t)p->n * p->batch * relt(p->prec);
-  size_t ob = (size_t)(p->n / 2 + 1) * p->batch * elt(p->prec);
-  if (p->n == 0) ob = 0;
-  return host_roundtrip(ib, ob, src, dst, [&](const void* i, void* o) { return gofft_exec_r2c(p, i, o, nullptr); });
+  return host_op(p, OP_R2C, src, 0, dst, GOFFT_FORWARD);
 }
-
 int gofft_exec_c2r_host(const gofft_plan* p, const void* src, int64_t bins, void* dst) {
-  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
-  if (bins < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative bin count");
-  RC_TRY(check_data(p, GOFFT_INVERSE, src, dst, bins > 0));
-  size_t ib = (size_t)bins * p->batch * elt(p->prec);
-  size_t ob = (size_t)p->n * p->batch * relt(p->prec);
-  return host_roundtrip(ib, ob, src, dst,
-                        [&](const void* i, void* o) { return gofft_exec_c2r(p, i, bins, o, nullptr); });
+  return host_op(p, OP_C2R, src, bins, dst, GOFFT_INVERSE);
 }
-
+int gofft_exec_nd_r2c_host(const gofft_plan* p, const void* src, int64_t bins, void* dst, int direction) {
+  return host_op(p, OP_ND_R2C, src, bins, dst, direction);
+}
+int gofft_exec_psd_host(const gofft_plan* p, const void* src, void* dst) {
+  return host_op(p, OP_PSD, src, 0, dst, GOFFT_FORWARD);
+}
+int gofft_exec_spectrogram_host(const gofft_plan* p, const void* src, void* dst) {
+  return host_op(p, OP_SPECTROGRAM, src, 0, dst, GOFFT_FORWARD);
+}
 int gofft_vjp_c2c_host(const gofft_plan* p, const void* gy, void* gx, int direction) {
-  if (!p) return fail(GOFFT_ERR_INVALID_ARG, "null plan");
-  RC_TRY(check_direction(direction));
-  RC_TRY(check_data(p, GOFFT_INVERSE, gy, gx));
-  size_t b = (size_t)gofft_plan_in_elems(p, direction) * elt(p->prec);
-  return host_roundtrip(b, b, gy, gx, [&](const void* i, void* o) {
-    return gofft_vjp_c2c(p, i, o, direction, nullptr);
-  });
+  return host_op(p, OP_VJP_C2C, gy, 0, gx, direction);
 }
-
 int gofft_vjp_r2c_host(const gofft_plan* p, const void* gy, void* gx) {
-  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
-  RC_TRY(check_data(p, GOFFT_INVERSE, gy, gx));
-  size_t ib = p->n > 0 ? (size_t)(p->n / 2 + 1) * p->batch * elt(p->prec) : 0;
-  size_t ob = (size_t)p->n * p->batch * relt(p->prec);
-  return host_roundtrip(ib, ob, gy, gx, [&](const void* i, void* o) { return gofft_vjp_r2c(p, i, o, nullptr); });
+  return host_op(p, OP_VJP_R2C, gy, 0, gx, GOFFT_FORWARD);
 }
-
 int gofft_vjp_c2r_host(const gofft_plan* p, const void* gx, void* gX) {
-  if (!p || p->kind != K_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an r2c plan");
-  RC_TRY(check_data(p, GOFFT_FORWARD, gx, gX));
-  size_t ib = (size_t)p->n * p->batch * relt(p->prec);
-  size_t ob = p->n > 0 ? (size_t)(p->n / 2 + 1) * p->batch * elt(p->prec) : 0;
-  return host_roundtrip(ib, ob, gx, gX, [&](const void* i, void* o) { return gofft_vjp_c2r(p, i, o, nullptr); });
+  return host_op(p, OP_VJP_C2R, gx, 0, gX, GOFFT_FORWARD);
+}
+int gofft_vjp_nd_r2c_host(const gofft_plan* p, const void* g_out, void* g_in, int direction) {
+  return host_op(p, OP_VJP_ND_R2C, g_out, 0, g_in, direction);
 }
 
-// A captured exec: the launches (and, for mixed-length plans, the stream
-// fork and join) of one gofft_exec_c2c on fixed buffers, replayed as one
-// hipGraph. The graph uses the plan's workspace of the capture stream.
+// ---------------------------------------------------------------------------
+// Graphs: the launches (and, for mixed-length plans, the stream fork and
+// join) of one exec on fixed buffers, replayed as one hipGraph. Each graph
+// owns its workspace (one set per lane of a mixed-length plan), allocated at
+// capture, so its replays do not touch the plan's per-stream workspaces.
+
 struct gofft_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<std::unique_ptr<DevBuf>> ws;
   ~gofft_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
   }
 };
 
-int gofft_graph_c2c(gofft_graph** out, const gofft_plan* p, const void* src, void* dst, int direction,
-                    void* stream) {
-  if (!out || !p) return fail(GOFFT_ERR_INVALID_ARG, "null argument");
+int gofft_graph_capture(gofft_graph** out, const gofft_plan* p, int op, const void* src, int64_t bins, void* dst,
+                        int direction, void* stream) {
+  if (!out) return fail(GOFFT_ERR_INVALID_ARG, "null argument");
   *out = nullptr;
-  if (p->kind != K_C2C && p->kind != K_MANY && p->kind != K_ND_C2C)
-    return fail(GOFFT_ERR_INVALID_ARG, "plan is not a complex-to-complex plan");
+  if (op < 0 || op >= OP_COUNT) return fail(GOFFT_ERR_INVALID_ARG, "unknown operation");
+  RC_TRY(check_op(p, op, src, bins, dst, direction));
   hipStream_t s = (hipStream_t)stream;
   if (!s) return fail(GOFFT_ERR_INVALID_ARG, "capture needs a stream other than the null stream");
-  char* ws = nullptr;
-  RC_TRY(p->ws(s, &ws));  // allocate before capture: no allocation is captured
   auto g = std::make_unique<gofft_graph>();
+  // the graph's own workspace, allocated before capture: no allocation is captured
+  WsSet w;
+  const size_t nws = p->ws_need ? 1 + p->lanes.size() : 0;
+  for (size_t i = 0; i < nws; i++) {
+    g->ws.push_back(std::make_unique<DevBuf>());
+    RC_TRY(g->ws.back()->alloc(p->ws_need));
+  }
+  w.main = nws ? (char*)g->ws[0]->p : nullptr;
+  for (size_t i = 0; i < p->lanes.size(); i++) w.lanes.push_back(nws ? (char*)g->ws[1 + i]->p : nullptr);
   HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  const int rc = gofft_exec_c2c(p, src, dst, direction, stream);
+  const int rc = run_op(p, op, src, bins, dst, direction, s, &w);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(s, &graph);
   if (rc != GOFFT_OK) {
@@ -3170,6 +3522,11 @@ int gofft_graph_c2c(gofft_graph** out, const gofft_plan* p, const void* src, voi
   HIP_TRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
   *out = g.release();
   return GOFFT_OK;
+}
+
+int gofft_graph_c2c(gofft_graph** out, const gofft_plan* p, const void* src, void* dst, int direction,
+                    void* stream) {
+  return gofft_graph_capture(out, p, OP_C2C, src, 0, dst, direction, stream);
 }
 
 int gofft_graph_launch(const gofft_graph* g, void* stream) {

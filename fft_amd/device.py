@@ -71,6 +71,16 @@ class _Plan:
         L.check(L.lib().gofft_plan_kernel_work(self._h, direction, kernel, ctypes.byref(n), ctypes.byref(b)))
         return n.value, b.value
 
+    def index_table(self, n, table):
+        """Integer index table `table` (L.TABLE_*) of the plan's length-n transform, as int64 numpy."""
+        import numpy as np
+        k = L.lib().gofft_plan_index_table(self._h, n, table, None, 0)
+        if k < 0:
+            raise L.GofftError(int(k), L.last_error())
+        out = np.zeros(max(1, k), dtype=np.int64)
+        L.lib().gofft_plan_index_table(self._h, n, table, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), k)
+        return out[:k]
+
     def in_elems(self, direction=0):
         return L.lib().gofft_plan_in_elems(self._h, direction)
 
@@ -78,15 +88,40 @@ class _Plan:
         return L.lib().gofft_plan_out_elems(self._h, direction)
 
 
-class Graph:
-    """A captured exec of a complex plan on fixed tensors (gofft_graph_c2c);
-    launch() replays it on a stream. Keeps the plan and tensors alive."""
+def run_op(plan, op, x, out, bins=0, inverse=False, stream=None):
+    """One call of operation `op` (L.OP_*: the exec and VJP entry points) on
+    device tensors, with `bins` and the direction where the op takes them."""
+    lib, h, s = L.lib(), plan.handle, _stream(stream)
+    d = int(bool(inverse))
+    xi, oi = _ptr(x), _ptr(out)
+    calls = {
+        L.OP_C2C: lambda: lib.gofft_exec_c2c(h, xi, oi, d, s),
+        L.OP_R2C: lambda: lib.gofft_exec_r2c(h, xi, oi, s),
+        L.OP_C2R: lambda: lib.gofft_exec_c2r(h, xi, int(bins), oi, s),
+        L.OP_ND_R2C: lambda: lib.gofft_exec_nd_r2c(h, xi, int(bins), oi, d, s),
+        L.OP_PSD: lambda: lib.gofft_exec_psd(h, xi, oi, s),
+        L.OP_SPECTROGRAM: lambda: lib.gofft_exec_spectrogram(h, xi, oi, s),
+        L.OP_VJP_C2C: lambda: lib.gofft_vjp_c2c(h, xi, oi, d, s),
+        L.OP_VJP_R2C: lambda: lib.gofft_vjp_r2c(h, xi, oi, s),
+        L.OP_VJP_C2R: lambda: lib.gofft_vjp_c2r(h, xi, oi, s),
+        L.OP_VJP_ND_R2C: lambda: lib.gofft_vjp_nd_r2c(h, xi, oi, d, s),
+    }
+    L.check(calls[op]())
+    return out
 
-    def __init__(self, plan, x, out, inverse=False, stream=None):
+
+class Graph:
+    """One captured call of a plan on fixed tensors (gofft_graph_capture):
+    operation `op` (L.OP_*, default the complex transform), with `bins` and
+    the direction (`inverse`) where the op takes them. launch() replays it on
+    a stream. The graph owns its workspace; it keeps the plan and tensors
+    alive."""
+
+    def __init__(self, plan, x, out, inverse=False, stream=None, op=L.OP_C2C, bins=0):
         self._g = ctypes.c_void_p()
         self._keep = (plan, x, out)
-        L.check(L.lib().gofft_graph_c2c(ctypes.byref(self._g), plan.handle, _ptr(x), _ptr(out),
-                                        int(bool(inverse)), _stream(stream)))
+        L.check(L.lib().gofft_graph_capture(ctypes.byref(self._g), plan.handle, int(op), _ptr(x), int(bins),
+                                            _ptr(out), int(bool(inverse)), _stream(stream)))
 
     def launch(self, stream=None):
         L.check(L.lib().gofft_graph_launch(self._g, _stream(stream)))

@@ -15,8 +15,13 @@
  *    memory and return after the result is in host memory.
  *  - Forward transforms are unnormalised; inverses divide by n (by the product
  *    of the axis lengths for N-D), as the reference does (doc.go:21-26).
- *  - dst may alias src for every transform (plan.go:88-90); src is never
- *    modified unless it aliases dst.
+ *  - For complex-to-complex transforms (c2c, mixed-length and N-D C2C plans)
+ *    and their VJPs, dst may alias src (plan.go:88-90). The real transforms
+ *    (r2c, c2r, PSD, Spectrogram, RFFT2/IRFFT2) and their VJPs return
+ *    GOFFT_ERR_INVALID_ARG when the input and output ranges overlap: the
+ *    reference's real calls cannot alias either, their slices having
+ *    different element types (realplan.go:77, 181). src is never modified
+ *    unless it aliases dst.
  *  - Every call returns GOFFT_OK (0) or a negative status; gofft_last_error()
  *    returns a thread-local message for the last failure on this thread.
  *  - Transform and VJP calls on a plan with data reject a NULL data pointer,
@@ -24,8 +29,11 @@
  *    GOFFT_ERR_INVALID_ARG before any work is enqueued (an inverse given
  *    zero bins may pass NULL for its input).
  *  - Plans are immutable after creation and safe to execute concurrently from
- *    several threads provided each concurrent call uses its own stream; the
- *    plan keeps one workspace per stream (plan.go:12-14, realplan.go:10-11).
+ *    several threads provided each concurrent device call uses its own
+ *    stream; the plan keeps one workspace per stream (plan.go:12-14,
+ *    realplan.go:10-11). Host-staged calls need no stream: each concurrent
+ *    call gets its own stream, device buffers and pinned staging from the
+ *    plan, made on first use and reused, and waits for that stream only.
  */
 #ifndef GOFFT_H
 #define GOFFT_H
@@ -155,6 +163,22 @@ enum {
 int gofft_plan_kernel_work(const gofft_plan* p, int direction, int kernel, int64_t* launches,
                            int64_t* bytes);
 
+/* Integer index tables of the plan's transform of length n, for checking
+   them bit for bit (no reference counterpart; diagnostics). Copies up to
+   `cap` entries into out (may be NULL) and returns the table's length, or a
+   negative status. The permutations are read back from the device copies
+   the kernels index with; the chirp indices are the ones the plan's chirp
+   tables were built from. */
+enum {
+  GOFFT_TABLE_RADER_GENERATOR = 0, /* {g}: smallest primitive root mod n (rader.go:178-197) */
+  GOFFT_TABLE_RADER_PERM = 1,      /* perm[k] = g^k mod n, k < n-1 (rader.go:78-82) */
+  GOFFT_TABLE_RADER_IPERM = 2,     /* iperm[k] = perm[(n-1-k) mod (n-1)] (rader.go:83-86) */
+  GOFFT_TABLE_RADER_CONV = 3,      /* {convolution length, 1 if cyclic} */
+  GOFFT_TABLE_CHIRP_INDEX = 4,     /* j^2 mod 2n, j < n (bluestein.go:35-41) */
+  GOFFT_TABLE_BLUESTEIN_CONV = 5   /* {convolution length m} */
+};
+int64_t gofft_plan_index_table(const gofft_plan* plan, int64_t n, int table, int64_t* out, int64_t cap);
+
 /* --- execution on device memory -------------------------------------------- */
 
 /* c2c / c2c_many / nd-C2C plans. */
@@ -191,26 +215,58 @@ int gofft_vjp_nd_r2c(const gofft_plan* plan, const void* g_out, void* g_in, int 
 
 /* --- host convenience (reference slice API) -------------------------------- */
 
-/* Plan.FFT / Plan.IFFT on host slices (plan.go:91-109): stages through device
- * memory, runs, copies back. n*batch complex elements in and out. */
+/* The same operations on HOST buffers (ordinary pageable memory, e.g. Go
+ * slices): the input is copied through the plan's pinned staging buffers to
+ * device memory, the transform runs on a stream of the call's own, and the
+ * output comes back the same way; the call returns when the output is in
+ * dst and waits for nothing else. dst may alias src here for every
+ * transform (the device copies are separate). Element counts as for the
+ * device calls. */
+/* Plan.FFT / Plan.IFFT (plan.go:91-109), FFT/IFFT (fft.go:14-40), and
+ * FFTN/IFFTN/FFT2/IFFT2 (ndfft.go:22-45) with an nd-C2C plan. */
 int gofft_exec_c2c_host(const gofft_plan* plan, const void* src, void* dst, int direction);
+/* RealPlan.RFFT (realplan.go:77-116) / RFFT (rfft.go:19-26). */
 int gofft_exec_r2c_host(const gofft_plan* plan, const void* src, void* dst);
+/* RealPlan.IRFFT (realplan.go:181-192) / IRFFT (rfft.go:39-45). */
 int gofft_exec_c2r_host(const gofft_plan* plan, const void* src, int64_t bins, void* dst);
-/* Host-staged VJPs (no reference counterpart): as gofft_vjp_c2c, _r2c and
-   _c2r, with host buffers. */
+/* RFFT2 / IRFFT2 (rfft2.go:22-138). */
+int gofft_exec_nd_r2c_host(const gofft_plan* plan, const void* src, int64_t bins, void* dst, int direction);
+/* PSD (freq.go:69-90). */
+int gofft_exec_psd_host(const gofft_plan* plan, const void* src, void* dst);
+/* Spectrogram (freq.go:107-127): frames x (segment/2+1) densities, row-major. */
+int gofft_exec_spectrogram_host(const gofft_plan* plan, const void* src, void* dst);
+/* Host-staged VJPs (no reference counterpart): as gofft_vjp_c2c, _r2c, _c2r
+   and _nd_r2c, with host buffers. */
 int gofft_vjp_c2c_host(const gofft_plan* plan, const void* gy, void* gx, int direction);
 int gofft_vjp_r2c_host(const gofft_plan* plan, const void* gy, void* gx);
 int gofft_vjp_c2r_host(const gofft_plan* plan, const void* gx, void* gX);
+int gofft_vjp_nd_r2c_host(const gofft_plan* plan, const void* g_out, void* g_in, int direction);
 
 /* --- graphs (no reference counterpart) ------------------------------------ */
 
-/* One gofft_exec_c2c of a c2c, mixed-length or nd-c2c plan on fixed device
-   buffers, captured on `stream` (not the null stream) as a hipGraph. A
-   launch replays every kernel of that exec (and a mixed-length plan's stream
-   fork and join) at once. The graph uses the plan's workspace of the capture
-   stream, so its replays must not overlap each other or execs of the same
-   plan on that stream. The plan must outlive the graph. */
+/* Operations a graph can capture (the exec and VJP calls above). */
+#define GOFFT_OP_C2C 0         /* gofft_exec_c2c */
+#define GOFFT_OP_R2C 1         /* gofft_exec_r2c */
+#define GOFFT_OP_C2R 2         /* gofft_exec_c2r */
+#define GOFFT_OP_ND_R2C 3      /* gofft_exec_nd_r2c */
+#define GOFFT_OP_PSD 4         /* gofft_exec_psd */
+#define GOFFT_OP_SPECTROGRAM 5 /* gofft_exec_spectrogram */
+#define GOFFT_OP_VJP_C2C 6     /* gofft_vjp_c2c */
+#define GOFFT_OP_VJP_R2C 7     /* gofft_vjp_r2c */
+#define GOFFT_OP_VJP_C2R 8     /* gofft_vjp_c2r */
+#define GOFFT_OP_VJP_ND_R2C 9  /* gofft_vjp_nd_r2c */
+
+/* One call of operation `op` on the plan with fixed device buffers (and
+   `bins` / `direction` where the op takes them), captured on `stream` (not
+   the null stream) as a hipGraph: a launch replays every kernel of that call
+   (and a mixed-length plan's stream fork and join) at once. Each graph owns
+   its workspace, allocated here, so replays of one graph may overlap execs of
+   the plan and replays of other graphs, but not each other. The plan must
+   outlive the graph. */
 typedef struct gofft_graph gofft_graph;
+int gofft_graph_capture(gofft_graph** graph, const gofft_plan* plan, int op, const void* src, int64_t bins,
+                        void* dst, int direction, void* stream);
+/* gofft_graph_capture of GOFFT_OP_C2C. */
 int gofft_graph_c2c(gofft_graph** graph, const gofft_plan* plan, const void* src, void* dst, int direction,
                     void* stream);
 int gofft_graph_launch(const gofft_graph* graph, void* stream);

@@ -59,4 +59,7 @@ def real_signal(n):
 def oracle():
     from oracle import oracle as O
     O.lib()
+    # the line fan-out of the N-D transforms (parallel.go) on at most 16
+    # threads, the host share of one GPU box
+    O.set_workers(min(16, os.cpu_count() or 1))
     return O

@@ -115,3 +115,43 @@ def test_empty_plans_accept_null_pointers(env):
         assert lib.gofft_exec_c2c(p.handle, None, None, L.FORWARD, None) == L.OK, L.last_error()
     r = D.R2CPlan(0, 1, torch.float64)
     assert lib.gofft_exec_r2c(r.handle, None, None, None) == L.OK, L.last_error()
+
+
+def test_real_calls_reject_overlapping_buffers(env):
+    # Real transforms (and their VJPs) do not run in place: a batched call
+    # writes line l's output where line l+1's input lies. Overlapping input
+    # and output ranges return GOFFT_ERR_INVALID_ARG; the complex transforms
+    # keep accepting dst == src.
+    D, L, lib, torch = env
+    n, batch = 1024, 4
+    p = D.R2CPlan(n, batch, torch.float64)
+    buf = torch.zeros(batch * (n + 2), dtype=torch.float64, device="cuda")
+    h = p.handle
+    msg = "must not overlap"
+    _rejects(L, lib.gofft_exec_r2c(h, _ptr(buf), _ptr(buf), None), msg)
+    _rejects(L, lib.gofft_exec_c2r(h, _ptr(buf), n // 2 + 1, _ptr(buf), None), msg)
+    _rejects(L, lib.gofft_vjp_r2c(h, _ptr(buf), _ptr(buf), None), msg)
+    _rejects(L, lib.gofft_vjp_c2r(h, _ptr(buf), _ptr(buf), None), msg)
+    # a partial overlap: output starting in the middle of the input
+    _rejects(L, lib.gofft_exec_r2c(h, _ptr(buf), ctypes.c_void_p(buf.data_ptr() + 8 * n), None), msg)
+    ps = D.PSDPlan(n, batch, 1.0, torch.float64)
+    _rejects(L, lib.gofft_exec_psd(ps.handle, _ptr(buf), _ptr(buf), None), msg)
+    sp = D.SpectrogramPlan(batch * n, 256, 128, [1.0] * 256, 1.0, torch.float64)
+    _rejects(L, lib.gofft_exec_spectrogram(sp.handle, _ptr(buf), _ptr(buf), None), msg)
+    nd = D.NDPlan((32, 64), L.ND_R2C, 1, torch.complex128)
+    _rejects(L, lib.gofft_exec_nd_r2c(nd.handle, _ptr(buf), 0, _ptr(buf), L.FORWARD, None), msg)
+    _rejects(L, lib.gofft_exec_nd_r2c(nd.handle, _ptr(buf), 32 * 33, _ptr(buf), L.INVERSE, None), msg)
+    _rejects(L, lib.gofft_vjp_nd_r2c(nd.handle, _ptr(buf), _ptr(buf), L.FORWARD, None), msg)
+    # adjacent, non-overlapping ranges are fine
+    x = torch.randn(batch * n + batch * (n + 2), dtype=torch.float64, device="cuda")
+    X = x[batch * n:].view(torch.complex128)
+    assert lib.gofft_exec_r2c(h, _ptr(x), _ptr(X), None) == L.OK, L.last_error()
+    torch.cuda.synchronize()
+    want = torch.fft.rfft(x[: batch * n].reshape(batch, n)).reshape(-1)
+    assert ((X - want).norm() / want.norm()).item() <= 1e-13
+    c = D.C2CPlan(n, batch, torch.complex128)
+    z = torch.randn(batch * n, dtype=torch.complex128, device="cuda")
+    want = torch.fft.fft(z.reshape(batch, n)).reshape(-1)
+    assert lib.gofft_exec_c2c(c.handle, _ptr(z), _ptr(z), L.FORWARD, None) == L.OK
+    torch.cuda.synchronize()
+    assert ((z - want).norm() / want.norm()).item() <= 1e-13

@@ -1,7 +1,9 @@
 """GPU checks at the full sizes of the spectral and multi-dimensional
-workloads, through properties that do not need a full oracle run: round
-trips, Parseval's identity, single bins against direct sums with exact integer
-phases, and sampled spectrogram frames against the oracle's PSD.
+workloads: every output against the oracle (FP64 restatement of the
+reference, its N-D and spectrogram line fan-out on 16 host threads) on the
+same input, plus properties the oracle does not supply: round trips,
+Parseval's identity, and single bins against direct sums with exact integer
+phases.
 """
 import math
 
@@ -92,15 +94,20 @@ def test_spectrogram_2_26(dev, oracle, prec):
     out = sp.exec(xt).cpu().numpy().astype(np.float64).reshape(sp.frames, sp.bins)
     xs = x if prec == "f64" else x.astype(np.float32).astype(np.float64)
     bar = 1e-14 * 10 if prec == "f64" else 1e-6
+    # every frame against the oracle's Spectrogram (freq.go:107-127)
+    want = oracle.spectrogram(xs, seg, ov, win, 1.0, threads=16)
+    assert want.shape == out.shape
+    err = np.linalg.norm(out - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert err.max() <= bar, (prec, int(err.argmax()), float(err.max()))
+    # and a few frames through the oracle's PSD of the windowed frame
     rng = np.random.default_rng(1)
-    for t in [0, sp.frames - 1] + list(rng.choice(sp.frames, size=14, replace=False)):
+    for t in [0, sp.frames - 1] + list(rng.choice(sp.frames, size=4, replace=False)):
         frame = xs[t * (seg - ov): t * (seg - ov) + seg] * win
-        want = oracle.psd(frame, 1.0)
-        assert rel_l2(out[t], want) <= bar, (prec, t, rel_l2(out[t], want))
+        assert rel_l2(out[t], oracle.psd(frame, 1.0)) <= bar, (prec, t)
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_psd_2_26_parseval_and_bins(dev, prec):
+def test_psd_2_26(dev, oracle, prec):
     D, torch = dev
     n = 1 << 26
     x = signal(n, seed=2)
@@ -109,6 +116,11 @@ def test_psd_2_26_parseval_and_bins(dev, prec):
     xt = torch.from_numpy(x).to("cuda", rt)
     got = p.exec(xt).cpu().numpy().astype(np.float64)
     xs = x if prec == "f64" else x.astype(np.float32).astype(np.float64)
+    # the whole PSD against the oracle's (freq.go:69-90); |X|^2 doubles the
+    # spectrum's relative error
+    want = oracle.psd(xs, 0.5)
+    bar = 2e-14 * 25 if prec == "f64" else 1e-6
+    assert rel_l2(got, want) <= bar, (prec, rel_l2(got, want))
     # Parseval: sum_k psd_k = d * sum_j x_j^2
     want_sum = 0.5 * float(np.sum(xs * xs))
     tol = 1e-12 if prec == "f64" else 1e-5
@@ -122,17 +134,27 @@ def test_psd_2_26_parseval_and_bins(dev, prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_rfft_irfft_2_24(dev, prec):
+def test_rfft_irfft_2_24(dev, oracle, prec):
     D, torch = dev
     n = 1 << 24
     x = signal(n, seed=3)
     rt = torch.float64 if prec == "f64" else torch.float32
+    ct = torch.complex128 if prec == "f64" else torch.complex64
     p = D.R2CPlan(n, 1, rt)
     xt = torch.from_numpy(x).to("cuda", rt)
     X = p.forward(xt)
     back = p.inverse(X).cpu().numpy().astype(np.float64)
     Xh = X.cpu().numpy().astype(np.complex128)
     xs = x if prec == "f64" else x.astype(np.float32).astype(np.float64)
+    bar = 1e-14 * 24 if prec == "f64" else 1e-6
+    # the whole spectrum against the oracle's RealPlan.RFFT (realplan.go:77-116)
+    want = oracle.rfft(xs)
+    assert rel_l2(Xh, want) <= bar, (prec, rel_l2(Xh, want))
+    # the inverse of the oracle's spectrum against the oracle's IRFFT
+    Wd = torch.from_numpy(want).to("cuda", ct)
+    got_i = p.inverse(Wd).cpu().numpy().astype(np.float64)
+    want_i = oracle.irfft(want if prec == "f64" else want.astype(np.complex64).astype(np.complex128), n)
+    assert rel_l2(got_i, want_i) <= bar, (prec, rel_l2(got_i, want_i))
     assert rel_l2(back, xs) <= (1e-14 * 24 if prec == "f64" else 1e-6)
     scale = math.sqrt(float(np.sum(xs * xs)))
     for k in [0, 3, 4999999, n // 2]:
@@ -141,7 +163,7 @@ def test_rfft_irfft_2_24(dev, prec):
 
 
 @pytest.mark.parametrize("prec", ["c128", "c64"])
-def test_fft2_4096(dev, prec):
+def test_fft2_4096(dev, oracle, prec):
     D, torch = dev
     from fft_amd import _lib as L
     rows = cols = 4096
@@ -155,13 +177,21 @@ def test_fft2_4096(dev, prec):
     back = p.exec(y, inverse=True).cpu().numpy().astype(np.complex128)
     Y = y.cpu().numpy().astype(np.complex128).reshape(rows, cols)
     assert rel_l2(back, xs.ravel()) <= (1e-14 * 24 if prec == "c128" else 1e-6)
+    # the whole transform against the oracle's FFT2 (ndfft.go:22-45)
+    bar = 1e-14 * 24 if prec == "c128" else 1e-6
+    want = oracle.fftn(xs.ravel(), (rows, cols))
+    assert rel_l2(Y, want) <= bar, (prec, rel_l2(Y, want))
+    del want
+    wanti = oracle.fftn(xs.ravel(), (rows, cols), inverse=True)
+    goti = p.exec(xt, inverse=True).cpu().numpy().astype(np.complex128)
+    assert rel_l2(goti, wanti) <= bar, (prec, rel_l2(goti, wanti))
     scale = math.sqrt(float(np.sum(np.abs(xs) ** 2)))
     for u, v in [(0, 0), (1, 4095), (2049, 77)]:
         want = dft_bin_2d(xs, u, v)
         assert abs(Y[u, v] - want) <= bin_tol(prec) * (scale + abs(want)), (prec, u, v)
 
 
-def test_fftn_256_cubed(dev):
+def test_fftn_256_cubed(dev, oracle):
     D, torch = dev
     from fft_amd import _lib as L
     shape = (256, 256, 256)
@@ -173,11 +203,12 @@ def test_fftn_256_cubed(dev):
     back = p.exec(y, inverse=True).cpu().numpy()
     assert rel_l2(back, x.ravel()) <= 1e-6
     Y = y.cpu().numpy().astype(np.complex128).reshape(shape)
-    want = np.fft.fftn(x.astype(np.complex128))
+    want = oracle.fftn(x.astype(np.complex128).ravel(), shape).reshape(shape)
     assert rel_l2(Y, want) <= 1e-6
+    assert rel_l2(want, np.fft.fftn(x.astype(np.complex128))) <= 1e-13  # the oracle itself
 
 
-def test_rfft2_8192(dev):
+def test_rfft2_8192(dev, oracle):
     D, torch = dev
     from fft_amd import _lib as L
     rows = cols = 8192
@@ -190,6 +221,15 @@ def test_rfft2_8192(dev):
     assert rel_l2(back, x.ravel()) <= 1e-6
     Xh = X.cpu().numpy().astype(np.complex128).reshape(rows, cols // 2 + 1)
     xs = x.astype(np.float64)
+    # the whole half spectrum against the oracle's RFFT2 (rfft2.go:22-74), and
+    # the inverse of the oracle's spectrum against its IRFFT2 (rfft2.go:87-138)
+    want = oracle.rfft2(xs.ravel(), (rows, cols))
+    assert rel_l2(Xh, want) <= 1e-6, rel_l2(Xh, want)
+    Wd = torch.from_numpy(want.astype(np.complex64)).to("cuda")
+    got_i = p.irfft2(Wd).cpu().numpy().astype(np.float64)
+    want_i = oracle.irfft2(want.astype(np.complex64).astype(np.complex128), (rows, cols))
+    del want, Wd
+    assert rel_l2(got_i, want_i) <= 1e-6, rel_l2(got_i, want_i)
     scale = math.sqrt(float(np.sum(xs * xs)))
     for u, v in [(0, 0), (4096, 4096), (123, 4000)]:
         want = dft_bin_2d(xs, u, v)

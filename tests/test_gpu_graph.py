@@ -62,3 +62,92 @@ def test_graph_nd(dev):
     p = D.NDPlan((64, 48, 20), dtype=torch.complex128)
     _replays(D, torch, p, 64 * 48 * 20, torch.complex128)
     _replays(D, torch, p, 64 * 48 * 20, torch.complex128, inverse=True)
+
+
+def _replay_op(D, torch, plan, op, make_in, out_shape, out_dtype, bins=0, inverse=False):
+    """Capture op on fixed buffers, replay 4 times with fresh inputs, eager work
+    and synchronisations between replays; each replay equals the eager call."""
+    s = torch.cuda.Stream()
+    x = make_in(0)
+    y = torch.empty(out_shape, dtype=out_dtype, device="cuda")
+    with torch.cuda.stream(s):
+        g = D.Graph(plan, x, y, inverse=inverse, stream=s, op=op, bins=bins)
+    for rep in range(4):
+        fresh = make_in(rep + 1)
+        with torch.cuda.stream(s):
+            x.copy_(fresh)
+            g.launch(stream=s)
+        s.synchronize()
+        got = y.clone()
+        want = torch.empty_like(y)
+        D.run_op(plan, op, fresh.clone(), want, bins=bins, inverse=inverse)  # eager, current stream
+        torch.cuda.synchronize()
+        assert torch.equal(got, want), (op, rep)
+        D.run_op(plan, op, torch.zeros_like(x), torch.empty_like(y), bins=bins, inverse=inverse)
+        torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_graph_real_ops(dev, prec):
+    # graphs of the real plans: RFFT (single pass and multi-pass), IRFFT with
+    # a short spectrum, PSD, Spectrogram, RFFT2/IRFFT2, and the VJPs
+    D, torch = dev
+    from fft_amd import _lib as L
+    rt, ct = (torch.float64, torch.complex128) if prec == "f64" else (torch.float32, torch.complex64)
+
+    def rin(shape, seed):
+        g = torch.Generator().manual_seed(seed)
+        return torch.randn(shape, generator=g, dtype=torch.float64).to("cuda", rt)
+
+    def cin(shape, seed):
+        g = torch.Generator().manual_seed(seed)
+        return torch.randn(shape, generator=g, dtype=torch.complex128).to("cuda", ct)
+
+    for n, batch in ((1024, 64), (1 << 20, 2), (20014, 3)):
+        p = D.R2CPlan(n, batch, rt)
+        nb = n // 2 + 1
+        _replay_op(D, torch, p, L.OP_R2C, lambda k: rin(n * batch, k), batch * nb, ct)
+        _replay_op(D, torch, p, L.OP_C2R, lambda k: cin(batch * 100, k), batch * n, rt, bins=100)
+        _replay_op(D, torch, p, L.OP_VJP_R2C, lambda k: cin(batch * nb, k), batch * n, rt)
+        _replay_op(D, torch, p, L.OP_VJP_C2R, lambda k: rin(batch * n, k), batch * nb, ct)
+    ps = D.PSDPlan(1 << 22, 1, 0.5, rt)
+    _replay_op(D, torch, ps, L.OP_PSD, lambda k: rin(1 << 22, k), (1 << 21) + 1, rt)
+    from oracle import oracle as O
+    sp = D.SpectrogramPlan(1 << 20, 1024, 512, list(O.window("hann", 1024)), 1.0, rt)
+    _replay_op(D, torch, sp, L.OP_SPECTROGRAM, lambda k: rin(1 << 20, k), sp.frames * sp.bins, rt)
+    rows, cols = 512, 1000
+    nd = D.NDPlan((rows, cols), L.ND_R2C, 2, ct)
+    h = rows * (cols // 2 + 1)
+    _replay_op(D, torch, nd, L.OP_ND_R2C, lambda k: rin(2 * rows * cols, k), 2 * h, ct)
+    _replay_op(D, torch, nd, L.OP_ND_R2C, lambda k: cin(2 * 3000, k), 2 * rows * cols, rt, bins=3000,
+               inverse=True)
+    _replay_op(D, torch, nd, L.OP_VJP_ND_R2C, lambda k: cin(2 * h, k), 2 * rows * cols, rt)
+    _replay_op(D, torch, nd, L.OP_VJP_ND_R2C, lambda k: rin(2 * rows * cols, k), 2 * h, ct, inverse=True)
+
+
+def test_graph_replay_overlaps_execs_on_other_streams(dev):
+    # A graph owns its workspace: its replay may overlap execs of the same
+    # mixed-length plan on another stream, and replays of a second graph.
+    D, torch = dev
+    lengths = [1024, 65536, 10007, 20014, 6720] * 4
+    p = D.ManyPlan(lengths, torch.complex64)
+    tot = sum(lengths)
+    s1, s2, s3 = torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Stream()
+    xa = torch.from_numpy(cgauss(tot, seed=1)).to("cuda", torch.complex64)
+    xb = torch.from_numpy(cgauss(tot, seed=2)).to("cuda", torch.complex64)
+    xc = torch.from_numpy(cgauss(tot, seed=3)).to("cuda", torch.complex64)
+    wa, wb, wc = p.exec(xa), p.exec(xb), p.exec(xc)
+    ya, yb, yc = torch.empty_like(xa), torch.empty_like(xb), torch.empty_like(xc)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s1):
+        ga = D.Graph(p, xa, ya, stream=s1)
+    with torch.cuda.stream(s3):
+        gc = D.Graph(p, xc, yc, stream=s3)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        ga.launch(stream=s1)
+        with torch.cuda.stream(s2):
+            p.exec(xb, out=yb, stream=s2)
+        gc.launch(stream=s3)
+        torch.cuda.synchronize()
+        assert torch.equal(ya, wa) and torch.equal(yb, wb) and torch.equal(yc, wc)

@@ -152,14 +152,14 @@ def gaussian_h(rows, cols, sigma):
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_blur_loss_gradient(dev, dtype):
-    # Differentiation workload at 1024 x 1024 (batch 4 here): loss
+    # Differentiation workload at its full size, 64 images of 1024 x 1024: loss
     # ||IRFFT2(RFFT2(x) H) - y||^2 with a Gaussian (sigma 3 px) H; the gradient
     # from the adjoint kernels against 2 A (A x - y) (A symmetric) in FP64 and
     # against central differences of the FP64 loss on 32 pixels.
     D, torch = dev
     from fft_amd import _lib as L
     rows = cols = 1024
-    batch = 4
+    batch = 64
     rc = cols // 2 + 1
     rng = np.random.default_rng(5)
     x = rng.uniform(0, 1, size=(batch, rows, cols))
