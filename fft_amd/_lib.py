@@ -68,6 +68,7 @@ SIGNATURES = {
     "gofft_exec_nd_r2c": (ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.c_int, _vp]),
     "gofft_exec_psd": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "gofft_exec_spectrogram": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "gofft_exec_c2c_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "gofft_vjp_c2c": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "gofft_vjp_r2c": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "gofft_vjp_c2r": (ctypes.c_int, [_vp, _vp, _vp, _vp]),

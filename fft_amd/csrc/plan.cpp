@@ -63,8 +63,16 @@ int fail(int code, const std::string& msg) {
 const char* const kVersionText = "gofft-hip 0.2 (gfx950)";
 const char* const kVersionTag = "0.2";
 
+// Scratch of the no-allocation entry point (gofft_exec_c2c_noalloc): a
+// file-scope device array, one copy per device, held until the process
+// ends. Host code never uses its name as an address: current_device() reads
+// each device's copy's address once, with that device current.
+constexpr size_t kScratchBytes = (size_t)GOFFT_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 struct DevInfo {
   int id = -1;
+  unsigned char* scratch = nullptr;  // this device's copy of `scratch`
   int lds_bytes = 65536;  // dynamic LDS per workgroup beside k_pass's static tables
   int lds_full = 65536;   // dynamic LDS per workgroup of the kernels without static LDS
   int cus = 0;
@@ -100,6 +108,7 @@ int current_device(DevInfo* out) {
     for (char& ch : d.arch)
       if (ch == ' ') ch = '_';
     if (d.arch.empty()) d.arch = "unknown";
+    HIP_TRY(hipGetSymbolAddress((void**)&d.scratch, HIP_SYMBOL(scratch)));
     it = g_devs.emplace(id, d).first;
   }
   *out = it->second;
@@ -3439,6 +3448,25 @@ int gofft_vjp_c2r(const gofft_plan* p, const void* gx, void* gX, void* stream) {
 }
 int gofft_vjp_nd_r2c(const gofft_plan* p, const void* g_out, void* g_in, int direction, void* stream) {
   return exec_op(p, OP_VJP_ND_R2C, g_out, 0, g_in, direction, stream);
+}
+
+// The complex transform with the plan's workspace in the device's static
+// scratch instead of the per-stream pool: no allocation of any kind. A
+// mixed-length plan's side lanes each get their own slice.
+int gofft_exec_c2c_noalloc(const gofft_plan* p, const void* src, void* dst, int direction, void* stream) {
+  RC_TRY(check_op(p, OP_C2C, src, 0, dst, direction));
+  DevInfo dev;
+  RC_TRY(current_device(&dev));
+  const size_t slice = (p->ws_need + 255) / 256 * 256;
+  const size_t sets = p->ws_need ? 1 + p->lanes.size() : 0;
+  if (sets * slice > kScratchBytes)
+    return fail(GOFFT_ERR_UNSUPPORTED, "plan needs " + std::to_string(sets * slice) +
+                                           " bytes of workspace; gofft_exec_c2c_noalloc has " +
+                                           std::to_string(kScratchBytes) + " bytes of scratch per device");
+  WsSet w;
+  w.main = (char*)dev.scratch;
+  for (size_t i = 0; i < p->lanes.size(); i++) w.lanes.push_back((char*)dev.scratch + (i + 1) * slice);
+  return run_op(p, OP_C2C, src, 0, dst, direction, (hipStream_t)stream, &w);
 }
 
 // ---------------------------------------------------------------------------

@@ -197,6 +197,18 @@ int gofft_exec_nd_r2c(const gofft_plan* plan, const void* src, int64_t bins, voi
 int gofft_exec_psd(const gofft_plan* plan, const void* src, void* dst, void* stream);
 int gofft_exec_spectrogram(const gofft_plan* plan, const void* src, void* dst, void* stream);
 
+/* gofft_exec_c2c without any allocation: the plan's workspace comes from a
+ * static device array of GOFFT_NOALLOC_SCRATCH_BYTES per device (held until
+ * the process ends) instead of the per-stream pool, which allocates on a
+ * stream's first use. No counterpart in the reference; for callers that
+ * cannot allocate (code under graph capture, tight serving loops). Plans
+ * whose workspace (times the side lanes of a mixed-length plan) exceeds the
+ * scratch return GOFFT_ERR_UNSUPPORTED. Every call on one device shares that
+ * device's scratch: calls that overlap on one device race. Call it with the
+ * plan's device current. */
+#define GOFFT_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int gofft_exec_c2c_noalloc(const gofft_plan* plan, const void* src, void* dst, int direction, void* stream);
+
 /* --- vector-Jacobian products (no reference counterpart) ------------------ */
 
 /* For y = FFT(x) (complex, unnormalised) and an upstream gradient gy = dL/dconj(y)
