@@ -454,8 +454,28 @@ def workload_blur(D, torch, dev, steps, warmup):
           {"err_vs_fp64": err, "adjoint_err_vs_fp64": err_adj})
 
 
+def workload_groups(D, torch, dev, steps, warmup, lengths=None, precisions=("c64", "c128")):
+    """Each length group of the mixed-length batch alone: a plan of exactly
+    that group's lines (the lengths and inputs of the headline step), one
+    line per group and precision. For focused timing and kernel profiles."""
+    all_lengths = workload_lengths(0)
+    x64 = gaussian_complex(sum(all_lengths), seed=0)
+    for n in sorted(set(all_lengths)) if lengths is None else lengths:
+        count = sum(1 for v in all_lengths if v == n)
+        for prec in precisions:
+            ct, eb = (torch.complex64, 8) if prec == "c64" else (torch.complex128, 16)
+            p = D.C2CPlan(n, count, ct)
+            x = torch.from_numpy(x64[: n * count].astype(np.complex64 if prec == "c64" else np.complex128)).to(dev)
+            y = torch.empty_like(x)
+            ms = time_events(lambda: p.exec(x, out=y), steps, warmup, torch)
+            _line("group", ms, count * 5.0 * n * math.log2(n), 2 * n * count * eb, prec,
+                  "%d transforms of length %d (the mixed batch's group), one plan" % (count, n),
+                  {"plan": p.describe().strip()})
+            del p, x, y
+
+
 WORKLOADS = {"roundtrip4096": workload_roundtrip4096, "spectral": workload_spectral, "nd": workload_nd,
-             "blur": workload_blur}
+             "blur": workload_blur, "groups": workload_groups}
 
 
 def summarize_trace(path, skip=None, take=None):
@@ -523,6 +543,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", choices=["mixed", "all"] + sorted(WORKLOADS), default="mixed",
                     help="mixed: the headline line; the others print one line per transform")
+    ap.add_argument("--lengths", help="groups workload: comma-separated lengths (default: all seven)")
+    ap.add_argument("--precisions", default="c64,c128", help="groups workload: c64 and/or c128")
     ap.add_argument("--summarize-trace", nargs="+", metavar="STATS_CSV [SKIP TAKE]")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     args = ap.parse_args()
@@ -552,6 +574,9 @@ def main():
             fn = WORKLOADS[name]
             if name in ("spectral", "nd"):
                 fn(D, torch, dev, args.steps, warm, cpu=not args.no_cpu)
+            elif name == "groups":
+                lens = [int(v) for v in args.lengths.split(",")] if args.lengths else None
+                fn(D, torch, dev, args.steps, warm, lens, args.precisions.split(","))
             else:
                 fn(D, torch, dev, args.steps, warm)
         return

@@ -242,31 +242,39 @@ template <int RAD, typename T2> __device__ __forceinline__ void dft(T2* v) {
 
 __device__ __forceinline__ int lp(int p, int sh) { return p + (p >> sh); }
 
+// The complex64 convolution's wide form (24 points per thread, a single
+// column of up to 20480 points in the whole LDS) is unpadded: its stages run
+// the odd radices first, so every stage's write stride is odd or at least
+// a wavefront wide, and the reads walk consecutive butterflies. Without
+// pads every LDS address is a base plus a constant step.
+template <int MAXP> __device__ __forceinline__ int lpm(int p, int sh) { return MAXP == 24 ? p : lp(p, sh); }
+
 template <typename T2, int RAD, int MAXP>
 __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage& s, const T2* stw,
                                           int c, int lt, int cdig) {
   constexpr int NB = MAXP / RAD > 0 ? MAXP / RAD : 1;
+  constexpr bool NOPAD = MAXP == 24;
   T2* col = lds + c * f.Rp;
   const int nb = s.nb, ns = s.ns, tpc = f.tpc, sh = f.psh;
   // Uniform per stage: reads j + i*nb and writes o + i*ns are affine when
   // nb (ns) is a multiple of the pad block; the first stage's writes (ns = 1)
-  // when RAD divides the pad block.
-  const bool rd_aff = (nb & ((1 << sh) - 1)) == 0;
-  const bool wr_aff = (ns & ((1 << sh) - 1)) == 0 || (ns == 1 && ((1 << sh) % RAD) == 0);
-  const int rd_step = nb + (nb >> sh);
-  const int wr_step = ns == 1 ? 1 : ns + (ns >> sh);
+  // when RAD divides the pad block; always without pads.
+  const bool rd_aff = NOPAD || (nb & ((1 << sh) - 1)) == 0;
+  const bool wr_aff = NOPAD || (ns & ((1 << sh) - 1)) == 0 || (ns == 1 && ((1 << sh) % RAD) == 0);
+  const int rd_step = NOPAD ? nb : nb + (nb >> sh);
+  const int wr_step = ns == 1 ? 1 : NOPAD ? ns : ns + (ns >> sh);
   T2 v[NB][RAD];
 #pragma unroll
   for (int m = 0; m < NB; ++m) {
     int j = lt + m * tpc;
     if (j < nb) {
       if (rd_aff) {
-        const T2* p = col + lp(j, sh);
+        const T2* p = col + lpm<MAXP>(j, sh);
 #pragma unroll
         for (int i = 0; i < RAD; ++i) v[m][i] = p[i * rd_step];
       } else {
 #pragma unroll
-        for (int i = 0; i < RAD; ++i) v[m][i] = col[lp(j + i * nb, sh)];
+        for (int i = 0; i < RAD; ++i) v[m][i] = col[lpm<MAXP>(j + i * nb, sh)];
       }
       if (ns > 1 || s.cmul) {
         int q = (int)fastdiv_q((uint32_t)j, s.nsd);
@@ -287,12 +295,12 @@ __device__ __forceinline__ void lds_stage(T2* lds, const SubFFT& f, const Stage&
       int k = j - q * ns;
       const int o = q * ns * RAD + k;
       if (wr_aff) {
-        T2* p = col + lp(o, sh);
+        T2* p = col + lpm<MAXP>(o, sh);
 #pragma unroll
         for (int i = 0; i < RAD; ++i) p[i * wr_step] = v[m][i];
       } else {
 #pragma unroll
-        for (int i = 0; i < RAD; ++i) col[lp(o + i * ns, sh)] = v[m][i];
+        for (int i = 0; i < RAD; ++i) col[lpm<MAXP>(o + i * ns, sh)] = v[m][i];
       }
     }
   }
@@ -839,8 +847,12 @@ constexpr int reg2_hk(int mode) { return mode >= 5 ? 1 : 2; }
 // untangle fused into its store (untangle_store above). MODE 12: the first
 // pass of a real inverse's half-length FFT with the pack fused into its load
 // (pack_step above).
-template <typename T, int MAXP, int MODE>
-__global__ void __launch_bounds__(1024) k_pass(PassArgs a) {
+// LB: the largest workgroup the instantiation serves. The complex128 forms
+// that spill under the 128-register cap of a 1024-thread workgroup (0, 3
+// and 4) also have an LB = 256 instantiation for passes of at most 256
+// threads, which holds them in registers (126-131 VGPRs).
+template <typename T, int MAXP, int MODE, int LB = 1024>
+__global__ void __launch_bounds__(LB) k_pass(PassArgs a) {
   constexpr bool EXT = MODE >= 1;
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
@@ -1330,8 +1342,10 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
 
 // ---------------------------------------------------------------------------
 // Fused convolution (Rader / Bluestein), C transforms per workgroup.
-template <typename T, int MAXP>
-__global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
+// NT: the largest workgroup the instantiation serves (complex128 also has
+// NT = 256, which lifts the 128-register cap it spills under at 1024).
+template <typename T, int MAXP, int NT = 1024>
+__global__ void __launch_bounds__(NT) k_conv(ConvArgs a) {
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
   const SubFFT& f = a.f;
@@ -1384,7 +1398,7 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
           T2 v = buf[u];
           if (a.conj_in) v.y = -v.y;
           if (a.kind == 2 && r < n) v = cmul(v, chirp[r]);
-          lds[c * Rp + lp(r, f.psh)] = v;
+          lds[c * Rp + lpm<MAXP>(r, f.psh)] = v;
         }
       }
     }
@@ -1411,7 +1425,7 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
     for (int b = 0; b < SB; ++b) {
       const int r = r0 + b * tpc;
       if (r < R) {
-        T2* q = lds + c * Rp + lp(r, f.psh);
+        T2* q = lds + c * Rp + lpm<MAXP>(r, f.psh);
         *q = cconj(cmul(*q, w[b]));
       }
     }
@@ -1436,7 +1450,7 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
       for (int b = 0; b < SB; ++b) {
         const int k = k0 + b * tpc;
         if (k < n) {
-          T2 v = cmul(cconj(lds[c * Rp + lp(k, f.psh)]), w[b]);
+          T2 v = cmul(cconj(lds[c * Rp + lpm<MAXP>(k, f.psh)]), w[b]);
           if (a.conj_out) v.y = -v.y;
           out[k] = cscale(v, sc);
         }
@@ -1455,7 +1469,7 @@ __global__ void __launch_bounds__(1024) k_conv(ConvArgs a) {
       for (int b = 0; b < SB; ++b) {
         const int k = k0 + b * tpc;
         if (k < q) {
-          T2 v = cadd(x0, cconj(lds[c * Rp + lp(k + off, f.psh)]));
+          T2 v = cadd(x0, cconj(lds[c * Rp + lpm<MAXP>(k + off, f.psh)]));
           if (a.conj_out) v.y = -v.y;
           out[ip[b]] = cscale(v, sc);
         }
@@ -1838,16 +1852,25 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
     }
   }
 #define GOFFT_PASS(T, M) hipLaunchKernelGGL((k_pass<T, 16, M>), dim3(groups), dim3(threads), bytes, s, a)
+#define GOFFT_PASS_LB(T, M, LB) hipLaunchKernelGGL((k_pass<T, 16, M, LB>), dim3(groups), dim3(threads), bytes, s, a)
   if (prec == 0) {
     // complex128 has forms 0, 1, 3, 4, 11, 12 (the others are complex64 only)
     const size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
-    switch (mode) {
-      case 1: GOFFT_PASS(double, 1); break;
-      case 3: GOFFT_PASS(double, 3); break;
-      case 4: GOFFT_PASS(double, 4); break;
-      case 11: GOFFT_PASS(double, 11); break;
-      case 12: GOFFT_PASS(double, 12); break;
-      default: GOFFT_PASS(double, 0); break;
+    if (threads <= 256 && (mode == 0 || mode == 3 || mode == 4)) {
+      switch (mode) {
+        case 3: GOFFT_PASS_LB(double, 3, 256); break;
+        case 4: GOFFT_PASS_LB(double, 4, 256); break;
+        default: GOFFT_PASS_LB(double, 0, 256); break;
+      }
+    } else {
+      switch (mode) {
+        case 1: GOFFT_PASS(double, 1); break;
+        case 3: GOFFT_PASS(double, 3); break;
+        case 4: GOFFT_PASS(double, 4); break;
+        case 11: GOFFT_PASS(double, 11); break;
+        case 12: GOFFT_PASS(double, 12); break;
+        default: GOFFT_PASS(double, 0); break;
+      }
     }
   } else {
     const size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
@@ -1865,6 +1888,7 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
     }
   }
 #undef GOFFT_PASS
+#undef GOFFT_PASS_LB
   return hipGetLastError();
 }
 
@@ -1912,7 +1936,10 @@ hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipSt
   long long groups = (a.nlines + a.f.C - 1) / a.f.C;
   if (prec == 0) {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
-    GOFFT_LAUNCH16(k_conv, double, a, bytes, threads, groups, s);
+    if (threads <= 256)
+      hipLaunchKernelGGL((k_conv<double, 16, 256>), dim3(groups), dim3(threads), bytes, s, a);
+    else
+      GOFFT_LAUNCH16(k_conv, double, a, bytes, threads, groups, s);
   } else {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
     if (maxp == 24)

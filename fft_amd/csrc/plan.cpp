@@ -398,6 +398,7 @@ std::vector<int> composite_schedule(int R) {
 }
 
 constexpr int kWideP = 24;  // the complex64 fused convolution's wide form
+constexpr int kNoPad = 16;  // pad shift of an unpadded column (every point < 2^16)
 
 // LDS column pitch for R points (kernels.hip: one pad slot per 16 points plus
 // one, odd so adjacent columns fall in different banks).
@@ -417,7 +418,7 @@ struct SubPlan {
 std::string describe_sub(i64 R, const SubPlan& sp) {
   std::string s = std::to_string(R) + "[C" + std::to_string(sp.f.C) + " T" + std::to_string(sp.threads) +
                   (sp.maxp != kMaxP ? " P" + std::to_string(sp.maxp) : "") +
-                  (sp.f.psh != 4 ? " pad" + std::to_string(1 << sp.f.psh) : "") + ":";
+                  (sp.f.psh == kNoPad ? " unpadded" : sp.f.psh != 4 ? " pad" + std::to_string(1 << sp.f.psh) : "") + ":";
   for (int i = 0; i < sp.f.nstages; i++) {
     const int r = sp.f.st[i].radix;
     s += (i ? "," : "") + (r > 100 ? "r2^" + std::to_string(ilog2c(r - 100)) : std::to_string(r));
@@ -740,9 +741,13 @@ bool fused_conv_plan(i64 m, int prec, i64 pts, SubPlan& sp, int colcap = 16) {
   if (prec != GOFFT_C64 || m > 24 * 1024) return false;
   for (int sh : {4, 5, 6})
     if (pitch(m, sh) <= pts)
-      for (int maxp : {kMaxP, kWideP})
-        for (int maxrad : {16, 8})
-          if (make_sub((int)m, 1, sp, 0, sh, maxp, maxrad)) return true;
+      for (int maxrad : {16, 8})
+        if (make_sub((int)m, 1, sp, 0, sh, kMaxP, maxrad)) return true;
+  // the wide form, unpadded (kernels.hip lpm): psh = kNoPad makes r >> psh
+  // zero for every point, so the pitch is m + 1
+  if (pitch(m, kNoPad) <= pts)
+    for (int maxrad : {16, 8})
+      if (make_sub((int)m, 1, sp, 0, kNoPad, kWideP, maxrad)) return true;
   return false;
 }
 
@@ -1675,15 +1680,10 @@ class RaderEngine : public Engine {
     if (lines == 0) return GOFFT_OK;
     if (fused) {
       ConvArgs a{};
-      a.in = in;
-      a.out = out;
       a.stw = stw.p;
       a.spec = spec.p;
       a.perm = (const int*)perm.p;
       a.iperm = (const int*)iperm.p;
-      a.in_off = li.off;
-      a.out_off = lo.off;
-      a.nlines = lines;
       a.in_stride = li.stride;
       a.out_stride = lo.stride;
       a.n = (int)n;
@@ -1692,6 +1692,11 @@ class RaderEngine : public Engine {
       a.conj_out = co;
       a.scale = scale;
       a.f = sp.f;
+      a.in = in;
+      a.out = out;
+      a.in_off = li.off;
+      a.out_off = lo.off;
+      a.nlines = lines;
       HIP_TRY(launch_conv(prec, sp.maxp, a, sp.threads, c.s));
       return GOFFT_OK;
     }
@@ -1836,14 +1841,9 @@ class BluesteinEngine : public Engine {
     if (ref_inverse) ci = co = 0;
     if (fused) {
       ConvArgs a{};
-      a.in = in;
-      a.out = out;
       a.stw = stw.p;
       a.spec = b;
       a.chirp = w;
-      a.in_off = li.off;
-      a.out_off = lo.off;
-      a.nlines = lines;
       a.in_stride = li.stride;
       a.out_stride = lo.stride;
       a.n = (int)n;
@@ -1852,6 +1852,11 @@ class BluesteinEngine : public Engine {
       a.conj_out = co;
       a.scale = scale;
       a.f = sp.f;
+      a.in = in;
+      a.out = out;
+      a.in_off = li.off;
+      a.out_off = lo.off;
+      a.nlines = lines;
       HIP_TRY(launch_conv(prec, sp.maxp, a, sp.threads, c.s));
       return GOFFT_OK;
     }
