@@ -474,8 +474,63 @@ def workload_groups(D, torch, dev, steps, warmup, lengths=None, precisions=("c64
             del p, x, y
 
 
+def workload_twiddles(D, torch, dev, steps, warmup):
+    """The complex64 pass-twiddle forms side by side: FP64-formed (the
+    default), plain FP32 tables and products, and hi + lo float pairs in
+    double-float arithmetic, on the transforms whose passes carry twiddles
+    (FFT2 4096^2 has none: both of its axes are single 4096-point passes).
+    Error against the FP64 GPU path on the same input."""
+    L = D.L
+    modes = (("fp64_formed", 0), ("fp32", L.FLAG_TWIDDLES_FP32), ("split", L.FLAG_TWIDDLES_SPLIT))
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    cases = []
+    x = torch.randn(16 << 20, dtype=torch.complex128, device=dev, generator=g)
+    cases.append(("fft 2^20 x 16", lambda f: D.C2CPlan(1 << 20, 16, torch.complex64, f),
+                  lambda p, v: p.exec(v), x, 16 * 5.0 * (1 << 20) * 20, 2 * (16 << 20) * 8))
+    cases.append(("fft2 4096x4096", lambda f: D.NDPlan((4096, 4096), L.ND_C2C, 1, torch.complex64, f),
+                  lambda p, v: p.exec(v), x, 5.0 * (1 << 24) * 24, 2 * (1 << 24) * 8))
+    r = torch.rand(1 << 26, dtype=torch.float64, device=dev, generator=g)
+    cases.append(("rfft2 8192x8192 U[0,1)", lambda f: D.NDPlan((8192, 8192), L.ND_R2C, 1, torch.complex64, f),
+                  lambda p, v: p.rfft2(v), r, 2.5 * (1 << 26) * 26, (1 << 26) * 4 + 8192 * 4097 * 8))
+    sig = _sinusoids(torch, dev, 1 << 26, torch.float64)
+    cases.append(("rfft 2^24 sinusoids", lambda f: D.R2CPlan(1 << 24, 1, torch.float32, f),
+                  lambda p, v: p.forward(v), sig[: 1 << 24].contiguous(), 2.5 * (1 << 24) * 24,
+                  (1 << 24) * 4 + ((1 << 23) + 1) * 8))
+    cases.append(("psd 2^26 sinusoids", lambda f: D.PSDPlan(1 << 26, 1, 1.0, torch.float32, f),
+                  lambda p, v: p.exec(v), sig, 2.5 * (1 << 26) * 26, (1 << 26) * 4 + ((1 << 25) + 1) * 4))
+    for name, mk, run, x64, flops, nbytes in cases:
+        xin = x64.to(torch.complex64 if x64.is_complex() else torch.float32)
+        if name.startswith("fft 2^20"):
+            xin = xin[: 16 << 20].contiguous()
+        elif name.startswith("fft2"):
+            xin = xin[: 1 << 24].contiguous()
+        # the FP64 reference: the same transform in complex128 / float64 on
+        # the rounded input, so that the error is the FP32 arithmetic's
+        xw = xin.to(torch.complex128 if xin.is_complex() else torch.float64)
+        if name.startswith("fft 2^20"):
+            ref = D.C2CPlan(1 << 20, 16, torch.complex128).exec(xw)
+        elif name.startswith("fft2"):
+            ref = D.NDPlan((4096, 4096), L.ND_C2C, 1, torch.complex128).exec(xw)
+        elif name.startswith("rfft2"):
+            ref = D.NDPlan((8192, 8192), L.ND_R2C, 1, torch.complex128).rfft2(xw)
+        elif name.startswith("rfft"):
+            ref = D.R2CPlan(1 << 24, 1, torch.float64).forward(xw)
+        else:
+            ref = D.PSDPlan(1 << 26, 1, 1.0, torch.float64).exec(xw)
+        del xw
+        for mname, flag in modes:
+            p = mk(flag)
+            y = run(p, xin)
+            ms = time_events(lambda: run(p, xin), steps, warmup, torch)
+            _line("twiddles", ms, flops, nbytes, "c64", name, {"twiddle_form": mname, "err_vs_fp64": _err(y, ref)})
+            del p, y
+        del ref, xin
+        torch.cuda.empty_cache()
+
+
 WORKLOADS = {"roundtrip4096": workload_roundtrip4096, "spectral": workload_spectral, "nd": workload_nd,
-             "blur": workload_blur, "groups": workload_groups}
+             "blur": workload_blur, "groups": workload_groups, "twiddles": workload_twiddles}
 
 
 def summarize_trace(path, skip=None, take=None):

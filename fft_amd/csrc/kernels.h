@@ -95,6 +95,11 @@ struct PassArgs {
   const void* stw;          // stage twiddles, precision of the data
   const double2* ptw_hi;    // pass twiddle W_M^e = hi[e >> s] * lo[e & (2^s - 1)]
   const double2* ptw_lo;
+  // tw_mode 1 (FP32) and 2 (hi + lo float) of complex64 plain and real
+  // first/last passes: the same two-level tables as float2 / float4 entries
+  int tw_mode;
+  const void* ptwx_hi;
+  const void* ptwx_lo;
   long long ncols;
   long long tw_M;           // 0: no pass twiddle
   int tw_shift;

@@ -440,6 +440,88 @@ __device__ __forceinline__ double2 dmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Pass twiddles W_M^(r k) of a thread's points r = r0 + u * step (u = 0, 1,
+// ...) in the column whose twiddle digit is k. TWM selects how they are
+// formed (PassArgs::tw_mode):
+//  0: FP64 two-level tables, FP64 product and running product, each twiddle
+//     rounded once to the data precision (every complex128 pass, and the
+//     complex64 default);
+//  1: FP32 two-level tables, FP32 product and FP32 running product;
+//  2: two-level tables of hi + lo float pairs, each twiddle formed directly
+//     in double-float arithmetic (error-free products by FMA), rounded once.
+template <typename T2, int TWM> struct PassTw;
+
+template <typename T2> struct PassTw<T2, 0> {
+  double2 w = make_double2(1.0, 0.0), st = make_double2(1.0, 0.0);
+  __device__ __forceinline__ void init(const PassArgs& a, long long k, int r0, int step) {
+    if (k) {
+      w = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)r0 * k, a.tw_shift);
+      st = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)step * k, a.tw_shift);
+    }
+  }
+  __device__ __forceinline__ T2 next() {
+    const T2 r = from_d2<T2>(w);
+    w = dmul2(w, st);
+    return r;
+  }
+};
+
+template <> struct PassTw<float2, 1> {
+  float2 w = make_float2(1.f, 0.f), st = make_float2(1.f, 0.f);
+  __device__ __forceinline__ static float2 lookup(const PassArgs& a, long long e) {
+    const float2* hi = reinterpret_cast<const float2*>(a.ptwx_hi);
+    const float2* lo = reinterpret_cast<const float2*>(a.ptwx_lo);
+    return cmul(hi[e >> a.tw_shift], lo[e & ((1ll << a.tw_shift) - 1)]);
+  }
+  __device__ __forceinline__ void init(const PassArgs& a, long long k, int r0, int step) {
+    if (k) {
+      w = lookup(a, (long long)r0 * k);
+      st = lookup(a, (long long)step * k);
+    }
+  }
+  __device__ __forceinline__ float2 next() {
+    const float2 r = w;
+    w = cmul(w, st);
+    return r;
+  }
+};
+
+#pragma clang fp contract(off)
+// (A_hi + A_lo)(B_hi + B_lo) rounded to float2: the products of the high
+// parts exactly (FMA residuals), their sums with their rounding errors, plus
+// the cross terms; error about 2^-44 relative before the final rounding.
+__device__ __forceinline__ float2 ddf_cmul(float4 A, float4 B) {
+  const float p1 = A.x * B.x, e1 = __fmaf_rn(A.x, B.x, -p1);
+  const float p2 = A.y * B.y, e2 = __fmaf_rn(A.y, B.y, -p2);
+  const float sr = p1 - p2, br = sr - p1, er = (p1 - (sr - br)) + (-p2 - br);
+  const float cr = ((e1 - e2) + er) + (A.x * B.z + A.z * B.x - A.y * B.w - A.w * B.y);
+  const float p3 = A.x * B.y, e3 = __fmaf_rn(A.x, B.y, -p3);
+  const float p4 = A.y * B.x, e4 = __fmaf_rn(A.y, B.x, -p4);
+  const float si = p3 + p4, bi = si - p3, ei = (p3 - (si - bi)) + (p4 - bi);
+  const float ci = ((e3 + e4) + ei) + (A.x * B.w + A.z * B.y + A.y * B.z + A.w * B.x);
+  return make_float2(sr + cr, si + ci);
+}
+#pragma clang fp contract(on)
+
+template <> struct PassTw<float2, 2> {
+  long long e = 0, de = 0;
+  const float4* hi = nullptr;
+  const float4* lo = nullptr;
+  int sh = 0;
+  __device__ __forceinline__ void init(const PassArgs& a, long long k, int r0, int step) {
+    e = (long long)r0 * k;
+    de = (long long)step * k;
+    hi = reinterpret_cast<const float4*>(a.ptwx_hi);
+    lo = reinterpret_cast<const float4*>(a.ptwx_lo);
+    sh = a.tw_shift;
+  }
+  __device__ __forceinline__ float2 next() {
+    const float2 r = ddf_cmul(hi[e >> sh], lo[e & ((1ll << sh) - 1)]);
+    e += de;
+    return r;
+  }
+};
+
 extern __shared__ unsigned char g_dyn_lds[];
 
 // ---------------------------------------------------------------------------
@@ -851,7 +933,7 @@ constexpr int reg2_hk(int mode) { return mode >= 5 ? 1 : 2; }
 // that spill under the 128-register cap of a 1024-thread workgroup (0, 3
 // and 4) also have an LB = 256 instantiation for passes of at most 256
 // threads, which holds them in registers (126-131 VGPRs).
-template <typename T, int MAXP, int MODE, int LB = 1024>
+template <typename T, int MAXP, int MODE, int LB = 1024, int TWM = 0>
 __global__ void __launch_bounds__(LB) k_pass(PassArgs a) {
   constexpr bool EXT = MODE >= 1;
   typedef typename Vec2<T>::type T2;
@@ -993,16 +1075,10 @@ __global__ void __launch_bounds__(LB) k_pass(PassArgs a) {
     }
   }
   // Pass twiddles W_M^(r*k) for this thread's column k digit and points
-  // r = ir0 + u * tpc: one base and one step lookup, then an FP64 running
-  // product.
-  double2 tw_w = make_double2(1.0, 0.0), tw_step = make_double2(1.0, 0.0);
-  if (a.tw_M) {
-    const long long k = s_tk[ic];
-    if (k) {
-      tw_w = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)ir0 * k, a.tw_shift);
-      tw_step = pass_twiddle(a.ptw_hi, a.ptw_lo, (long long)frs * k, a.tw_shift);
-    }
-  }
+  // r = ir0 + u * tpc (PassTw: by default one base and one step lookup, then
+  // an FP64 running product).
+  PassTw<T2, TWM> ptw;
+  if (a.tw_M) ptw.init(a, s_tk[ic], ir0, frs);
   // The LDS slots of a column's points r0 + u * tpc are affine in u when tpc
   // is a whole number of pad blocks.
   const bool lds_aff = (frs & ((1 << f.psh) - 1)) == 0;
@@ -1021,10 +1097,7 @@ __global__ void __launch_bounds__(LB) k_pass(PassArgs a) {
     const int r = ir0 + u * frs;
     if (r < R) {
       T2 v = buf[u];
-      if (a.tw_M) {
-        v = cmul(v, from_d2<T2>(tw_w));
-        tw_w = dmul2(tw_w, tw_step);
-      }
+      if (a.tw_M) v = cmul(v, ptw.next());
       if (a.conj_in) v.y = -v.y;
       if (in_hook == 1 && s_ib[ic] >= 0) {
         const long long i = s_il[ic] + (long long)r * a.im.g;
@@ -1872,6 +1945,21 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
         default: GOFFT_PASS(double, 0); break;
       }
     }
+  } else if (a.tw_M && a.tw_mode != 0 && (mode == 0 || mode == 11 || mode == 12)) {
+    // the complex64 pass-twiddle variants (PassTw), for the plain passes and
+    // the real transforms' fused first and last passes
+    const size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
+#define GOFFT_PASS_TW(M, TW) hipLaunchKernelGGL((k_pass<float, 16, M, 1024, TW>), dim3(groups), dim3(threads), bytes, s, a)
+    if (a.tw_mode == 1) {
+      if (mode == 11) GOFFT_PASS_TW(11, 1);
+      else if (mode == 12) GOFFT_PASS_TW(12, 1);
+      else GOFFT_PASS_TW(0, 1);
+    } else {
+      if (mode == 11) GOFFT_PASS_TW(11, 2);
+      else if (mode == 12) GOFFT_PASS_TW(12, 2);
+      else GOFFT_PASS_TW(0, 2);
+    }
+#undef GOFFT_PASS_TW
   } else {
     const size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
     switch (mode) {

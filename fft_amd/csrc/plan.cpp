@@ -934,6 +934,10 @@ class StockhamEngine : public Engine {
   i64 inner = 1;     // lines interleaved at distance 1 (the axis stride)
   std::vector<Pass> passes;
   DevBuf stw, ptw;
+  // complex64 pass-twiddle form (kernels.hip PassTw): 0 FP64-formed, 1 FP32,
+  // 2 hi + lo float pairs; ptwx holds the tables of forms 1 and 2
+  int tw_mode = 0;
+  DevBuf ptwx;
   int P() const { return (int)passes.size(); }
 
   // Power-of-two transform with the reference's recurrence twiddles
@@ -1010,7 +1014,7 @@ class StockhamEngine : public Engine {
   }
 
   int init(unsigned flags) {
-    (void)flags;
+    if (prec == GOFFT_C64) tw_mode = (flags & GOFFT_FLAG_TWIDDLES_SPLIT) ? 2 : (flags & GOFFT_FLAG_TWIDDLES_FP32) ? 1 : 0;
     DevInfo dev;
     RC_TRY(current_device(&dev));
     const i64 pts = dev.lds_bytes / (i64)elt(prec);
@@ -1150,6 +1154,20 @@ class StockhamEngine : public Engine {
     if (stw_h.empty()) stw_h.push_back(cld(1, 0));
     RC_TRY(upload_complex(stw, stw_h, prec));
     if (!ptw_h.empty()) RC_TRY(upload_complex(ptw, ptw_h, 0));
+    if (tw_mode == 1 && !ptw_h.empty()) RC_TRY(upload_complex(ptwx, ptw_h, GOFFT_C64));
+    if (tw_mode == 2 && !ptw_h.empty()) {
+      // each entry as (hi.re, hi.im, lo.re, lo.im): hi = fl(W), lo = fl(W - hi)
+      std::vector<float> h(4 * ptw_h.size());
+      for (size_t i = 0; i < ptw_h.size(); i++) {
+        const long double re = ptw_h[i].real(), im = ptw_h[i].imag();
+        const float hr = (float)re, hi = (float)im;
+        h[4 * i] = hr;
+        h[4 * i + 1] = hi;
+        h[4 * i + 2] = (float)(re - (long double)hr);
+        h[4 * i + 3] = (float)(im - (long double)hi);
+      }
+      RC_TRY(ptwx.upload(h.data(), h.size() * sizeof(float)));
+    }
     return GOFFT_OK;
   }
 
@@ -1220,6 +1238,13 @@ class StockhamEngine : public Engine {
       a.stw = (const char*)stw.p + p.stw_off * elt(prec);
       a.ptw_hi = p.tw_M ? (const double2*)ptw.p + p.ptw_hi : nullptr;
       a.ptw_lo = p.tw_M ? (const double2*)ptw.p + p.ptw_lo : nullptr;
+      if (p.tw_M && tw_mode) {
+        // same layout, entries of 8 (FP32) or 16 (hi + lo) bytes
+        const size_t xb = tw_mode == 1 ? 8 : 16;
+        a.tw_mode = tw_mode;
+        a.ptwx_hi = (const char*)ptwx.p + p.ptw_hi * xb;
+        a.ptwx_lo = (const char*)ptwx.p + p.ptw_lo * xb;
+      }
       a.ncols = lines * inner * (n / p.R);
       a.tw_M = p.tw_M;
       a.tw_shift = p.tw_shift;

@@ -75,6 +75,16 @@ extern "C" {
  * precision), and in the file named by GOFFT_TUNE_CACHE if set. The env var
  * GOFFT_AUTOTUNE=1 sets this flag for every plan. No reference counterpart. */
 #define GOFFT_FLAG_AUTOTUNE 4u
+/* complex64 only: how the pass twiddles of multi-pass transforms are formed
+ * (no reference counterpart; the reference is FP64, plan.go:319-327). By
+ * default they come from FP64 two-level tables built from exact integer
+ * phases, multiplied and stepped in FP64 and rounded once to FP32.
+ * TWIDDLES_FP32 uses FP32 tables and FP32 products; TWIDDLES_SPLIT uses
+ * tables of hi + lo float pairs and forms each twiddle in double-float
+ * arithmetic. They apply to the plain passes and the real transforms' fused
+ * first and last passes; convolution chains keep the default. */
+#define GOFFT_FLAG_TWIDDLES_FP32 8u
+#define GOFFT_FLAG_TWIDDLES_SPLIT 16u
 
 /* N-D kinds */
 #define GOFFT_ND_C2C 0
