@@ -75,6 +75,7 @@ SIGNATURES = {
     "gofft_vjp_r2c": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "gofft_vjp_c2r": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "gofft_vjp_nd_r2c": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
+    "gofft_filter2_loss_grad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "gofft_exec_c2c_host": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "gofft_exec_r2c_host": (ctypes.c_int, [_vp, _vp, _vp]),
     "gofft_exec_c2r_host": (ctypes.c_int, [_vp, _vp, _i64, _vp]),

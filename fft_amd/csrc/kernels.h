@@ -281,6 +281,16 @@ hipError_t launch_frames(int prec, const void* x, const void* w, void* out, long
 // outer*n*st + inner + i*st. dir 0 gathers to contiguous, 1 scatters back.
 hipError_t launch_axis_copy(int prec, const void* in, void* out, long long n, long long st,
                             long long lines, int dir, hipStream_t s);
+// Spectral filter loss: S[b*half + i] *= H[i] (conj(H[i]) when conj_h), b < nimg.
+hipError_t launch_filter_mul(int prec, void* S, const void* H, long long half, long long nimg, int conj_h,
+                             hipStream_t s);
+// Partial sums per image that launch_residual_loss writes (its workspace
+// holds nimg * residual_parts(per) reals).
+long long residual_parts(long long per);
+// r = r - y (nimg images of `per` reals), loss[b] = sum of (r - y)^2 over
+// image b, then r = 2 (r - y); partial is workspace (see residual_parts).
+hipError_t launch_residual_loss(int prec, void* r, const void* y, long long per, long long nimg, void* partial,
+                                void* loss, hipStream_t s);
 // Fill n real elements of T with a value (graph-capturable, unlike a memset).
 hipError_t launch_fill(int prec, void* out, long long n, double value, hipStream_t s);
 // Scale a real or complex array in place or out of place (n elements of T).

@@ -1877,6 +1877,55 @@ __global__ void k_scale(const T* in, T* out, long long n, double scale) {
     out[i] = in[i] * (T)scale;
 }
 
+// Spectral filter loss helpers (gofft_filter2_loss_grad).
+// S[b*half + i] *= H[i] (or conj(H[i])), b < nimg.
+template <typename T>
+__global__ void k_filter_mul(void* S_, const void* H_, long long half, long long nimg, int conj_h) {
+  typedef typename Vec2<T>::type T2;
+  T2* S = reinterpret_cast<T2*>(S_);
+  const T2* H = reinterpret_cast<const T2*>(H_);
+  const long long total = half * nimg;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    T2 h = H[i % half];
+    if (conj_h) h.y = -h.y;
+    S[i] = cmul(S[i], h);
+  }
+}
+
+// r = r - y over nimg images of `per` reals; block (x, b) also writes the sum
+// of its r^2 to partial[b * gridDim.x + x] and stores 2 r (the loss's
+// gradient with respect to r). Block sums are tree reductions in a fixed
+// order, so the loss is the same for any number of images per call.
+template <typename T>
+__global__ void __launch_bounds__(256) k_residual(T* r, const T* y, long long per, T* partial) {
+  __shared__ T red[256];
+  const long long b = blockIdx.y;
+  T acc = 0;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < per; i += (long long)gridDim.x * 256) {
+    const T v = r[b * per + i] - y[b * per + i];
+    acc += v * v;
+    r[b * per + i] = v + v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[b * gridDim.x + blockIdx.x] = red[0];
+}
+
+// loss[b] = sum of partial[b * nparts .. + nparts) in order.
+template <typename T>
+__global__ void k_sum_parts(const T* partial, long long nparts, long long nimg, T* loss) {
+  const long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (b >= nimg) return;
+  T acc = 0;
+  for (long long k = 0; k < nparts; k++) acc += partial[b * nparts + k];
+  loss[b] = acc;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers.
 
@@ -2198,6 +2247,40 @@ hipError_t launch_frames(int prec, const void* x, const void* w, void* out, long
 hipError_t launch_axis_copy(int prec, const void* in, void* out, long long n, long long st,
                             long long lines, int dir, hipStream_t s) {
   GOFFT_LAUNCH_EW(k_axis_copy, n * lines, in, out, n, st, lines, dir);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_mul(int prec, void* S, const void* H, long long half, long long nimg, int conj_h,
+                             hipStream_t s) {
+  unsigned g = grid_for(half * nimg, 256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_filter_mul<double>, dim3(g), dim3(256), 0, s, S, H, half, nimg, conj_h);
+  else
+    hipLaunchKernelGGL(k_filter_mul<float>, dim3(g), dim3(256), 0, s, S, H, half, nimg, conj_h);
+  return hipGetLastError();
+}
+
+long long residual_parts(long long per) {
+  long long p = (per + 256 * 8 - 1) / (256 * 8);
+  return p < 1 ? 1 : p > 1024 ? 1024 : p;
+}
+
+hipError_t launch_residual_loss(int prec, void* r, const void* y, long long per, long long nimg, void* partial,
+                                void* loss, hipStream_t s) {
+  if (nimg <= 0) return hipSuccess;
+  const long long parts = residual_parts(per);
+  const unsigned lg = (unsigned)((nimg + 255) / 256);
+  if (prec == 0) {
+    hipLaunchKernelGGL(k_residual<double>, dim3((unsigned)parts, (unsigned)nimg), dim3(256), 0, s, (double*)r,
+                       (const double*)y, per, (double*)partial);
+    hipLaunchKernelGGL(k_sum_parts<double>, dim3(lg), dim3(256), 0, s, (const double*)partial, parts, nimg,
+                       (double*)loss);
+  } else {
+    hipLaunchKernelGGL(k_residual<float>, dim3((unsigned)parts, (unsigned)nimg), dim3(256), 0, s, (float*)r,
+                       (const float*)y, per, (float*)partial);
+    hipLaunchKernelGGL(k_sum_parts<float>, dim3(lg), dim3(256), 0, s, (const float*)partial, parts, nimg,
+                       (float*)loss);
+  }
   return hipGetLastError();
 }
 

@@ -2265,9 +2265,11 @@ class RealEngine {
 
   // mode 0: complex bins (stride ostride per line), mode 1: PSD with psd_scale,
   // mode 2: complex bins times w_edge (DC, Nyquist) or w_mid (the others).
+  // (the first nl lines when nl >= 0, every line otherwise)
   int forward(const void* in, i64 istride, void* out, i64 ostride, int mode, double psd_scale,
-              const void* window, const Ctx& c, double w_edge = 1.0, double w_mid = 1.0) const {
-    if (batch == 0 || n == 0) return GOFFT_OK;
+              const void* window, const Ctx& c, double w_edge = 1.0, double w_mid = 1.0, i64 nl = -1) const {
+    const i64 lines = nl < 0 ? batch : nl;
+    if (lines == 0 || n == 0) return GOFFT_OK;
     if (single) {
       RealFwdArgs a{};
       a.in = in;
@@ -2275,7 +2277,7 @@ class RealEngine {
       a.stw = stw.p;
       a.utw = utw.p;
       a.window = window;
-      a.nlines = batch;
+      a.nlines = lines;
       a.in_stride = istride;
       a.out_stride = ostride;
       a.mode = mode;
@@ -2301,38 +2303,39 @@ class RealEngine {
         rl.psd = psd_scale;
         rl.w_edge = w_edge;
         rl.w_mid = w_mid;
-        return static_cast<const StockhamEngine*>(half.get())->run_real(in, istride / 2, out, rl, ic, batch);
+        return static_cast<const StockhamEngine*>(half.get())->run_real(in, istride / 2, out, rl, ic, lines);
       }
-      RC_TRY(half->run(in, LineIO{nullptr, istride / 2}, Z, LineIO{nullptr, m}, 0, 0, 1.0, ic));
-      HIP_TRY(launch_untangle(prec, Z, out, utw.p, m, batch, m, ostride, mode, psd_scale, w_edge, w_mid, c.s));
+      RC_TRY(half->run(in, LineIO{nullptr, istride / 2}, Z, LineIO{nullptr, m}, 0, 0, 1.0, ic, lines));
+      HIP_TRY(launch_untangle(prec, Z, out, utw.p, m, lines, m, ostride, mode, psd_scale, w_edge, w_mid, c.s));
       return GOFFT_OK;
     }
     char* A = c.ws;
     char* B = c.ws + (size_t)n * batch * elt(prec);
     Ctx ic{c.s, B + (size_t)n * batch * elt(prec)};
-    HIP_TRY(launch_promote(prec, in, A, n, batch, istride, n, c.s));
-    RC_TRY(half->run(A, LineIO{nullptr, n}, B, LineIO{nullptr, n}, 0, 0, 1.0, ic));
+    HIP_TRY(launch_promote(prec, in, A, n, lines, istride, n, c.s));
+    RC_TRY(half->run(A, LineIO{nullptr, n}, B, LineIO{nullptr, n}, 0, 0, 1.0, ic, lines));
     if (mode == 0)
-      HIP_TRY(launch_copy_lines(prec, B, out, n / 2 + 1, batch, n, ostride, c.s));
+      HIP_TRY(launch_copy_lines(prec, B, out, n / 2 + 1, lines, n, ostride, c.s));
     else if (mode == 2)
-      HIP_TRY(launch_weight_bins(prec, B, out, n / 2 + 1, n, batch, n, ostride, w_edge, w_mid, c.s));
+      HIP_TRY(launch_weight_bins(prec, B, out, n / 2 + 1, n, lines, n, ostride, w_edge, w_mid, c.s));
     else
-      HIP_TRY(launch_power(prec, B, out, n / 2 + 1, n, batch, n, ostride, psd_scale, c.s));
+      HIP_TRY(launch_power(prec, B, out, n / 2 + 1, n, lines, n, ostride, psd_scale, c.s));
     return GOFFT_OK;
   }
 
   // IRFFT: `bins` bins per input line (stride istride), n reals per output
   // line; the output times gain, the bins other than DC and Nyquist times w_mid.
   int inverse(const void* in, i64 istride, i64 bins, void* out, i64 ostride, const Ctx& c, double gain = 1.0,
-              double w_mid = 1.0) const {
-    if (batch == 0 || n <= 0) return GOFFT_OK;
+              double w_mid = 1.0, i64 nl = -1) const {
+    const i64 lines = nl < 0 ? batch : nl;
+    if (lines == 0 || n <= 0) return GOFFT_OK;
     if (single) {
       RealInvArgs a{};
       a.in = in;
       a.out = out;
       a.stw = stw.p;
       a.utw = utw.p;
-      a.nlines = batch;
+      a.nlines = lines;
       a.in_stride = istride;
       a.in_bins = bins;
       a.out_stride = ostride;
@@ -2353,18 +2356,18 @@ class RealEngine {
         rf.bins = std::max<i64>(0, std::min<i64>(bins, m + 1));
         rf.w_mid = w_mid;
         return static_cast<const StockhamEngine*>(half.get())
-            ->run_real_inv(in, istride, out, ostride / 2, gain / (double)m, rf, ic, batch);
+            ->run_real_inv(in, istride, out, ostride / 2, gain / (double)m, rf, ic, lines);
       }
-      HIP_TRY(launch_herm_pack(prec, in, Z, utw.p, m, batch, istride, bins, m, w_mid, c.s));
-      RC_TRY(half->run(Z, LineIO{nullptr, m}, out, LineIO{nullptr, ostride / 2}, 1, 1, gain / (double)m, ic));
+      HIP_TRY(launch_herm_pack(prec, in, Z, utw.p, m, lines, istride, bins, m, w_mid, c.s));
+      RC_TRY(half->run(Z, LineIO{nullptr, m}, out, LineIO{nullptr, ostride / 2}, 1, 1, gain / (double)m, ic, lines));
       return GOFFT_OK;
     }
     char* A = c.ws;
     char* B = c.ws + (size_t)n * batch * elt(prec);
     Ctx ic{c.s, B + (size_t)n * batch * elt(prec)};
-    HIP_TRY(launch_herm_full(prec, in, A, n, batch, istride, bins, n, w_mid, c.s));
-    RC_TRY(half->run(A, LineIO{nullptr, n}, B, LineIO{nullptr, n}, 1, 1, gain / (double)n, ic));
-    HIP_TRY(launch_take_real(prec, B, out, n, batch, n, ostride, 1.0, c.s));
+    HIP_TRY(launch_herm_full(prec, in, A, n, lines, istride, bins, n, w_mid, c.s));
+    RC_TRY(half->run(A, LineIO{nullptr, n}, B, LineIO{nullptr, n}, 1, 1, gain / (double)n, ic, lines));
+    HIP_TRY(launch_take_real(prec, B, out, n, lines, n, ostride, 1.0, c.s));
     return GOFFT_OK;
   }
 };
@@ -2499,7 +2502,9 @@ struct gofft_plan {
   DevBuf window;
   bool spec_gather = false;
   size_t ws_need = 0;
+  size_t ws_engines = 0;  // ND_R2C: the part of ws_need the row and column engines use
   mutable WorkspacePool pool;
+  mutable WorkspacePool loss_pool;  // gofft_filter2_loss_grad: per-stream chunk buffers
   mutable HostPool host;  // sessions of the host-staged calls
   // Mixed-length plans: streams (default priority) on which length groups run
   // beside the caller's stream, joined back by events. lane_of[g] is group
@@ -2721,6 +2726,31 @@ int run_nd_axes(const std::vector<NDAxis>& axes, int prec, i64 total, const void
   return GOFFT_OK;
 }
 
+// One axis of an N-D plan over its first `nouter` outer blocks (an N-D
+// transform's axis a.outer blocks are the arrays times the other axes'
+// blocks; for the column axis of an RFFT2 plan they are the arrays), in place
+// or src -> dst.
+int run_axis_blocks(const NDAxis& a, int prec, const void* src, void* dst, int ci, int co, double sc, i64 nouter,
+                    hipStream_t s, char* ws) {
+  if (a.n == 1) {
+    const i64 elems = a.st * nouter;
+    if (src != dst) HIP_TRY(launch_copy_lines(prec, src, dst, elems, 1, elems, elems, s));
+    if (sc != 1.0) HIP_TRY(launch_scale(prec, dst, dst, 2 * elems, sc, s));
+    return GOFFT_OK;
+  }
+  if (a.st == 1) return a.eng->run(src, LineIO{nullptr, a.n}, dst, LineIO{nullptr, a.n}, ci, co, sc, Ctx{s, ws}, nouter);
+  if (!a.gather) return a.eng->run(src, LineIO{nullptr, 0}, dst, LineIO{nullptr, 0}, ci, co, sc, Ctx{s, ws}, nouter);
+  const i64 elems = a.n * a.st * nouter;
+  char* A = ws;
+  char* B = ws + (size_t)a.n * a.st * a.outer * elt(prec);
+  char* rest = B + (size_t)a.n * a.st * a.outer * elt(prec);
+  const i64 lines = elems / a.n;
+  HIP_TRY(launch_axis_copy(prec, src, A, a.n, a.st, lines, 0, s));
+  RC_TRY(a.eng->run(A, LineIO{nullptr, a.n}, B, LineIO{nullptr, a.n}, ci, co, sc, Ctx{s, rest}, lines));
+  HIP_TRY(launch_axis_copy(prec, B, dst, a.n, a.st, lines, 1, s));
+  return GOFFT_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2868,6 +2898,7 @@ int gofft_plan_nd(gofft_plan** plan, const int64_t* shape, int rank, int kind, i
   RC_TRY(build_axis(dtype, p->flags, p->rows, p->rcols, batch, col, &need));
   p->axes.push_back(std::move(col));
   // IRFFT2 works on a padded copy of the spectrum.
+  p->ws_engines = need;
   p->ws_need = need + (size_t)p->rows * p->rcols * batch * elt(dtype);
   return new_plan(plan, p);
 }
@@ -3497,6 +3528,68 @@ int gofft_exec_c2c_noalloc(const gofft_plan* p, const void* src, void* dst, int 
   w.main = (char*)dev.scratch;
   for (size_t i = 0; i < p->lanes.size(); i++) w.lanes.push_back((char*)dev.scratch + (i + 1) * slice);
   return run_op(p, OP_C2C, src, 0, dst, direction, (hipStream_t)stream, &w);
+}
+
+// The spectral filter loss and its gradient over an ND_R2C plan's arrays, in
+// chunks of `chunk` arrays whose spectrum and residual are formed, used and
+// dropped in turn (the forward recomputed per chunk inside the backward
+// rather than stored for the whole batch).
+int gofft_filter2_loss_grad(const gofft_plan* p, const void* x, const void* H, const void* y, void* loss, void* gx,
+                            int64_t chunk, void* stream) {
+  if (!p || p->kind != K_ND_R2C) return fail(GOFFT_ERR_INVALID_ARG, "plan is not an nd r2c plan");
+  const i64 B = p->batch;
+  if (B == 0) return GOFFT_OK;
+  if (!x || !H || !y || !loss || !gx) return fail(GOFFT_ERR_INVALID_ARG, "null data pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t e = elt(p->prec), re = relt(p->prec);
+  const i64 half = p->rows * p->rcols, per = p->rows * p->cols;
+  const i64 parts = residual_parts(per);
+  // spectrum, residual and the loss's partial sums of one array
+  const size_t per_arr = (size_t)half * e + (size_t)per * re + (size_t)parts * re;
+  i64 ch = chunk;
+  if (ch <= 0) {
+    // the largest chunk that fits 80% of the free device memory, at most all
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    const double room = 0.8 * (double)fr - (double)p->ws_engines;
+    ch = room > 0 ? (i64)(room / (double)per_arr) : 1;
+  }
+  ch = std::max<i64>(1, std::min<i64>(ch, B));
+  // the engines' workspace, then the chunk buffers (aligned to 256 bytes)
+  const size_t eng = (p->ws_engines + 255) / 256 * 256;
+  const size_t sb = ((size_t)ch * half * e + 255) / 256 * 256, rb = ((size_t)ch * per * re + 255) / 256 * 256;
+  char* buf = nullptr;
+  RC_TRY(p->loss_pool.get(s, eng + sb + rb + (size_t)ch * parts * re, &buf));
+  char* ws = buf;
+  char* S = buf + eng;
+  char* r = S + sb;
+  char* partial = r + rb;
+  const NDAxis& col = p->axes[0];
+  const double inv = 1.0 / ((double)p->rows * (double)p->cols);
+  for (i64 b0 = 0; b0 < B; b0 += ch) {
+    const i64 nb = std::min<i64>(ch, B - b0);
+    const char* xc = (const char*)x + (size_t)(b0 * per) * re;
+    const char* yc = (const char*)y + (size_t)(b0 * per) * re;
+    char* gc = (char*)gx + (size_t)(b0 * per) * re;
+    char* lc = (char*)loss + (size_t)b0 * re;
+    // S = RFFT2(x) (rfft2.go:22-74)
+    RC_TRY(p->nd_row->forward(xc, p->cols, S, p->rcols, 0, 1.0, nullptr, Ctx{s, ws}, 1.0, 1.0, nb * p->rows));
+    RC_TRY(run_axis_blocks(col, p->prec, S, S, 0, 0, 1.0, nb, s, ws));
+    // r = IRFFT2(S H) (rfft2.go:87-138)
+    HIP_TRY(launch_filter_mul(p->prec, S, H, half, nb, 0, s));
+    RC_TRY(run_axis_blocks(col, p->prec, S, S, 1, 1, 1.0 / (double)p->rows, nb, s, ws));
+    RC_TRY(p->nd_row->inverse(S, p->rcols, p->rcols, r, p->cols, Ctx{s, ws}, 1.0, 1.0, nb * p->rows));
+    // loss = ||r - y||^2 per array, r <- 2 (r - y)
+    HIP_TRY(launch_residual_loss(p->prec, r, yc, per, nb, partial, lc, s));
+    // gS = VJP of IRFFT2 at r, times conj(H)
+    RC_TRY(p->nd_row->forward(r, p->cols, S, p->rcols, 2, 0.0, nullptr, Ctx{s, ws}, inv, 2.0 * inv, nb * p->rows));
+    RC_TRY(run_axis_blocks(col, p->prec, S, S, 0, 0, 1.0, nb, s, ws));
+    HIP_TRY(launch_filter_mul(p->prec, S, H, half, nb, 1, s));
+    // gx = VJP of RFFT2 at gS
+    RC_TRY(run_axis_blocks(col, p->prec, S, S, 1, 1, 1.0, nb, s, ws));
+    RC_TRY(p->nd_row->inverse(S, p->rcols, p->rcols, gc, p->cols, Ctx{s, ws}, (double)p->cols, 0.5, nb * p->rows));
+  }
+  return GOFFT_OK;
 }
 
 // ---------------------------------------------------------------------------

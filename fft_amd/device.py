@@ -252,6 +252,15 @@ class NDPlan(_Plan):
         L.check(L.lib().gofft_vjp_nd_r2c(self._h, _ptr(g), _ptr(out), direction, _stream(stream)))
         return out
 
+    def filter_loss_grad(self, x, H, y, chunk=0, stream=None):
+        """ND_R2C plans: (loss per array, gradient) of ||IRFFT2(RFFT2(x) H) - y||^2
+        (gofft_filter2_loss_grad), `chunk` arrays at a time (0: as many as fit)."""
+        loss = torch.empty(self.batch, dtype=REAL_OF[self.code], device=x.device)
+        gx = torch.empty_like(x)
+        L.check(L.lib().gofft_filter2_loss_grad(self._h, _ptr(x), _ptr(H), _ptr(y), _ptr(loss), _ptr(gx),
+                                                int(chunk), _stream(stream)))
+        return loss, gx
+
     def vjp_c2c(self, gy, out=None, inverse=False, stream=None):
         """ND_C2C plans: VJP of FFTN (inverse=False) or IFFTN (inverse=True)."""
         if out is None:

@@ -235,6 +235,21 @@ int gofft_vjp_c2r(const gofft_plan* plan, const void* gx, void* gX, void* stream
 int gofft_vjp_nd_r2c(const gofft_plan* plan, const void* g_out, void* g_in, int direction,
                      void* stream);
 
+/* Loss and gradient of a spectral filter, for each array b of an nd-R2C
+ * plan (no reference counterpart):
+ *   loss[b] = || IRFFT2(RFFT2(x_b) * H) - y_b ||^2,   gx_b = d loss[b] / d x_b,
+ * with H (rows x (cols/2+1) complex, shared by every array) multiplying the
+ * half spectrum. The gradient runs the adjoint transforms (as
+ * gofft_vjp_nd_r2c) on 2 (IRFFT2(RFFT2(x) H) - y) with conj(H) between them.
+ * The arrays are processed `chunk` at a time: the spectrum and residual of
+ * only that many arrays are held, formed from x again for each chunk rather
+ * than stored for the whole batch; chunk <= 0 takes the largest chunk that
+ * fits 80% of the free device memory (the whole batch when it fits). The
+ * results do not depend on the chunk. Device pointers: x, y, gx of
+ * batch*rows*cols reals, loss of batch reals. */
+int gofft_filter2_loss_grad(const gofft_plan* plan, const void* x, const void* H, const void* y, void* loss,
+                            void* gx, int64_t chunk, void* stream);
+
 /* --- host convenience (reference slice API) -------------------------------- */
 
 /* The same operations on HOST buffers (ordinary pageable memory, e.g. Go
