@@ -529,7 +529,50 @@ def workload_twiddles(D, torch, dev, steps, warmup):
         torch.cuda.empty_cache()
 
 
-WORKLOADS = {"roundtrip4096": workload_roundtrip4096, "spectral": workload_spectral, "nd": workload_nd,
+def workload_host(D, torch, dev, steps, warmup):
+    """The host-staged path a Go binding calls (Plan.FFT on host slices):
+    FFT + IFFT of 256 x 4096 complex128 from and to ordinary pageable host
+    memory through gofft_exec_c2c_host (pinned staging, the call's own
+    stream), beside the same round trip on device-resident data. Wall-clock
+    per round trip (the host calls synchronise themselves); the data rate
+    counts the API bytes (input and output of both calls)."""
+    import ctypes
+    L = D.L
+    n, batch = 4096, 256
+    x = gaussian_complex(n * batch, 0)
+    y = np.empty_like(x)
+    z = np.empty_like(x)
+    p = D.C2CPlan(n, batch, torch.complex128)
+    lib = L.lib()
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+
+    def host_step():
+        L.check(lib.gofft_exec_c2c_host(p.handle, vp(x), vp(y), L.FORWARD))
+        L.check(lib.gofft_exec_c2c_host(p.handle, vp(y), vp(z), L.INVERSE))
+    for _ in range(warmup):
+        host_step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        host_step()
+    ms_host = (time.perf_counter() - t0) * 1e3 / steps
+    xd = torch.from_numpy(x).to(dev)
+    yd = torch.empty_like(xd)
+    zd = torch.empty_like(xd)
+
+    def dev_step():
+        p.exec(xd, out=yd)
+        p.exec(yd, out=zd, inverse=True)
+    ms_dev = time_events(dev_step, steps, warmup, torch)
+    err = float(np.linalg.norm(z - x) / np.linalg.norm(x))
+    nbytes = 4 * n * batch * 16
+    _line("host_roundtrip4096", ms_host, 2 * batch * 5.0 * n * 12, nbytes, "c128",
+          "FFT + IFFT, N=4096, batch 256, complex128, host slices in and out (gofft_exec_c2c_host: "
+          "PCIe both ways, pinned staging)",
+          {"device_resident_ms": round(ms_dev, 4), "host_over_device": round(ms_host / ms_dev, 2),
+           "roundtrip_err": {"rel_l2": float("%.3g" % err)}})
+
+
+WORKLOADS = {"host": workload_host, "roundtrip4096": workload_roundtrip4096, "spectral": workload_spectral, "nd": workload_nd,
              "blur": workload_blur, "groups": workload_groups, "twiddles": workload_twiddles}
 
 
