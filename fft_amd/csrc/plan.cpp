@@ -2189,7 +2189,13 @@ class RealEngine {
       DevInfo dev;
       RC_TRY(current_device(&dev));
       const i64 pts = dev.lds_bytes / (i64)elt(prec);
-      const i64 budget = std::min<i64>(pts, std::max<i64>(pitch(m), soft_pts(prec)));
+      // complex64 takes about 16 KB of LDS per workgroup (two waves for
+      // 512-point halves): measured on the 2^26 Spectrogram (131071 frames
+      // of 1024), 4 frames per workgroup 0.296 ms against 8 frames 0.308 ms,
+      // 2 frames 0.309 ms and 1 frame 0.347 ms. complex128 keeps the larger
+      // budget (real_cols then picks by resident waves).
+      const i64 soft = prec == GOFFT_C64 ? 2100 : soft_pts(prec);
+      const i64 budget = std::min<i64>(pts, std::max<i64>(pitch(m), soft));
       int C = 0;
       const int cap = (int)std::min<i64>(kMaxCols, std::max<i64>(1, batch));
       if (pitch(m) <= pts && factors_are_small(m) &&
