@@ -27,6 +27,7 @@ FLAG_ACCURATE_TWIDDLES = 2
 FLAG_AUTOTUNE = 4
 FLAG_TWIDDLES_FP32 = 8
 FLAG_TWIDDLES_SPLIT = 16
+FLAG_QUEUE = 32
 ND_C2C = 0
 ND_R2C = 1
 KERNEL_PASS, KERNEL_CONV, KERNEL_REAL, KERNEL_AUX = 0, 1, 2, 3

@@ -18,8 +18,10 @@ LIB = os.path.join(HERE, "libgofft_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["kernels.hip", "plan.cpp"]
+SOURCES = ["kernels.hip", "queue.hip", "plan.cpp"]
 HEADERS = [os.path.join(CSRC, "kernels.h"), os.path.join(INCLUDE, "gofft.h")]
+# sources that include another source file
+INCLUDES = {"queue.hip": [os.path.join(CSRC, "kernels.hip")]}
 
 FLAGS = [
     "--offload-arch=" + ARCH,
@@ -38,7 +40,8 @@ def _stale(obj, src):
     if not os.path.exists(obj):
         return True
     t = os.path.getmtime(obj)
-    return any(os.path.getmtime(d) > t for d in [src] + HEADERS)
+    deps = [src] + HEADERS + INCLUDES.get(os.path.basename(src), [])
+    return any(os.path.getmtime(d) > t for d in deps)
 
 
 def build(force=False, verbose=False):

@@ -214,6 +214,71 @@ struct ConvArgs {
   SubFFT f;
 };
 
+// Persistent work queue of a mixed-length plan: one kernel whose workgroups
+// pop items -- the column groups of a pass, or the line groups of a fused
+// convolution -- of several launches ("stages") in order from a device
+// counter. A stage whose `dep` is set waits until every item of that earlier
+// stage has finished; items are popped in stage order and every dependency
+// points back, so the lowest unfinished item never waits and the queue
+// drains. Waits are bounded in wall-clock time and end in an error flag.
+constexpr int kMaxQueueStages = 48;
+struct QueueStage {
+  long long start;   // first item of the stage
+  long long nitems;  // its items
+  long long desc;    // byte offset of its PassArgs / ConvArgs in the descriptor block
+  int kind;          // 0 pass (k_pass form `form`), 1 fused convolution (points per thread `form`)
+  int form;
+  int dep;           // stage that must finish first, or -1
+  int signal;        // 1: a later stage waits on this one (its finished items are counted)
+};
+struct QueueArgs {
+  const char* descs;         // descriptor block with the exec's pointers resolved
+  unsigned long long* ticket;
+  unsigned* error;           // set to 1 when a wait exceeds spin_limit
+  unsigned* done;            // finished items per stage
+  long long nitems;
+  long long spin_limit;      // ticks of the 100 MHz wall clock a wait may take
+  long long sh_off;          // byte offset in dynamic LDS of the pass stages' per-column tables
+  int nstages;
+  int need_extra;            // added to every wait's item count (0; tests of the time-out path)
+  QueueStage st[kMaxQueueStages];
+};
+// One descriptor of the block: copied from the plan's template and its
+// pointer fields (up to four) set to base[sel] + add (sel 1 source, 2
+// destination, 3 workspace) for the exec at hand.
+struct QueueDesc {
+  long long off, bytes;
+  int nreloc;
+  int field[4];              // byte offsets of the pointer fields in the descriptor
+  int sel[4];
+  long long add[4];
+};
+struct QueueSetupArgs {
+  const char* tmpl;
+  char* descs;
+  const QueueDesc* d;
+  const char* base[4];
+  unsigned* ctr;             // counters to zero (ticket, error flag, stage counts)
+  int nctr;
+};
+// Forms of k_pass that the queue kernel of precision prec runs (bit f set
+// for form f), and the points per thread of its fused convolutions.
+unsigned queue_pass_forms(int prec);
+bool queue_conv_form(int prec, int maxp);
+// The k_pass form launch_pass picks for a, twiddle variants aside.
+int pass_form(int prec, const PassArgs& a);
+// Workgroups of the queue kernel (prec, threads) that one CU holds with
+// `lds` bytes of dynamic LDS each (0 when it does not fit).
+int queue_blocks_per_cu(int prec, int threads, size_t lds);
+// Bytes of the per-column tables that a pass stage of the queue kernel keeps
+// in dynamic LDS after its columns (at QueueArgs::sh_off), and of the queue
+// kernel's static LDS.
+size_t queue_pass_tables(int prec);
+size_t queue_static_lds();
+hipError_t launch_queue_setup(const QueueSetupArgs& a, int ndesc, hipStream_t s);
+hipError_t launch_queue(int prec, const QueueArgs& a, int threads, size_t lds, long long grid,
+                        hipStream_t s);
+
 // Launchers (defined in kernels.hip). Each returns a hipError_t.
 hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s);
 hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads, hipStream_t s);

@@ -373,7 +373,7 @@ __device__ __forceinline__ void lds_stage_rec(T2* lds, const SubFFT& f, const St
 // LDS (all of them by default). cdig is the
 // calling thread's column twiddle digit (only read by stages with cmul != 0).
 template <typename T2, int MAXP>
-__device__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw, int cdig = 0, int s0 = 0, int s1 = -1) {
+__device__ __forceinline__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw, int cdig = 0, int s0 = 0, int s1 = -1) {
   const int tid = threadIdx.x;
   const int c = tid >> f.log2tpc;
   const int lt = tid & (f.tpc - 1);
@@ -933,14 +933,31 @@ constexpr int reg2_hk(int mode) { return mode >= 5 ? 1 : 2; }
 // that spill under the 128-register cap of a 1024-thread workgroup (0, 3
 // and 4) also have an LB = 256 instantiation for passes of at most 256
 // threads, which holds them in registers (126-131 VGPRs).
-template <typename T, int MAXP, int MODE, int LB = 1024, int TWM = 0>
-__global__ void __launch_bounds__(LB) k_pass(PassArgs a) {
+//
+// pass_body is the workgroup's work for column group `blk` of a launch of
+// `nblk` groups; k_pass runs it for blockIdx.x, the work-queue kernel for
+// the groups it pops. sh holds the per-column tables in static LDS.
+template <typename T2>
+struct PassShared {
+  long long ib[kMaxCols], il[kMaxCols], ob[kMaxCols], ol[kMaxCols], tk[kMaxCols];
+  long long mk[kMaxCols], ln[kMaxCols];
+  T2 x0[kMaxCols];
+};
+
+template <typename T, int MAXP, int MODE, int TWM>
+__device__ __forceinline__ void pass_body(const PassArgs& a, long long blk, long long nblk,
+                                          PassShared<typename Vec2<T>::type>& sh) {
   constexpr bool EXT = MODE >= 1;
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
-  __shared__ long long s_ib[kMaxCols], s_il[kMaxCols], s_ob[kMaxCols], s_ol[kMaxCols], s_tk[kMaxCols];
-  __shared__ long long s_mk[kMaxCols], s_ln[kMaxCols];
-  __shared__ T2 s_x0[kMaxCols];
+  long long* const s_ib = sh.ib;
+  long long* const s_il = sh.il;
+  long long* const s_ob = sh.ob;
+  long long* const s_ol = sh.ol;
+  long long* const s_tk = sh.tk;
+  long long* const s_mk = sh.mk;
+  long long* const s_ln = sh.ln;
+  T2* const s_x0 = sh.x0;
   const SubFFT& f = a.f;
   const int in_hook = EXT ? a.in_hook : 0;
   const int out_hook = EXT ? a.out_hook : 0;
@@ -952,9 +969,8 @@ __global__ void __launch_bounds__(LB) k_pass(PassArgs a) {
   // 128-byte runs, or read and write unaligned lines), workgroup b takes
   // group (b mod 8) * q + b / 8 (q = G / 8; the last G mod 8 keep their
   // index), so consecutive groups run on one XCD and share its L2.
-  long long blk = blockIdx.x;
   if (a.xcd_group > 1) {
-    const long long G = gridDim.x, q = G >> 3;
+    const long long G = nblk, q = G >> 3;
     if (blk < (q << 3)) blk = (blk & 7) * q + (blk >> 3);
   }
   const long long g0 = blk * C;
@@ -1255,6 +1271,12 @@ __global__ void __launch_bounds__(LB) k_pass(PassArgs a) {
   }
 }
 
+template <typename T, int MAXP, int MODE, int LB = 1024, int TWM = 0>
+__global__ void __launch_bounds__(LB) k_pass(PassArgs a) {
+  __shared__ PassShared<typename Vec2<T>::type> sh;
+  pass_body<T, MAXP, MODE, TWM>(a, blockIdx.x, gridDim.x, sh);
+}
+
 // ---------------------------------------------------------------------------
 // Real forward, single pass: line l holds 2R real samples; Z = FFT_R(pairs),
 // then the untangle to R+1 bins (mode 0) or the one-sided PSD (mode 1).
@@ -1417,14 +1439,16 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
 // Fused convolution (Rader / Bluestein), C transforms per workgroup.
 // NT: the largest workgroup the instantiation serves (complex128 also has
 // NT = 256, which lifts the 128-register cap it spills under at 1024).
-template <typename T, int MAXP, int NT = 1024>
-__global__ void __launch_bounds__(NT) k_conv(ConvArgs a) {
+// conv_body is the workgroup's work for line group `blk`; k_conv runs it for
+// blockIdx.x, the work-queue kernel for the groups it pops.
+template <typename T, int MAXP>
+__device__ __forceinline__ void conv_body(const ConvArgs& a, long long blk) {
   typedef typename Vec2<T>::type T2;
   T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
   const SubFFT& f = a.f;
   const int tid = threadIdx.x;
   const int C = f.C, R = f.R, Rp = f.Rp;
-  const long long l0 = (long long)blockIdx.x * C;
+  const long long l0 = blk * C;
   const T2* in = reinterpret_cast<const T2*>(a.in);
   const T2* chirp = reinterpret_cast<const T2*>(a.chirp);
   const int n = a.n;
@@ -1555,6 +1579,17 @@ __global__ void __launch_bounds__(NT) k_conv(ConvArgs a) {
     }
   }
 }
+
+template <typename T, int MAXP, int NT = 1024>
+__global__ void __launch_bounds__(NT) k_conv(ConvArgs a) {
+  conv_body<T, MAXP>(a, blockIdx.x);
+}
+
+// Everything below but the work-queue section is compiled in this file's
+// own translation unit; queue.hip compiles the file again with
+// GOFFT_QUEUE_TU defined for the work-queue kernels alone (the kernel
+// templates above are instantiated only where they are launched).
+#ifndef GOFFT_QUEUE_TU
 
 // ---------------------------------------------------------------------------
 // Element-wise helpers.
@@ -1943,15 +1978,10 @@ template <typename T> static size_t elt2() { return 2 * sizeof(T); }
 #define GOFFT_LAUNCH16(KERNEL, T, ARGS, BYTES, THREADS, GRID, STREAM) \
   hipLaunchKernelGGL((KERNEL<T, 16>), dim3(GRID), dim3(THREADS), BYTES, STREAM, ARGS)
 
-hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s) {
-  (void)maxp;
-  long long groups = (a.ncols + a.f.C - 1) / a.f.C;
+int pass_form(int prec, const PassArgs& a) {
+  if (a.real_last || a.real_first) return a.real_last ? 11 : 12;
   const bool ext = a.in_hook || a.out_hook || a.mid_spec;
   int mode = a.mid256 && prec != 0 ? 2 : ext ? 1 : 0;  // form 2: complex64 only
-  if (a.real_last || a.real_first) {
-    mode = a.real_last ? 11 : 12;
-    groups = a.ru_lines * a.ru_wpl;
-  }
   // The two-stage register form: R = A x 16 planned radix A first, 16
   // threads per column, plain stage twiddles, no spectrum multiply. Complex64
   // Bluestein chains take the HK = 1 forms (they spill in complex128); the
@@ -1973,6 +2003,13 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
       if (A == 12) mode = 4;
     }
   }
+  return mode;
+}
+
+hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s) {
+  (void)maxp;
+  const int mode = pass_form(prec, a);
+  const long long groups = a.real_last || a.real_first ? a.ru_lines * a.ru_wpl : (a.ncols + a.f.C - 1) / a.f.C;
 #define GOFFT_PASS(T, M) hipLaunchKernelGGL((k_pass<T, 16, M>), dim3(groups), dim3(threads), bytes, s, a)
 #define GOFFT_PASS_LB(T, M, LB) hipLaunchKernelGGL((k_pass<T, 16, M, LB>), dim3(groups), dim3(threads), bytes, s, a)
   if (prec == 0) {
@@ -2304,5 +2341,198 @@ hipError_t launch_scale(int prec, const void* in, void* out, long long n, double
                        scale);
   return hipGetLastError();
 }
+
+#else  // GOFFT_QUEUE_TU
+
+// ---------------------------------------------------------------------------
+// Persistent work queue (mixed-length plans). Every workgroup loops: thread 0
+// takes the next ticket from the device counter, the workgroup finds the
+// stage holding it and runs that stage's pass or convolution body on the
+// column (line) group ticket - start. Workgroups are NT threads with one
+// dynamic LDS size; the host re-plans each stage's columns (lines) per
+// workgroup so that they fill NT threads.
+//
+// Ordering between stages: a workgroup that moves on from a stage whose
+// items others wait for (signal) first makes its stores visible at agent
+// scope -- every wave waits for its stores to complete, the workgroup
+// meets at a barrier, and thread 0 releases (agent scope) and adds the
+// number of that stage's items it finished to done[stage]. A workgroup
+// entering a stage with a dependency waits, once, until done[dep] reaches
+// the dep's item count (thread 0 polls with acquire loads, within a
+// wall-clock budget), then meets at a barrier. Tickets
+// are taken in stage order and dependencies point to earlier stages, so the
+// lowest unfinished ticket never waits on unfinished work: the queue always
+// drains, and a wait that outlives the budget means a planning error, which
+// sets *error and stops that workgroup (and, through the flag, the others).
+// complex64: at most 128 registers, so that 16 waves share a CU at any
+// workgroup size.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+k_queue(QueueArgs q) {
+  typedef typename Vec2<T>::type T2;
+  // The pass stages' per-column tables live in dynamic LDS past the largest
+  // pass stage's columns, so that a fused convolution can have all of it.
+  PassShared<T2>& sh = *reinterpret_cast<PassShared<T2>*>(g_dyn_lds + q.sh_off);
+  __shared__ long long s_ticket;
+  __shared__ int s_fail;
+  const int tid = threadIdx.x;
+  int cur = -1;       // stage of the previous item
+  unsigned pend = 0;  // items of a signalling stage finished since its last count
+  for (;;) {
+    if (tid == 0)
+      s_ticket = (long long)__hip_atomic_fetch_add(q.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    // (tickets stay below 2^31: the host plans at most 2^30 items)
+    const long long t = __builtin_amdgcn_readfirstlane((int)s_ticket);
+    // (every read of s_ticket ends before thread 0 can write it again)
+    __syncthreads();
+    int si = cur < 0 ? 0 : cur;
+    while (si < q.nstages && t >= q.st[si].start + q.st[si].nitems) ++si;
+    if (si != cur) {
+      if (pend) {
+        // every wave waits until its stores have reached the L2, then one
+        // agent-scope release (an L2 write-back) covers the workgroup
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          __hip_atomic_fetch_add(q.done + cur, pend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        pend = 0;
+      }
+      if (si >= q.nstages) break;
+      cur = si;
+      const int d = q.st[si].dep;
+      if (d >= 0) {
+        if (tid == 0) {
+          const unsigned need = (unsigned)(q.st[d].nitems + q.need_extra);
+          const unsigned long long t0 = wall_clock64();
+          int fail = 0;
+          while (__hip_atomic_load(q.done + d, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+            if (__hip_atomic_load(q.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+              fail = 1;
+              break;
+            }
+            if ((long long)(wall_clock64() - t0) > q.spin_limit) {
+              __hip_atomic_store(q.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              fail = 1;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+          }
+          s_fail = fail;
+        }
+        // (thread 0's acquire invalidated the CU's and the XCD's caches,
+        // which the whole workgroup reads through)
+        __syncthreads();
+        if (s_fail) break;
+      }
+    }
+    const QueueStage& st = q.st[si];
+    const long long blk = t - st.start;
+    // The descriptors are read through the constant address space, which
+    // lets the bodies load their fields with scalar loads as they load a
+    // kernel's own arguments (nothing writes them during the kernel).
+    typedef __attribute__((address_space(4))) const char cchar;
+    cchar* dp = (cchar*)q.descs + st.desc;
+    if (st.kind == 0) {
+      const PassArgs& a = *(const PassArgs*)dp;
+      switch (st.form) {
+        case 1: pass_body<T, 16, 1, 0>(a, blk, st.nitems, sh); break;
+        case 3: pass_body<T, 16, 3, 0>(a, blk, st.nitems, sh); break;
+        case 4: pass_body<T, 16, 4, 0>(a, blk, st.nitems, sh); break;
+        default:
+          if constexpr (sizeof(T) == 4) {
+            if (st.form == 2) { pass_body<T, 16, 2, 0>(a, blk, st.nitems, sh); break; }
+            if (st.form == 5) { pass_body<T, 16, 5, 0>(a, blk, st.nitems, sh); break; }
+            if (st.form == 6) { pass_body<T, 16, 6, 0>(a, blk, st.nitems, sh); break; }
+            if (st.form == 7) { pass_body<T, 16, 7, 0>(a, blk, st.nitems, sh); break; }
+          }
+          pass_body<T, 16, 0, 0>(a, blk, st.nitems, sh);
+          break;
+      }
+    } else {
+      const ConvArgs& a = *(const ConvArgs*)dp;
+      if constexpr (sizeof(T) == 4) {
+        if (st.form == 24) conv_body<T, 24>(a, blk);
+        else conv_body<T, 16>(a, blk);
+      } else {
+        conv_body<T, 16>(a, blk);
+      }
+    }
+    if (st.signal) ++pend;
+  }
+}
+
+// Copies the plan's descriptor template into the exec's workspace, one
+// workgroup per descriptor, and sets its pointer fields for this exec;
+// workgroup 0 also zeroes the queue's counters.
+__global__ void __launch_bounds__(256) k_queue_setup(QueueSetupArgs a) {
+  const QueueDesc& d = a.d[blockIdx.x];
+  const long long* src = reinterpret_cast<const long long*>(a.tmpl + d.off);
+  long long* dst = reinterpret_cast<long long*>(a.descs + d.off);
+  for (long long i = threadIdx.x; i < d.bytes / 8; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+  if ((int)threadIdx.x < d.nreloc) {
+    const int k = threadIdx.x;
+    *reinterpret_cast<const char**>(a.descs + d.off + d.field[k]) = a.base[d.sel[k]] + d.add[k];
+  }
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < a.nctr; i += blockDim.x) a.ctr[i] = 0u;
+}
+
+unsigned queue_pass_forms(int prec) { return prec == 0 ? 0x1bu : 0xffu; }  // complex128: 0, 1, 3, 4
+
+bool queue_conv_form(int prec, int maxp) { return maxp == 16 || (prec != 0 && maxp == 24); }
+
+size_t queue_pass_tables(int prec) { return prec == 0 ? sizeof(PassShared<double2>) : sizeof(PassShared<float2>); }
+
+size_t queue_static_lds() { return 64; }
+
+// Queue kernel instantiations: complex64 with 256, 512 or 1024 threads,
+// complex128 with 256 or 512.
+template <typename F>
+static bool queue_kernel(int prec, int threads, F&& f) {
+  if (prec == 0) {
+    if (threads == 256) return f(reinterpret_cast<const void*>(&k_queue<double, 256>));
+    if (threads == 512) return f(reinterpret_cast<const void*>(&k_queue<double, 512>));
+  } else {
+    if (threads == 256) return f(reinterpret_cast<const void*>(&k_queue<float, 256>));
+    if (threads == 512) return f(reinterpret_cast<const void*>(&k_queue<float, 512>));
+    if (threads == 1024) return f(reinterpret_cast<const void*>(&k_queue<float, 1024>));
+  }
+  return false;
+}
+
+int queue_blocks_per_cu(int prec, int threads, size_t lds) {
+  int nb = 0;
+  queue_kernel(prec, threads, [&](const void* k) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, threads, lds) != hipSuccess) nb = 0;
+    return true;
+  });
+  (void)hipGetLastError();
+  return nb;
+}
+
+hipError_t launch_queue_setup(const QueueSetupArgs& a, int ndesc, hipStream_t s) {
+  hipLaunchKernelGGL(k_queue_setup, dim3(ndesc), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_queue(int prec, const QueueArgs& a, int threads, size_t lds, long long grid, hipStream_t s) {
+  if (prec == 0) {
+    if (threads == 256) hipLaunchKernelGGL((k_queue<double, 256>), dim3(grid), dim3(256), lds, s, a);
+    else if (threads == 512) hipLaunchKernelGGL((k_queue<double, 512>), dim3(grid), dim3(512), lds, s, a);
+    else return hipErrorInvalidValue;
+  } else {
+    if (threads == 256) hipLaunchKernelGGL((k_queue<float, 256>), dim3(grid), dim3(256), lds, s, a);
+    else if (threads == 512) hipLaunchKernelGGL((k_queue<float, 512>), dim3(grid), dim3(512), lds, s, a);
+    else if (threads == 1024) hipLaunchKernelGGL((k_queue<float, 1024>), dim3(grid), dim3(1024), lds, s, a);
+    else return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+#endif  // GOFFT_QUEUE_TU
 
 }  // namespace gofft

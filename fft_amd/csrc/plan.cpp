@@ -12,9 +12,11 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstddef>
 #include <cstdlib>
 #include <cstdio>
 #include <functional>
+#include <limits>
 #include <tuple>
 #include <cstring>
 #include <map>
@@ -811,6 +813,9 @@ class Engine {
   // Whether a workgroup of this engine takes more than half a CU's LDS, so
   // that it can start only on a CU the other launches have left.
   virtual bool whole_cu() const { return false; }
+  // Whether run() enqueues only pass and fused-convolution launches (so that
+  // a mixed-length plan's work queue can take them over).
+  virtual bool queueable() const { return false; }
   // out = scale * conj?( FFT( conj?(in) ) ) per line, for the first `nl`
   // lines (all `batch` lines when nl < 0).
   virtual int run(const void* in, LineIO li, void* out, LineIO lo, int conj_in, int conj_out,
@@ -846,6 +851,45 @@ void* line_ptr(void* p, const LineIO& li, i64 b0, size_t eb) {
 LineIO line_sub(const LineIO& li, i64 b0) { return LineIO{li.off ? li.off + b0 : nullptr, li.stride}; }
 
 std::unique_ptr<Engine> make_engine(i64 n, i64 batch, int prec, unsigned flags, int* rc);
+
+// Launch recording: while t_record is set on a thread, the engines' pass and
+// fused-convolution launches are appended to it instead of being enqueued.
+// A mixed-length plan records its groups' runs this way, at plan creation
+// with stand-in data pointers, to build its work-queue stages.
+struct RecordedLaunch {
+  int kind = 0;  // 0 pass, 1 fused convolution
+  PassArgs pa{};
+  ConvArgs ca{};
+  int threads = 0;
+  int maxp = 0;
+};
+thread_local std::vector<RecordedLaunch>* t_record = nullptr;
+
+hipError_t issue_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s) {
+  if (t_record) {
+    RecordedLaunch r;
+    r.kind = 0;
+    r.pa = a;
+    r.threads = threads;
+    r.maxp = maxp;
+    t_record->push_back(r);
+    return hipSuccess;
+  }
+  return launch_pass(prec, maxp, a, threads, s);
+}
+
+hipError_t issue_conv(int prec, int maxp, const ConvArgs& a, int threads, hipStream_t s) {
+  if (t_record) {
+    RecordedLaunch r;
+    r.kind = 1;
+    r.ca = a;
+    r.threads = threads;
+    r.maxp = maxp;
+    t_record->push_back(r);
+    return hipSuccess;
+  }
+  return launch_conv(prec, maxp, a, threads, s);
+}
 
 // Trivial engine (n <= 1): out = scale * in (conjugations cancel for n = 1).
 class TrivialEngine : public Engine {
@@ -939,6 +983,7 @@ class StockhamEngine : public Engine {
   int tw_mode = 0;
   DevBuf ptwx;
   int P() const { return (int)passes.size(); }
+  bool queueable() const override { return st == 1 && inner == 1 && tw_mode == 0; }
 
   // Power-of-two transform with the reference's recurrence twiddles
   // (kernels.Radix2): radix-2 layers only, grouped into passes of up to 2^8
@@ -1326,7 +1371,7 @@ class StockhamEngine : public Engine {
         // inverse's fused pack gains nothing from it).
         a.xcd_group = 2;
       }
-      HIP_TRY(launch_pass(prec, kMaxP, a, threads, c.s));
+      HIP_TRY(issue_pass(prec, kMaxP, a, threads, c.s));
     }
     return GOFFT_OK;
   }
@@ -1564,7 +1609,7 @@ struct ConvChain {
       // complex64 0.277 -> 0.271 ms; the middle launches (aligned workspace
       // rows) gain nothing from it.
       if (k == 0 || k == K - 1) a.xcd_group = 2;
-      HIP_TRY(launch_pass(prec, kMaxP, a, threads, s));
+      HIP_TRY(issue_pass(prec, kMaxP, a, threads, s));
     }
     return GOFFT_OK;
   }
@@ -1685,6 +1730,7 @@ class RaderEngine : public Engine {
 
   size_t ws_bytes() const override { return fused ? 0 : chain.ws_bytes(); }
   bool whole_cu() const override { return fused && big_lds; }
+  bool queueable() const override { return true; }
   int launches() const override {
     return fused ? 1 : (int)(chain.launches() * ((batch + cb - 1) / cb));
   }
@@ -1722,7 +1768,7 @@ class RaderEngine : public Engine {
       a.in_off = li.off;
       a.out_off = lo.off;
       a.nlines = lines;
-      HIP_TRY(launch_conv(prec, sp.maxp, a, sp.threads, c.s));
+      HIP_TRY(issue_conv(prec, sp.maxp, a, sp.threads, c.s));
       return GOFFT_OK;
     }
     ConvChain::Hooks h;
@@ -1839,6 +1885,7 @@ class BluesteinEngine : public Engine {
 
   size_t ws_bytes() const override { return fused ? 0 : chain.ws_bytes(); }
   bool whole_cu() const override { return fused && big_lds; }
+  bool queueable() const override { return true; }
   int launches() const override {
     return fused ? 1 : (int)(chain.launches() * ((batch + cb - 1) / cb));
   }
@@ -1882,7 +1929,7 @@ class BluesteinEngine : public Engine {
       a.in_off = li.off;
       a.out_off = lo.off;
       a.nlines = lines;
-      HIP_TRY(launch_conv(prec, sp.maxp, a, sp.threads, c.s));
+      HIP_TRY(issue_conv(prec, sp.maxp, a, sp.threads, c.s));
       return GOFFT_OK;
     }
     ConvChain::Hooks h;
@@ -2521,6 +2568,27 @@ struct gofft_plan {
   std::vector<int> lane_of;
   std::vector<size_t> order;  // enqueue order of the groups
   mutable std::mutex lane_mu;  // one fork/join enqueue at a time
+  // Work queue (GOFFT_FLAG_QUEUE): the groups it takes run as the stages of
+  // one persistent kernel (kernels.hip k_queue) on the caller's stream; the
+  // others run on the side lanes beside it. The workspace starts with the
+  // queue's counters and descriptor block, then each queued group's own
+  // workspace.
+  struct QueueVariant {
+    DevBuf tmpl, table;  // descriptor template block, its QueueDesc table
+    int ndesc = 0;
+    QueueArgs args{};    // stage table; pointers set per exec
+  };
+  struct Queue {
+    bool on = false;
+    bool check = false;  // GOFFT_QUEUE_CHECK: each exec waits and reads the error flag
+    int threads = 0;
+    size_t lds = 0;
+    long long grid = 0;
+    size_t off_desc = 0, off_groups = 0, ctr_words = 0;
+    std::vector<char> in_queue;  // per group
+    QueueVariant var[4];         // index: conjugation + 2 * (1/n scaling)
+    std::string text;
+  } q;
 
   int ws(hipStream_t s, char** out) const {
     if (ws_need == 0) {
@@ -2547,7 +2615,15 @@ namespace {
 int assign_lanes(gofft_plan& p) {
   int want = 5;
   if (const char* e = std::getenv("GOFFT_LANES")) want = std::max(1, std::min(16, std::atoi(e)));
-  const int n = (int)std::min<size_t>((size_t)want, p.groups.size());
+  // With a work queue on the caller's stream, the groups it does not take
+  // share the side lanes.
+  size_t lane_groups = p.groups.size();
+  if (p.q.on) {
+    lane_groups = 0;
+    for (size_t g = 0; g < p.groups.size(); g++) lane_groups += p.q.in_queue[g] ? 0 : 1;
+    if (lane_groups > 0) lane_groups++;
+  }
+  const int n = (int)std::min<size_t>((size_t)want, lane_groups);
   p.lane_of.assign(p.groups.size(), 0);
   // Enqueue order: groups whose workgroups need more than half a CU's LDS
   // first, while every CU is still empty; the others after, largest first.
@@ -2559,7 +2635,9 @@ int assign_lanes(gofft_plan& p) {
       if (p.groups[g]->eng->whole_cu() == (pass == 0)) p.order.push_back(g);
   if (n <= 1) return GOFFT_OK;
   std::vector<double> load(n, 0.0);
+  if (p.q.on) load[0] = std::numeric_limits<double>::infinity();  // the queue's stream
   for (size_t g = 0; g < p.groups.size(); g++) {
+    if (p.q.on && p.q.in_queue[g]) continue;
     const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     p.lane_of[g] = k;
     load[k] += (double)p.groups[g]->len * (double)p.groups[g]->eng->batch;
@@ -2573,6 +2651,312 @@ int assign_lanes(gofft_plan& p) {
     RC_TRY(p.ws(p.lanes[i], &w));
   }
   HIP_TRY(hipEventCreateWithFlags(&p.fork_ev, hipEventDisableTiming));
+  return GOFFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Work queue of a mixed-length plan (GOFFT_FLAG_QUEUE, or GOFFT_QUEUE=1).
+// Each queueable group's run is recorded once per variant (conjugation and
+// 1/n scaling) with stand-in data pointers; every recorded launch becomes a
+// stage of the queue, re-planned to the queue's workgroup size (columns or
+// lines per workgroup = threads / threads per column), and its pointer
+// fields that hold a stand-in become relocations that the setup kernel
+// resolves per exec. A group's stages run in order (each waits for the
+// previous one); groups do not wait for each other.
+
+// Stand-in bases: 1 source, 2 destination, 3 the queued groups' workspace.
+constexpr uintptr_t kFakeSpan = (uintptr_t)1 << 56;
+const void* fake_base(int sel) { return (const void*)((uintptr_t)sel << 60); }
+int fake_sel(const void* v, long long* add) {
+  const uintptr_t x = (uintptr_t)v;
+  for (int sel = 1; sel <= 3; sel++) {
+    const uintptr_t b = (uintptr_t)sel << 60;
+    if (x >= b && x < b + kFakeSpan) {
+      *add = (long long)(x - b);
+      return sel;
+    }
+  }
+  return 0;
+}
+
+struct QStage {
+  int group = 0, level = 0;
+  int kind = 0, form = 0;
+  i64 nitems = 0;
+  size_t lds = 0;
+  double work = 0;                 // data bytes of the stage (ordering only)
+  std::vector<char> desc[4];       // descriptor bytes per variant (relocated fields zero)
+  QueueDesc rel[4];                // relocations per variant (off and bytes set later)
+};
+
+i64 env_int(const char* name, i64 dflt) {
+  const char* e = std::getenv(name);
+  return e && *e ? std::atoll(e) : dflt;
+}
+
+// Re-plans one recorded launch for a workgroup of `threads` threads and
+// records its descriptor; false if the queue kernel cannot run it.
+bool queue_stage(int prec, int threads, const RecordedLaunch& r, QStage& st, int var) {
+  const size_t e2 = elt(prec);
+  auto relocate = [&](char* base, size_t bytes, std::initializer_list<size_t> fields, QueueDesc& qd) -> bool {
+    qd.nreloc = 0;
+    for (size_t f : fields) {
+      void* v = nullptr;
+      std::memcpy(&v, base + f, sizeof(v));
+      long long add = 0;
+      const int sel = fake_sel(v, &add);
+      if (!sel) continue;
+      if (qd.nreloc >= 4) return false;
+      qd.field[qd.nreloc] = (int)f;
+      qd.sel[qd.nreloc] = sel;
+      qd.add[qd.nreloc] = add;
+      qd.nreloc++;
+      std::memset(base + f, 0, sizeof(v));
+    }
+    // no other 8-byte word may hold a stand-in pointer
+    for (size_t o = 0; o + 8 <= bytes; o += 8) {
+      void* v = nullptr;
+      std::memcpy(&v, base + o, sizeof(v));
+      long long add = 0;
+      if (fake_sel(v, &add)) return false;
+    }
+    return true;
+  };
+  QueueDesc qd{};
+  if (r.kind == 0) {
+    PassArgs a = r.pa;
+    const int form = pass_form(prec, a);
+    if (form >= 11 || !((queue_pass_forms(prec) >> form) & 1u)) return false;
+    if (r.threads != a.f.C * a.f.tpc || a.tw_mode != 0) return false;
+    const int C = threads / a.f.tpc;
+    if (C < 1 || C > kMaxCols || C * a.f.tpc != threads) return false;
+    a.f.C = C;
+    a.f.log2C = ilog2(C);
+    int rp = a.f.Rp;
+    if (a.mid_spec) {
+      if (a.f2.tpc != a.f.tpc) return false;
+      a.f2.C = C;
+      a.f2.log2C = ilog2(C);
+      rp = std::max(rp, a.f2.Rp);
+    }
+    a.xcd_group = 1;  // the queue hands out column groups in ticket order
+    st.kind = 0;
+    st.form = form;
+    st.nitems = (a.ncols + C - 1) / C;
+    st.lds = (size_t)C * (size_t)rp * e2;
+    st.work = 2.0 * (double)a.ncols * (double)a.f.R * (double)e2;
+    st.desc[var].resize(sizeof(PassArgs));
+    std::memcpy(st.desc[var].data(), &a, sizeof(PassArgs));
+    if (!relocate(st.desc[var].data(), sizeof(PassArgs),
+                  {offsetof(PassArgs, in), offsetof(PassArgs, out), offsetof(PassArgs, x0buf), offsetof(PassArgs, dc)},
+                  qd))
+      return false;
+  } else {
+    ConvArgs a = r.ca;
+    if (!queue_conv_form(prec, r.maxp) || r.threads != a.f.C * a.f.tpc) return false;
+    const int C = threads / a.f.tpc;
+    if (C < 1 || C > kMaxCols || C * a.f.tpc != threads) return false;
+    a.f.C = C;
+    a.f.log2C = ilog2(C);
+    st.kind = 1;
+    st.form = r.maxp;
+    st.nitems = (a.nlines + C - 1) / C;
+    st.lds = (size_t)C * (size_t)a.f.Rp * e2;
+    st.work = 2.0 * (double)a.nlines * (double)a.f.R * (double)e2;
+    st.desc[var].resize(sizeof(ConvArgs));
+    std::memcpy(st.desc[var].data(), &a, sizeof(ConvArgs));
+    if (!relocate(st.desc[var].data(), sizeof(ConvArgs), {offsetof(ConvArgs, in), offsetof(ConvArgs, out)}, qd))
+      return false;
+  }
+  st.rel[var] = qd;
+  return true;
+}
+
+// Builds the plan's work queue; leaves it off (the lanes run every group)
+// when no group fits it. Sets p.q and may raise p.ws_need.
+int build_queue(gofft_plan& p) {
+  auto& Q = p.q;  // (fresh: called once, at plan creation)
+  DevInfo dev;
+  RC_TRY(current_device(&dev));
+  const int prec = p.prec;
+  // Workgroup size and workgroups per CU: complex64 1024 threads, one per CU
+  // (every group of the mixed-length batches fits, the whole-CU fused
+  // convolutions included); complex128 256 threads, two per CU (its kernel
+  // holds 216 registers).
+  const int threads = (int)env_int("GOFFT_QUEUE_THREADS", prec == 0 ? 256 : 1024);
+  const int per_cu = (int)std::max<i64>(1, env_int("GOFFT_QUEUE_WGS", prec == 0 ? 2 : 1));
+  // LDS of a workgroup: a pass stage's columns plus its per-column tables,
+  // or a fused convolution's lines
+  const size_t tables = queue_pass_tables(prec);
+  const i64 budget = (i64)dev.lds_full / per_cu - (i64)queue_static_lds();
+  if (budget <= 0) return GOFFT_OK;
+  const size_t G = p.groups.size();
+  Q.in_queue.assign(G, 0);
+  std::vector<std::vector<QStage>> gst(G);
+  size_t area = 0;  // queued groups' workspace
+  for (size_t g = 0; g < G; g++) {
+    const Engine& e = *p.groups[g]->eng;
+    if (!e.queueable() || e.batch == 0) continue;
+    const size_t off = area;
+    bool ok = true;
+    for (int var = 0; var < 4 && ok; var++) {
+      const int c = var & 1;
+      const bool scaled = var >= 2;
+      std::vector<RecordedLaunch> rec;
+      t_record = &rec;
+      const i64* o = (const i64*)p.groups[g]->off.p;
+      const int rc = e.run(fake_base(1), LineIO{o, 0}, (void*)fake_base(2), LineIO{o, 0}, c, c,
+                           scaled ? 1.0 / (double)p.groups[g]->len : 1.0, Ctx{nullptr, (char*)fake_base(3) + off});
+      t_record = nullptr;
+      if (rc != GOFFT_OK || (int)rec.size() != e.launches() || rec.empty()) {
+        ok = false;
+        break;
+      }
+      if (var == 0) gst[g].resize(rec.size());
+      if (gst[g].size() != rec.size()) ok = false;
+      for (size_t k = 0; k < rec.size() && ok; k++) {
+        QStage& st = gst[g][k];
+        const int kind = st.kind, form = st.form;
+        const i64 ni = st.nitems;
+        ok = queue_stage(prec, threads, rec[k], st, var);
+        if (ok && var > 0) ok = st.kind == kind && st.form == form && st.nitems == ni;
+        if (ok) ok = (i64)(st.kind == 0 ? ((st.lds + 255) & ~(size_t)255) + tables : st.lds) <= budget;
+        st.group = (int)g;
+        st.level = (int)k;
+      }
+    }
+    if (!ok) {
+      gst[g].clear();
+      continue;
+    }
+    Q.in_queue[g] = 1;
+    area += (e.ws_bytes() + 255) & ~(size_t)255;
+  }
+  size_t nq = 0;
+  for (size_t g = 0; g < G; g++) nq += gst[g].size();
+  i64 nitems_all = 0;
+  for (size_t g = 0; g < G; g++)
+    for (const QStage& st : gst[g]) nitems_all += st.nitems;
+  if (nq == 0 || nq > (size_t)kMaxQueueStages || nitems_all >= ((i64)1 << 30)) {
+    Q.in_queue.assign(G, 0);
+    return GOFFT_OK;
+  }
+  // Stage order. Multi-stage groups (chains, multi-pass transforms) take
+  // their first stages at the head of the queue, longest chain first; the
+  // single-stage groups fill the gaps between later levels (split by data
+  // bytes), heaviest items first, so that a dependent stage is reached only
+  // after other work has given its dependency time to finish.
+  std::vector<const QStage*> order;
+  int maxl = 0;
+  for (size_t g = 0; g < G; g++) maxl = std::max(maxl, (int)gst[g].size());
+  std::vector<size_t> multi, single;
+  for (size_t g = 0; g < G; g++) {
+    if (gst[g].size() > 1) multi.push_back(g);
+    if (gst[g].size() == 1) single.push_back(g);
+  }
+  std::stable_sort(multi.begin(), multi.end(), [&](size_t a, size_t b) { return gst[a].size() > gst[b].size(); });
+  auto item_work = [&](size_t g) { return gst[g][0].work / (double)std::max<i64>(1, gst[g][0].nitems); };
+  std::stable_sort(single.begin(), single.end(), [&](size_t a, size_t b) { return item_work(a) > item_work(b); });
+  double fill_total = 0;
+  for (size_t g : single) fill_total += gst[g][0].work;
+  const int gaps = std::max(1, maxl - 1);
+  size_t si = 0;
+  double placed = 0;
+  for (int L = 0; L < std::max(1, maxl); L++) {
+    for (size_t g : multi)
+      if ((int)gst[g].size() > L) order.push_back(&gst[g][L]);
+    if (L < gaps) {
+      const double upto = fill_total * (double)(L + 1) / (double)gaps;
+      while (si < single.size() && (L == gaps - 1 || placed < upto)) {
+        placed += gst[single[si]][0].work;
+        order.push_back(&gst[single[si++]][0]);
+      }
+    }
+  }
+  while (si < single.size()) order.push_back(&gst[single[si++]][0]);
+  // Stage table, dependencies, LDS and the descriptor blocks.
+  std::map<const QStage*, int> pos;
+  for (size_t i = 0; i < order.size(); i++) pos[order[i]] = (int)i;
+  size_t pass_lds = 0, conv_lds = 0;
+  bool any_pass = false;
+  i64 items = 0;
+  for (const QStage* st : order) {
+    if (st->kind == 0) {
+      pass_lds = std::max(pass_lds, st->lds);
+      any_pass = true;
+    } else {
+      conv_lds = std::max(conv_lds, st->lds);
+    }
+    items += st->nitems;
+  }
+  const size_t sh_off = (pass_lds + 255) & ~(size_t)255;
+  Q.threads = threads;
+  Q.lds = (std::max(any_pass ? sh_off + tables : 0, conv_lds) + 15) & ~(size_t)15;
+  const int bpc = queue_blocks_per_cu(prec, threads, Q.lds);
+  if (bpc < 1) {
+    Q.in_queue.assign(G, 0);
+    return GOFFT_OK;
+  }
+  Q.grid = std::min<i64>(items, (i64)bpc * dev.cus);
+  Q.ctr_words = 4 + order.size();  // ticket (2 words), error, pad, done[stage]
+  Q.off_desc = 256 * ((Q.ctr_words * 4 + 255) / 256);
+  size_t desc_bytes = 0;
+  for (int var = 0; var < 4; var++) {
+    auto& V = Q.var[var];
+    std::vector<char> block;
+    std::vector<QueueDesc> table;
+    QueueArgs& A = V.args;
+    A = QueueArgs{};
+    A.nstages = (int)order.size();
+    A.nitems = items;
+    A.spin_limit = env_int("GOFFT_QUEUE_SPIN_TICKS", 200000000);  // 2 s of the 100 MHz clock
+    A.need_extra = (int)env_int("GOFFT_QUEUE_TEST_STALL", 0);
+    A.sh_off = (long long)sh_off;
+    i64 start = 0;
+    for (size_t i = 0; i < order.size(); i++) {
+      const QStage& st = *order[i];
+      const size_t off = (block.size() + 63) & ~(size_t)63;
+      block.resize(off + st.desc[var].size());
+      std::memcpy(block.data() + off, st.desc[var].data(), st.desc[var].size());
+      // (a workspace relocation's offset already holds its group's offset:
+      // the group ran on the stand-in base plus that offset)
+      QueueDesc qd = st.rel[var];
+      qd.off = (long long)off;
+      qd.bytes = (long long)st.desc[var].size();
+      table.push_back(qd);
+      QueueStage& qs = A.st[i];
+      qs.start = start;
+      qs.nitems = st.nitems;
+      qs.desc = (long long)off;
+      qs.kind = st.kind;
+      qs.form = st.form;
+      qs.dep = st.level > 0 ? pos[&gst[st.group][st.level - 1]] : -1;
+      qs.signal = st.level + 1 < (int)gst[st.group].size() ? 1 : 0;
+      start += st.nitems;
+    }
+    desc_bytes = std::max(desc_bytes, block.size());
+    RC_TRY(V.tmpl.upload(block.data(), block.size()));
+    RC_TRY(V.table.upload(table.data(), table.size() * sizeof(QueueDesc)));
+    V.ndesc = (int)table.size();
+  }
+  Q.off_groups = Q.off_desc + 256 * ((desc_bytes + 255) / 256);
+  p.ws_need = std::max(p.ws_need, Q.off_groups + area);
+  Q.check = env_int("GOFFT_QUEUE_CHECK", 0) != 0;
+  Q.on = true;
+  int ng = 0;
+  for (size_t g = 0; g < G; g++) ng += Q.in_queue[g];
+  Q.text = "queue threads=" + std::to_string(threads) + " lds=" + std::to_string(Q.lds) +
+           " grid=" + std::to_string(Q.grid) + " (" + std::to_string(bpc) + "/CU) stages=" +
+           std::to_string(order.size()) + " items=" + std::to_string(items) + " groups=" + std::to_string(ng) + "/" +
+           std::to_string(G) + "\n  order:";
+  // stage list: length.launch (items, LDS KiB) and "<k" for a wait on stage k
+  for (size_t i = 0; i < order.size(); i++) {
+    const QStage& st = *order[i];
+    const QueueStage& qs = Q.var[0].args.st[i];
+    Q.text += " " + std::to_string(p.groups[st.group]->len) + "." + std::to_string(st.level) + "(" +
+              std::to_string(st.nitems) + "," + std::to_string((st.lds + 1023) / 1024) + "K" +
+              (qs.dep >= 0 ? ",<" + std::to_string(qs.dep) : std::string()) + ")";
+  }
   return GOFFT_OK;
 }
 
@@ -2595,6 +2979,44 @@ int exec_ws(const gofft_plan* p, hipStream_t s, const WsSet* w, char** out) {
 
 // Runs every group of a mixed-length plan: on the caller's stream alone, or
 // forked over the plan's lanes and joined back.
+// The queued groups of a mixed-length plan on stream s: the setup kernel
+// resolves the variant's descriptors into the workspace and zeroes the
+// counters, then the queue kernel runs every stage.
+int run_queue(const gofft_plan* p, const void* src, void* dst, int c, bool scaled, hipStream_t s,
+              const WsSet* w) {
+  const auto& Q = p->q;
+  char* ws = nullptr;
+  RC_TRY(exec_ws(p, s, w, &ws));
+  const auto& V = Q.var[(c ? 1 : 0) + (scaled ? 2 : 0)];
+  QueueSetupArgs sa{};
+  sa.tmpl = (const char*)V.tmpl.p;
+  sa.descs = ws + Q.off_desc;
+  sa.d = (const QueueDesc*)V.table.p;
+  sa.base[1] = (const char*)src;
+  sa.base[2] = (const char*)dst;
+  sa.base[3] = ws + Q.off_groups;
+  sa.ctr = (unsigned*)ws;
+  sa.nctr = (int)Q.ctr_words;
+  HIP_TRY(launch_queue_setup(sa, V.ndesc, s));
+  QueueArgs a = V.args;
+  a.descs = ws + Q.off_desc;
+  a.ticket = (unsigned long long*)ws;
+  a.error = (unsigned*)(ws + 8);
+  a.done = (unsigned*)(ws + 16);
+  HIP_TRY(launch_queue(p->prec, a, Q.threads, Q.lds, Q.grid, s));
+  if (Q.check) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(s, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+      unsigned err = 0;
+      HIP_TRY(hipStreamSynchronize(s));
+      HIP_TRY(hipMemcpy(&err, a.error, sizeof(err), hipMemcpyDeviceToHost));
+      if (err) return fail(GOFFT_ERR_HIP, "work queue: a wait for an earlier stage exceeded its time limit");
+    }
+  }
+  return GOFFT_OK;
+}
+
 int run_groups(const gofft_plan* p, const void* src, void* dst, int ci, int co, bool inverse_scale,
                hipStream_t s, const WsSet* w) {
   auto one = [&](const gofft_plan::Group& g, hipStream_t st, int lane) -> int {
@@ -2607,14 +3029,20 @@ int run_groups(const gofft_plan* p, const void* src, void* dst, int ci, int co, 
     const i64* o = (const i64*)g.off.p;
     return g.eng->run(src, LineIO{o, 0}, dst, LineIO{o, 0}, ci, co, inverse_scale ? 1.0 / (double)g.len : 1.0, c);
   };
+  const bool queue = p->q.on;
+  auto queued = [&](size_t g) { return queue && p->q.in_queue[g]; };
   if (p->lanes.empty()) {
-    for (size_t g : p->order) RC_TRY(one(*p->groups[g], s, 0));
+    for (size_t g : p->order)
+      if (!queued(g)) RC_TRY(one(*p->groups[g], s, 0));
+    if (queue) RC_TRY(run_queue(p, src, dst, ci, inverse_scale, s, w));
     return GOFFT_OK;
   }
   std::lock_guard<std::mutex> lk(p->lane_mu);
   HIP_TRY(hipEventRecord(p->fork_ev, s));
   for (hipStream_t l : p->lanes) HIP_TRY(hipStreamWaitEvent(l, p->fork_ev, 0));
+  if (queue) RC_TRY(run_queue(p, src, dst, ci, inverse_scale, s, w));
   for (size_t g : p->order) {
+    if (queued(g)) continue;
     const int k = p->lane_of[g];
     RC_TRY(one(*p->groups[g], k == 0 ? s : p->lanes[k - 1], k));
   }
@@ -2640,6 +3068,8 @@ int check_dtype(int dtype) {
 // unless accurate twiddles are requested.
 unsigned default_flags(int dtype, unsigned flags) {
   if (dtype == GOFFT_C128 && !(flags & GOFFT_FLAG_ACCURATE_TWIDDLES)) flags |= GOFFT_FLAG_REFERENCE_TWIDDLES;
+  if (const char* e = std::getenv("GOFFT_QUEUE"))
+    if (std::atoi(e) != 0) flags |= GOFFT_FLAG_QUEUE;
   return flags;
 }
 
@@ -2830,6 +3260,7 @@ int gofft_plan_c2c_many(gofft_plan** plan, const int64_t* lengths, int64_t count
     p->ws_need = std::max(p->ws_need, g->eng->ws_bytes());
     p->groups.push_back(std::move(g));
   }
+  if (p->flags & GOFFT_FLAG_QUEUE) RC_TRY(build_queue(*p));
   RC_TRY(assign_lanes(*p));
   return new_plan(plan, p);
 }
@@ -2995,8 +3426,9 @@ int gofft_plan_num_launches(const gofft_plan* p, int direction) {
   switch (p->kind) {
     case K_C2C: return p->eng->launches();
     case K_MANY: {
-      int k = 0;
-      for (auto& g : p->groups) k += g->eng->launches();
+      int k = p->q.on ? 2 : 0;  // the queue's setup and queue kernels
+      for (size_t g = 0; g < p->groups.size(); g++)
+        if (!p->q.on || !p->q.in_queue[g]) k += p->groups[g]->eng->launches();
       return k;
     }
     case K_R2C:
@@ -3023,7 +3455,9 @@ int64_t gofft_plan_describe(const gofft_plan* p, char* buf, int64_t len) {
   switch (p->kind) {
     case K_C2C: line(p->eng->describe()); break;
     case K_MANY:
-      for (const auto& g : p->groups) line(g->eng->describe());
+      if (p->q.on) line(p->q.text);
+      for (size_t g = 0; g < p->groups.size(); g++)
+        line(std::string(p->q.on && p->q.in_queue[g] ? "queued: " : "") + p->groups[g]->eng->describe());
       break;
     case K_R2C:
     case K_PSD:

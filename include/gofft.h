@@ -85,6 +85,17 @@ extern "C" {
  * first and last passes; convolution chains keep the default. */
 #define GOFFT_FLAG_TWIDDLES_FP32 8u
 #define GOFFT_FLAG_TWIDDLES_SPLIT 16u
+/* Mixed-length plans (gofft_plan_c2c_many): run the length groups as the
+ * stages of one persistent kernel over a device work queue instead of
+ * launches on several streams. Each group's launches become stages, re-planned
+ * to one workgroup size; workgroups take column or line groups in stage order
+ * from a device counter, and a stage that needs an earlier one's results waits
+ * (bounded in time) for all of its items. Groups the queue kernel cannot run
+ * (twiddle-form flags, workgroups beyond its LDS share) stay on the streams
+ * beside it. Same results as without the flag, bit for bit. The env var
+ * GOFFT_QUEUE=1 sets it for every plan. No reference counterpart (the
+ * reference runs transforms one by one). */
+#define GOFFT_FLAG_QUEUE 32u
 
 /* N-D kinds */
 #define GOFFT_ND_C2C 0

@@ -17,6 +17,7 @@
 #include <mutex>
 #include <string>
 
+#include "gofft.h"
 #include "kernels.h"
 
 namespace {
@@ -27,6 +28,17 @@ long g_hostallocs = 0, g_fail_host = 0;
 long g_live = 0;                        // live device allocations
 std::map<std::string, long> g_counts;   // launches per launcher
 std::string g_bad;                      // first launch that broke a kernel assumption
+std::map<uintptr_t, size_t> g_regions;  // "device" memory: allocations, the scratch symbol, registered buffers
+
+// Whether [p, p + bytes) lies in one region.
+bool in_region(const void* p, size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const uintptr_t x = (uintptr_t)p;
+  auto it = g_regions.upper_bound(x);
+  if (it == g_regions.begin()) return false;
+  --it;
+  return x >= it->first && x + bytes <= it->first + it->second;
+}
 int g_token = 0;
 
 void* token() { return (void*)(uintptr_t)(0x1000 + 16 * (++g_token)); }
@@ -98,6 +110,14 @@ void hipsim_reset_counts() {
 long hipsim_allocs() { return g_allocs; }
 long hipsim_launches() { return g_launches; }
 long hipsim_live() { return g_live; }
+void hipsim_region(void* p, size_t n) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_regions[(uintptr_t)p] = n;
+}
+void hipsim_unregion(void* p) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_regions.erase((uintptr_t)p);
+}
 const char* hipsim_bad() { return g_bad.c_str(); }
 }
 
@@ -117,6 +137,7 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
 }
 hipError_t hipGetSymbolAddress(void** out, const void* sym) {
   *out = const_cast<void*>(sym);
+  hipsim_region(*out, (size_t)GOFFT_NOALLOC_SCRATCH_BYTES);  // the library's only device symbol
   return hipSuccess;
 }
 hipError_t hipMalloc(void** p, size_t n) {
@@ -130,10 +151,12 @@ hipError_t hipMalloc(void** p, size_t n) {
     g_live++;
   }
   *p = aligned_alloc(256, (n + 255) / 256 * 256);
+  if (*p) hipsim_region(*p, n);
   return *p ? hipSuccess : hipErrorOutOfMemory;
 }
 hipError_t hipFree(void* p) {
   if (p) {
+    hipsim_unregion(p);
     std::lock_guard<std::mutex> lk(g_mu);
     g_live--;
   }
@@ -194,6 +217,10 @@ const char* hipGetErrorString(hipError_t e) {
   return e == hipErrorOutOfMemory ? "hipErrorOutOfMemory" : e == hipErrorLaunchFailure ? "hipErrorLaunchFailure" : "hipError";
 }
 hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipSuccess; }
+hipError_t hipStreamIsCapturing(hipStream_t, hipStreamCaptureStatus* st) {
+  *st = hipStreamCaptureStatusNone;
+  return hipSuccess;
+}
 hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
   *g = (hipGraph_t)token();
   return hipSuccess;
@@ -231,6 +258,108 @@ hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipSt
   if (maxp != 16 && !(maxp == 24 && prec == 1)) bad("k_conv: unsupported points per thread");
   if (a.nlines <= 0) bad("k_conv: no lines");
   return launch("conv");
+}
+// Mirror of kernels.hip pass_form (the form k_pass would run).
+int pass_form(int prec, const PassArgs& a) {
+  if (a.real_last || a.real_first) return a.real_last ? 11 : 12;
+  const bool ext = a.in_hook || a.out_hook || a.mid_spec;
+  int mode = a.mid256 && prec != 0 ? 2 : ext ? 1 : 0;
+  const SubFFT& f = a.f;
+  const int A = f.st[0].radix;
+  if (mode <= 1 && f.nstages == 2 && f.tpc == 16 && f.st[1].radix == 16 && f.R == 16 * A && f.st[0].cmul == 0 &&
+      f.st[1].cmul == 0 && !a.mid_spec) {
+    const bool hooks = a.in_hook || a.out_hook;
+    const bool rader = a.in_hook == 2 || a.out_hook == 2;
+    if (hooks && prec != 0 && !rader) {
+      if (A == 10) mode = 6;
+      if (A == 12) mode = 7;
+      if (A == 16) mode = 5;
+    } else if (!hooks || prec == 0) {
+      if (A == 10) mode = 3;
+      if (A == 12) mode = 4;
+    }
+  }
+  return mode;
+}
+unsigned queue_pass_forms(int prec) { return prec == 0 ? 0x1bu : 0xffu; }
+bool queue_conv_form(int prec, int maxp) { return maxp == 16 || (prec != 0 && maxp == 24); }
+size_t queue_pass_tables(int prec) { return (size_t)kMaxCols * (7 * 8 + (prec == 0 ? 16 : 8)); }
+size_t queue_static_lds() { return 64; }
+// LDS and waves per CU (2048 threads); complex128's queue kernel holds two
+// 256-thread or one 512-thread workgroup per CU by its registers.
+int queue_blocks_per_cu(int prec, int threads, size_t lds) {
+  const long by_lds = kLds / (long)(lds + queue_static_lds());
+  long by_regs = 2048 / threads;
+  if (prec == 0) by_regs = threads == 256 ? 2 : 1;
+  return (int)(by_lds < by_regs ? by_lds : by_regs);
+}
+// The setup kernel's work, done on the host ("device" memory is host memory).
+hipError_t launch_queue_setup(const QueueSetupArgs& a, int ndesc, hipStream_t) {
+  if (ndesc < 1 || !a.tmpl || !a.descs || !a.d || !a.ctr || a.nctr < 4) bad("k_queue_setup: arguments");
+  for (int i = 0; i < ndesc; i++) {
+    const QueueDesc& d = a.d[i];
+    if (d.off % 8 || d.bytes % 8 || d.nreloc < 0 || d.nreloc > 4) bad("k_queue_setup: descriptor layout");
+    memcpy(a.descs + d.off, a.tmpl + d.off, (size_t)d.bytes);
+    for (int k = 0; k < d.nreloc; k++) {
+      if (d.sel[k] < 1 || d.sel[k] > 3 || d.field[k] < 0 || d.field[k] + 8 > d.bytes) bad("k_queue_setup: relocation");
+      const char* v = a.base[d.sel[k]] + d.add[k];
+      memcpy(a.descs + d.off + d.field[k], &v, sizeof v);
+    }
+  }
+  for (int i = 0; i < a.nctr; i++) a.ctr[i] = 0;
+  return launch("queue_setup");
+}
+// A pointer the queue resolved: set, and no stand-in left (those sit at or
+// above 1 << 60).
+static bool resolved(const void* p) { return p && (uintptr_t)p < ((uintptr_t)1 << 56); }
+hipError_t launch_queue(int prec, const QueueArgs& a, int threads, size_t lds, long long grid, hipStream_t) {
+  char b[256];
+  const size_t e = prec == 0 ? 16 : 8;
+  if (a.nstages < 1 || a.nstages > kMaxQueueStages || grid < 1 || !a.ticket || !a.error || !a.done ||
+      a.need_extra != 0 || a.spin_limit <= 0)
+    bad("k_queue: arguments");
+  if (queue_blocks_per_cu(prec, threads, lds) < 1) bad("k_queue: workgroup does not fit a CU");
+  if (!in_region(a.ticket, 8) || !in_region(a.error, 4) || !in_region(a.done, 4 * (size_t)a.nstages))
+    bad("k_queue: counters outside device memory");
+  long long start = 0;
+  for (int i = 0; i < a.nstages; i++) {
+    const QueueStage& st = a.st[i];
+    snprintf(b, sizeof b, "k_queue stage %d", i);
+    if (st.start != start || st.nitems < 1 || st.dep >= i || st.dep < -1 || (st.dep >= 0 && !a.st[st.dep].signal)) {
+      bad(std::string(b) + ": table (start, items or dependency)");
+      return launch("queue");
+    }
+    start += st.nitems;
+    if (st.kind == 0) {
+      const PassArgs& p = *reinterpret_cast<const PassArgs*>(a.descs + st.desc);
+      // the columns end before the per-column tables, which end within the LDS
+      check_sub(b, p.f, threads, 16, (long)a.sh_off, e);
+      if (p.mid_spec) check_sub(b, p.f2, threads, 16, (long)a.sh_off, e);
+      if (a.sh_off % 256 || a.sh_off + (long long)queue_pass_tables(prec) > (long long)lds) bad(std::string(b) + ": tables");
+      if (pass_form(prec, p) != st.form || !((queue_pass_forms(prec) >> st.form) & 1u)) bad(std::string(b) + ": form");
+      if ((p.ncols + p.f.C - 1) / p.f.C != st.nitems) bad(std::string(b) + ": items");
+      if (!resolved(p.in) || !resolved(p.out)) bad(std::string(b) + ": data pointers");
+      if ((p.in_hook == 2 || p.out_hook == 2) && !resolved(p.x0buf)) bad(std::string(b) + ": x0 slots");
+      if (p.dc && !resolved(p.dc)) bad(std::string(b) + ": DC slots");
+      // every pointer inside device memory; lines without an offset table
+      // (workspace buffers) whole: ncols columns of R points
+      const size_t span = (size_t)p.ncols * (size_t)p.f.R * e;
+      if (!in_region(p.in, p.in_off || p.in_hook ? 1 : span)) bad(std::string(b) + ": input outside device memory");
+      if (!in_region(p.out, p.out_off || p.out_hook ? 1 : span)) bad(std::string(b) + ": output outside device memory");
+      if (p.x0buf && !in_region(p.x0buf, e)) bad(std::string(b) + ": x0 slots outside device memory");
+      if (p.dc && !in_region(p.dc, e)) bad(std::string(b) + ": DC slots outside device memory");
+      if (p.xcd_group != 1) bad(std::string(b) + ": XCD grouping in the queue");
+    } else {
+      const ConvArgs& c = *reinterpret_cast<const ConvArgs*>(a.descs + st.desc);
+      check_sub(b, c.f, threads, st.form, (long)lds, e);
+      if (!queue_conv_form(prec, st.form)) bad(std::string(b) + ": points per thread");
+      if ((c.nlines + c.f.C - 1) / c.f.C != st.nitems) bad(std::string(b) + ": items");
+      if (!resolved(c.in) || !resolved(c.out)) bad(std::string(b) + ": data pointers");
+      if (!in_region(c.in, 1) || !in_region(c.out, 1)) bad(std::string(b) + ": data outside device memory");
+    }
+  }
+  if (start != a.nitems) bad("k_queue: item count");
+  return launch("queue");
 }
 int real_kernel_vgprs(int prec) { return prec == 0 ? 126 : 84; }
 long long residual_parts(long long per) {

@@ -23,6 +23,8 @@ long hipsim_allocs();
 long hipsim_launches();
 long hipsim_live();
 const char* hipsim_bad();
+void hipsim_region(void* p, size_t n);
+void hipsim_unregion(void* p);
 }
 
 static int g_failures = 0;
@@ -41,8 +43,14 @@ static std::string err() { return gofft_last_error(); }
 // A "device" buffer of the simulated runtime (host memory).
 struct Buf {
   void* p = nullptr;
-  explicit Buf(size_t bytes) { p = std::calloc(1, bytes + 64); }
-  ~Buf() { std::free(p); }
+  explicit Buf(size_t bytes) {
+    p = std::calloc(1, bytes + 64);
+    hipsim_region(p, bytes + 64);
+  }
+  ~Buf() {
+    hipsim_unregion(p);
+    std::free(p);
+  }
 };
 
 static size_t elt(int prec) { return prec == GOFFT_C128 ? 16 : 8; }
@@ -252,11 +260,31 @@ static void launch_failures() {
       const int rc = gofft_exec_c2c(p, a.p, b.p, GOFFT_FORWARD, (void*)0x30);
       hipsim_fail_launch(0);
       if (rc != GOFFT_ERR_HIP) w = "launch " + std::to_string(k) + ": status " + std::to_string(rc);
-      else if (err().find("launch_") == std::string::npos || err().find("hipErrorLaunchFailure") == std::string::npos)
+      else if ((err().find("launch_") == std::string::npos && err().find("issue_") == std::string::npos) ||
+               err().find("hipErrorLaunchFailure") == std::string::npos)
         w = "message '" + err() + "'";
     }
     gofft_plan_destroy(p);
     report("launch failure n=" + std::to_string(n) + " (" + std::to_string(launches) + " launches)", w);
+  }
+  // a work-queue plan: its setup and queue launches
+  for (int prec : {GOFFT_C128, GOFFT_C64}) {
+    std::vector<int64_t> mix = {1009, 20014, 65536, 4096, 10007};
+    gofft_plan* p = nullptr;
+    gofft_plan_c2c_many(&p, mix.data(), (int64_t)mix.size(), prec, GOFFT_FLAG_QUEUE);
+    Buf a(200000 * 16), b(200000 * 16);
+    const int launches = gofft_plan_num_launches(p, 0);
+    std::string w;
+    for (int k = 1; k <= launches && w.empty(); k++) {
+      hipsim_reset_counts();
+      hipsim_fail_launch(k);
+      const int rc = gofft_exec_c2c(p, a.p, b.p, GOFFT_FORWARD, (void*)0x30);
+      hipsim_fail_launch(0);
+      if (rc != GOFFT_ERR_HIP) w = "launch " + std::to_string(k) + ": status " + std::to_string(rc);
+      else if (err().find("hipErrorLaunchFailure") == std::string::npos) w = "message '" + err() + "'";
+    }
+    gofft_plan_destroy(p);
+    report("launch failure queue prec=" + std::to_string(prec) + " (" + std::to_string(launches) + " launches)", w);
   }
 }
 
@@ -362,6 +390,75 @@ static void index_tables() {
   report("index tables", w);
 }
 
+// Mixed-length plans with the work queue: every op (the queue launchers of
+// the stand-in resolve each exec's descriptors and check every stage), the
+// queue's launch count and plan text, groups the queue cannot take (complex64
+// twiddle-form flags) staying on the lanes, and every allocation of the
+// plan's creation failing in turn without a leak.
+static void queue_plans() {
+  std::vector<int64_t> mix;
+  for (int r = 0; r < 40; r++)
+    for (int64_t n : {1024, 4096, 65536, 6720, 1009, 10007, 20014}) mix.push_back(n);
+  mix.push_back(600011);
+  mix.push_back(17);
+  mix.push_back(1);
+  mix.push_back(0);
+  for (int prec : {GOFFT_C128, GOFFT_C64})
+    for (unsigned extra : {0u, (unsigned)GOFFT_FLAG_TWIDDLES_FP32}) {
+      if (extra && prec == GOFFT_C128) continue;
+      const std::string name = "queue prec=" + std::to_string(prec) + (extra ? " fp32 twiddles" : "");
+      gofft_plan* p = nullptr;
+      if (gofft_plan_c2c_many(&p, mix.data(), (int64_t)mix.size(), prec, GOFFT_FLAG_QUEUE | extra) != GOFFT_OK) {
+        report(name, "plan: " + err());
+        continue;
+      }
+      std::vector<char> text(1 << 16);
+      gofft_plan_describe(p, text.data(), (int64_t)text.size());
+      const std::string t(text.data());
+      std::string w;
+      if (t.find("queue threads=") == std::string::npos || t.find("queued: ") == std::string::npos)
+        w = "no queue in the plan: " + t;
+      // the queue takes the Stockham, Rader and Bluestein groups; with FP32
+      // twiddles the multi-pass groups (65536, 600011) stay on the lanes
+      if (w.empty() && extra && t.find("queued: stockham n=65536") != std::string::npos) w = "65536 queued with FP32 twiddles";
+      if (w.empty() && !extra && t.find("queued: stockham n=65536") == std::string::npos) w = "65536 not queued";
+      if (w.empty() && t.find("trivial n=1") != std::string::npos && t.find("queued: trivial") != std::string::npos)
+        w = "trivial group queued";
+      if (w.empty()) {
+        hipsim_reset_counts();
+        void* s = (void*)0x10;
+        Buf a(sizeof(double) * 2 * 8000000), b(sizeof(double) * 2 * 8000000);
+        if (gofft_exec_c2c(p, a.p, b.p, GOFFT_FORWARD, s) != GOFFT_OK) w = "exec: " + err();
+        else if (hipsim_launches() != gofft_plan_num_launches(p, GOFFT_FORWARD))
+          w = "launches " + std::to_string(hipsim_launches()) + " != " + std::to_string(gofft_plan_num_launches(p, 0));
+      }
+      if (w.empty()) w = exercise(p, 0, prec);
+      gofft_plan_destroy(p);
+      report(name, w);
+    }
+  for (int prec : {GOFFT_C128, GOFFT_C64}) {
+    std::vector<int64_t> small = {1009, 20014, 65536, 4096, 1009, 10007};
+    hipsim_reset_counts();
+    gofft_plan* p = nullptr;
+    gofft_plan_c2c_many(&p, small.data(), (int64_t)small.size(), prec, GOFFT_FLAG_QUEUE);
+    if (p) gofft_plan_destroy(p);
+    const long total = hipsim_allocs();
+    std::string w;
+    for (long k = 1; k <= total && w.empty(); k++) {
+      hipsim_reset_counts();
+      const long live0 = hipsim_live();
+      hipsim_fail_alloc(k);
+      p = nullptr;
+      const int rc = gofft_plan_c2c_many(&p, small.data(), (int64_t)small.size(), prec, GOFFT_FLAG_QUEUE);
+      hipsim_fail_alloc(0);
+      if (rc != GOFFT_ERR_OOM || p) w = "allocation " + std::to_string(k) + ": status " + std::to_string(rc);
+      else if (hipsim_live() != live0) w = "allocation " + std::to_string(k) + ": leaked";
+      if (p) gofft_plan_destroy(p);
+    }
+    report("queue oom prec=" + std::to_string(prec) + " (" + std::to_string(total) + " allocations)", w);
+  }
+}
+
 int main() {
   gofft_init();
   sweep();
@@ -370,6 +467,7 @@ int main() {
   corrupt_tune_cache();
   argument_errors();
   index_tables();
+  queue_plans();
   std::printf("%s: %d failed\n", g_failures ? "FAILED" : "PASSED", g_failures);
   return g_failures ? 1 : 0;
 }
