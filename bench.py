@@ -608,18 +608,24 @@ def _mean(rows):
     return sum(float(r["Counter_Value"]) for r in rows) / len(rows), len(rows)
 
 
+def _calibration(rows, path):
+    """Mean counter value of the 1 GiB device copy's dispatches: the copy
+    dispatches that read (write) the most, within 10% of the largest (other
+    copies of the run are smaller, some with larger grids)."""
+    copies = [r for r in rows if "copy" in r["Kernel_Name"].lower()]
+    if not copies:
+        raise SystemExit("no copy dispatch in %s (run the workload with --calibrate)" % path)
+    top = max(float(r["Counter_Value"]) for r in copies)
+    return _mean([r for r in copies if float(r["Counter_Value"]) >= 0.9 * top])
+
+
 def summarize_pmc(fetch_csv, write_csv):
     """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the pass kernel, each
-    scaled by what the counter reads for the 1 GiB device copy of the same run
-    (the copy dispatches with the largest grid)."""
+    scaled by what the counter reads for the 1 GiB device copy of the same run."""
     out = {}
     for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         rows = _pmc_rows(path, name)
-        copies = [r for r in rows if "copy" in r["Kernel_Name"].lower()]
-        if not copies:
-            raise SystemExit("no copy dispatch in %s" % path)
-        g = max(int(r["Grid_Size"]) for r in copies)
-        cal, ncal = _mean([r for r in copies if int(r["Grid_Size"]) == g])
+        cal, ncal = _calibration(rows, path)
         kern = [r for r in rows if "k_pass<float" in r["Kernel_Name"]]
         if not kern:
             raise SystemExit("no pass-kernel dispatch in %s" % path)
@@ -631,6 +637,50 @@ def summarize_pmc(fetch_csv, write_csv):
         print("%s: %d pass-kernel dispatches, mean %.1f KiB raw; 1 GiB copy reads %.1f KiB "
               "(%d dispatches) -> factor %.4f -> %.0f bytes per launch" % (name, nk, raw, cal, ncal, factor, b))
     print("traffic_bytes_per_launch: %.0f" % (out["FETCH_SIZE"][5] + out["WRITE_SIZE"][5]))
+
+
+def _kernel_key(name):
+    """A kernel's name without its argument list (template arguments kept)."""
+    name = name.strip()
+    if name.endswith(")") and "(" in name:
+        depth = 0
+        for i in range(len(name) - 1, -1, -1):
+            depth += {")": 1, "(": -1}.get(name[i], 0)
+            if depth == 0:
+                return name[:i].replace("void ", "", 1).strip()
+    return name
+
+
+def summarize_kernels(stats_csv, fetch_csv=None, write_csv=None):
+    """Per kernel of a profiled run: dispatches and average duration
+    (rocprofv3 --kernel-trace --stats), and with the two counter runs the
+    HBM bytes per dispatch, FETCH_SIZE and WRITE_SIZE each scaled by what the
+    counter reads for the run's 1 GiB device copy (bench.py --calibrate)."""
+    rows = list(csv.DictReader(open(stats_csv)))
+    traffic = {}
+    factors = {}
+    for cname, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        if not path:
+            continue
+        prow = _pmc_rows(path, cname)
+        cal, _ = _calibration(prow, path)
+        factors[cname] = COPY_BYTES / (cal * 1024.0)
+        per = {}
+        for r in prow:
+            per.setdefault(_kernel_key(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+        for k, v in per.items():
+            traffic.setdefault(k, {})[cname] = sum(v) / len(v) * 1024.0 * factors[cname]
+    if factors:
+        print("calibration (1 GiB copy): " + ", ".join("%s factor %.4f" % kv for kv in factors.items()))
+    print("%-58s %6s %10s %10s %14s %14s" % ("kernel", "calls", "total_ms", "avg_us", "fetch_B/disp", "write_B/disp"))
+    for r in rows:
+        k = _kernel_key(r["Name"])
+        t = traffic.get(k, {})
+        name = k if len(k) <= 58 else k[:55] + "..."
+        f = "%14.0f" % t["FETCH_SIZE"] if "FETCH_SIZE" in t else "%14s" % "-"
+        w = "%14.0f" % t["WRITE_SIZE"] if "WRITE_SIZE" in t else "%14s" % "-"
+        print("%-58s %6s %10.3f %10.2f %s %s" % (name, r["Calls"], float(r["TotalDurationNs"]) / 1e6,
+                                                float(r["AverageNs"]) / 1e3, f, w))
 
 
 def main():
@@ -645,7 +695,12 @@ def main():
     ap.add_argument("--precisions", default="c64,c128", help="groups workload: c64 and/or c128")
     ap.add_argument("--summarize-trace", nargs="+", metavar="STATS_CSV [SKIP TAKE]")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+    ap.add_argument("--summarize-kernels", nargs="+", metavar="STATS_CSV [FETCH_CSV WRITE_CSV]")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="other workloads: also run the 1 GiB device copy that counter summaries scale by")
     args = ap.parse_args()
+    if args.summarize_kernels:
+        return summarize_kernels(*args.summarize_kernels[:3])
     if args.summarize_trace:
         t = args.summarize_trace
         return summarize_trace(t[0], *(int(v) for v in t[1:3])) if len(t) >= 3 else summarize_trace(t[0])
@@ -677,6 +732,8 @@ def main():
                 fn(D, torch, dev, args.steps, warm, lens, args.precisions.split(","))
             else:
                 fn(D, torch, dev, args.steps, warm)
+        if args.calibrate:
+            print(json.dumps({"workload": "copy_1GiB", **copy_bandwidth(torch, dev, 3, 1)}), flush=True)
         return
     lengths = workload_lengths(0)
     total = sum(lengths)

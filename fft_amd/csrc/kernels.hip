@@ -1554,21 +1554,27 @@ __device__ __forceinline__ void conv_body(const ConvArgs& a, long long blk) {
       }
     }
   } else {
+    // X[p] = x0 + conv[k + off] for the k with iperm[k] = p (p = 1..q),
+    // written in output order: each thread gathers its outputs' entries
+    // from LDS through oinv[p - 1] = k, so the stores to the line are
+    // contiguous rather than scattered by the permutation (which left
+    // partly written cache lines and about 3.4x the line's bytes of HBM
+    // writes).
     const int off = a.kind == 1 ? q : 0;
-    for (int k0 = lt; k0 < q && live; k0 += SB * tpc) {
-      int ip[SB];
+    for (int p0 = lt; p0 < q && live; p0 += SB * tpc) {
+      int kk[SB];
 #pragma unroll
       for (int b = 0; b < SB; ++b) {
-        const int k = k0 + b * tpc;
-        ip[b] = k < q ? a.iperm[k] : 0;
+        const int p = p0 + b * tpc;
+        kk[b] = p < q ? a.oinv[p] : 0;
       }
 #pragma unroll
       for (int b = 0; b < SB; ++b) {
-        const int k = k0 + b * tpc;
-        if (k < q) {
-          T2 v = cadd(x0, cconj(lds[c * Rp + lpm<MAXP>(k + off, f.psh)]));
+        const int p = p0 + b * tpc;
+        if (p < q) {
+          T2 v = cadd(x0, cconj(lds[c * Rp + lpm<MAXP>(kk[b] + off, f.psh)]));
           if (a.conj_out) v.y = -v.y;
-          out[ip[b]] = cscale(v, sc);
+          out[p + 1] = cscale(v, sc);
         }
       }
     }

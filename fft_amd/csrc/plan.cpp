@@ -1673,7 +1673,7 @@ class RaderEngine : public Engine {
  public:
   i64 q = 0, cl = 0, g = 0;  // g: the primitive root whose powers order the gather
   bool cyclic = true;
-  DevBuf perm, iperm, spec;
+  DevBuf perm, iperm, oinv, spec;
   // fused path
   bool fused = false;
   bool big_lds = false;  // its workgroup takes more than half a CU's LDS
@@ -1719,6 +1719,11 @@ class RaderEngine : public Engine {
       fused = true;
       big_lds = 2 * (i64)sp.f.C * sp.f.Rp * (i64)elt(prec) > dev.lds_full;
       RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
+      // the fused kernel writes X[p], p = 1..q, in order: oinv[p - 1] is the
+      // k with iperm[k] = p
+      std::vector<int> oi(q);
+      for (i64 k = 0; k < q; k++) oi[iph[k] - 1] = (int)k;
+      RC_TRY(oinv.upload(oi.data(), oi.size() * sizeof(int)));
     } else {
       cb = chunk_lines(2 * cl * (i64)elt(prec), batch);
       RC_TRY(chain.init(cl, cb, prec, false, force_cols));
@@ -1754,7 +1759,7 @@ class RaderEngine : public Engine {
       a.stw = stw.p;
       a.spec = spec.p;
       a.perm = (const int*)perm.p;
-      a.iperm = (const int*)iperm.p;
+      a.oinv = (const int*)oinv.p;
       a.in_stride = li.stride;
       a.out_stride = lo.stride;
       a.n = (int)n;

@@ -257,6 +257,7 @@ hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipSt
   check_sub("k_conv", a.f, threads, maxp, kLds, prec == 0 ? 16 : 8);
   if (maxp != 16 && !(maxp == 24 && prec == 1)) bad("k_conv: unsupported points per thread");
   if (a.nlines <= 0) bad("k_conv: no lines");
+  if (a.kind != 2 && (!a.perm || !a.oinv)) bad("k_conv: Rader without its gather or output-order table");
   return launch("conv");
 }
 // Mirror of kernels.hip pass_form (the form k_pass would run).
