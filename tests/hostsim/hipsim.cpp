@@ -88,6 +88,21 @@ void check_sub(const char* k, const gofft::SubFFT& f, int threads, int maxp, lon
       snprintf(b, sizeof b, "%s: stage %d radix %d ns %d nb %d (R=%d)", k, i, r, f.st[i].ns, f.st[i].nb, f.R);
       return bad(b);
     }
+    // kernels.hip lds_fft: the radices each form compiles, and every
+    // butterfly j < nb reached by j = lt + m * tpc, m < max(maxp / radix, 1)
+    if (f.st[i].radix < 100) {
+      static const int k16[] = {2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 13, 16};
+      static const int k24[] = {2, 3, 4, 5, 7, 8, 11, 13, 16};
+      bool known = false;
+      for (int x : k16) known |= maxp == 16 && x == r;
+      for (int x : k24) known |= maxp == 24 && x == r;
+      const int cap = maxp / r > 0 ? maxp / r : 1;
+      if (!known || (f.st[i].nb + f.tpc - 1) / f.tpc > cap) {
+        snprintf(b, sizeof b, "%s: stage %d radix %d (%d butterflies, %d threads per column) not run by the %d-point form",
+                 k, i, r, f.st[i].nb, f.tpc, maxp);
+        return bad(b);
+      }
+    }
     prod *= r;
     ns *= r;
   }
