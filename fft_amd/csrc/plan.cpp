@@ -969,6 +969,17 @@ bool factor_passes(i64 n, i64 rmax, int P, std::vector<i64>& out) {
   return !out.empty();
 }
 
+// Shortest column run (bytes) a two-pass 1-D plan may use. Measured on
+// MI355X against the three-pass plans (ms, complex64 / complex128):
+//   2^21 x 4  (1024 x 2048, 128- and 64-byte runs)  0.114 -> 0.092, 0.203 -> 0.185
+//   2^23      (2048 x 4096, 64 and 32 bytes)          0.097 -> 0.100, 0.202 -> 0.240
+//   2^24      (4096 x 4096, 32 bytes)                 0.195 -> 0.233, 0.364 -> 0.578
+//   RFFT 2^24 float32 (pairs of 32 bytes)             0.133 -> 0.136; PSD 2^26 (16 bytes) 0.519 -> 0.842
+// Passes of 4096 or more points take a whole CU's LDS for a few columns,
+// one workgroup per CU, and lose more to latency than the third pass costs.
+constexpr i64 kTwoPassMinRun = 64;
+static i64 two_pass_min_run() { return kTwoPassMinRun; }
+
 // Stockham engine over lines that may be strided: a line is n elements at
 // element stride `st`; `inner` lines share an outer block (adjacent in memory),
 // For a plain 1-D batch st = 1 and inner = 1.
@@ -982,6 +993,11 @@ class StockhamEngine : public Engine {
   // 2 hi + lo float pairs; ptwx holds the tables of forms 1 and 2
   int tw_mode = 0;
   DevBuf ptwx;
+  // The half-length transform of a real plan: the pass whose columns must
+  // pair with their mirrors (0 none, 1 the last: the forward untangle, 2 the
+  // first: the inverse pack), so that twice its columns fit LDS.
+  int pair_req = 0;
+  bool two_pass = false;  // planned by two_pass_lengths
   int P() const { return (int)passes.size(); }
   bool queueable() const override { return st == 1 && inner == 1 && tw_mode == 0; }
 
@@ -1090,9 +1106,65 @@ class StockhamEngine : public Engine {
     } else {
       std::vector<i64> lens;
       RC_TRY(multi_lengths(lens));
+      std::vector<i64> l2;
+      std::vector<int> c2;
+      if (lens.size() >= 3 && two_pass_lengths(pts, l2, c2)) {
+        two_pass = true;
+        return init_lengths(l2, c2);
+      }
       return init_lengths(lens);
     }
     return finish();
+  }
+
+  // Two passes instead of three or more for a contiguous 1-D transform: n =
+  // R1 x R2 with each pass taking fewer columns than the usual 128-byte runs
+  // (and, for a real plan's paired pass, half of what fits), at least
+  // 16-byte runs; consecutive column groups then share cache lines and run
+  // on one XCD (xcd_group). Of the factor pairs, the one whose narrowest
+  // pass has the widest runs, then the more balanced.
+  bool two_pass_lengths(i64 pts, std::vector<i64>& lens, std::vector<int>& cols) const {
+    if (st != 1 || inner != 1 || force_cols > 0 || two_pass_min_run() <= 0) return false;
+    const int cwant = prec == 0 ? 8 : 16;
+    i64 best_run = 0, best_bal = 0;
+    for (i64 R1 = 2; R1 <= std::min<i64>(16384, n / 2); R1++) {
+      if (n % R1) continue;
+      const i64 R2 = n / R1;
+      if (R2 > 16384) continue;
+      std::vector<i64> ls = {R1, R2};
+      std::vector<int> cs(2, 0);
+      i64 run = 1ll << 40;
+      bool ok = true;
+      for (int i = 0; i < 2 && ok; i++) {
+        const bool paired = (pair_req == 1 && i == 1) || (pair_req == 2 && i == 0);
+        int C = cwant;
+        while (C >= 1 && (i64)(paired ? 2 * C : C) * pitch(ls[i]) > pts) C /= 2;
+        SubPlan sp, sp2;
+        // a paired pass needs a schedule of 2C columns with the same stages
+        // (StockhamEngine::paired)
+        auto fits = [&](int c) {
+          if (!make_sub((int)ls[i], c, sp)) return false;
+          if (!paired) return true;
+          if (!make_sub((int)ls[i], 2 * c, sp2) || sp2.f.nstages != sp.f.nstages || sp2.tw != sp.tw) return false;
+          for (int k = 0; k < sp.f.nstages; k++)
+            if (sp2.f.st[k].radix != sp.f.st[k].radix) return false;
+          return true;
+        };
+        while (C >= 1 && !fits(C)) C /= 2;
+        if (C < 1) ok = false;
+        cs[i] = C;
+        run = std::min<i64>(run, (i64)C * (i64)elt(prec));
+      }
+      if (!ok || run < two_pass_min_run()) continue;
+      const i64 bal = -std::abs(ilog2((int)R1) - ilog2((int)R2));
+      if (run > best_run || (run == best_run && bal > best_bal)) {
+        best_run = run;
+        best_bal = bal;
+        lens = ls;
+        cols = cs;
+      }
+    }
+    return best_run > 0;
   }
 
   // Columns for one pass over strided lines that do not fit cwant columns:
@@ -1122,8 +1194,9 @@ class StockhamEngine : public Engine {
     return GOFFT_OK;
   }
 
-  // Multi-pass with the given pass lengths (product n), in order.
-  int init_lengths(const std::vector<i64>& lens) {
+  // Multi-pass with the given pass lengths (product n), in order; `cols`,
+  // if given, caps each pass's columns (narrow passes, placed by XCD).
+  int init_lengths(const std::vector<i64>& lens, const std::vector<int>& cols = {}) {
     DevInfo dev;
     RC_TRY(current_device(&dev));
     const i64 pts = dev.lds_bytes / (i64)elt(prec);
@@ -1141,12 +1214,14 @@ class StockhamEngine : public Engine {
     }
     passes.clear();
     i64 Ns = 1;
-    for (i64 R : lens) {
+    for (size_t li = 0; li < lens.size(); li++) {
+      const i64 R = lens[li];
       Pass p;
       int Cp = 0;
-      bool ok = pick_cols((int)R, cmp, pts, p.sp, &Cp);
+      bool ok = pick_cols((int)R, cols.empty() ? cmp : cols[li], pts, p.sp, &Cp);
       if (!ok)
         return fail(GOFFT_ERR_UNSUPPORTED, "no LDS schedule for pass length " + std::to_string(R));
+      if (!cols.empty() && (i64)Cp * (i64)elt(prec) < 128) p.xcd_group = 2;
       p.R = R;
       p.Ns = Ns;
       passes.push_back(p);
@@ -2257,11 +2332,19 @@ class RealEngine {
         return GOFFT_OK;
       }
       int rc = 0;
-      half = make_engine(m, batch, prec, flags, &rc);
+      half = make_real_half(m, 1, flags, &rc);
       if (!half) return rc;
       StockhamEngine* se = dynamic_cast<StockhamEngine*>(half.get());
       fused_last = se && se->prepare_untangle();
-      fused_first = se && se->prepare_pack();
+      // Two-pass halves pair different passes in the two directions, so
+      // the inverse gets its own (pass order and columns chosen for the
+      // pack in its first pass).
+      if (se && se->two_pass) {
+        half_inv = make_real_half(m, 2, flags, &rc);
+        if (!half_inv) return rc;
+      }
+      StockhamEngine* si = dynamic_cast<StockhamEngine*>(inv_half());
+      fused_first = si && si->prepare_pack();
       return GOFFT_OK;
     }
     int rc = 0;
@@ -2270,10 +2353,29 @@ class RealEngine {
   }
   bool fused_last = false;   // forward untangle done by the half transform's last pass
   bool fused_first = false;  // inverse pack done by the half transform's first pass
+  std::unique_ptr<Engine> half_inv;  // the inverse's own half transform (two-pass halves)
+  Engine* inv_half() const { return half_inv ? half_inv.get() : half.get(); }
+
+  // The m-point half transform, planned with `pair` (StockhamEngine::pair_req)
+  // when it is a Stockham transform the autotuner does not plan.
+  std::unique_ptr<Engine> make_real_half(i64 len, int pair, unsigned flags, int* rc) const {
+    const int r = route(len);
+    if ((r == 1 || r == 2) && !autotune_on(flags)) {
+      auto e = std::make_unique<StockhamEngine>();
+      e->prec = prec;
+      e->n = len;
+      e->batch = batch;
+      e->pair_req = pair;
+      *rc = e->init(flags);
+      return *rc == GOFFT_OK ? std::move(e) : nullptr;
+    }
+    return make_engine(len, batch, prec, flags, rc);
+  }
 
   size_t ws_bytes() const {
     if (single) return 0;
     size_t hw = half ? half->ws_bytes() : 0;
+    if (half_inv) hw = std::max(hw, half_inv->ws_bytes());
     // (the inverse still packs into m points, so the workspace keeps them)
     if (even) return (size_t)m * batch * elt(prec) + hw;
     return 2 * (size_t)n * batch * elt(prec) + hw;
@@ -2288,8 +2390,14 @@ class RealEngine {
     if (single) return s + " fused " + describe_sub(m, sp);
     if (even && fused_last) {
       const StockhamEngine* se = static_cast<const StockhamEngine*>(half.get());
-      return s + " packed: " + half->describe() + "; untangle in the last pass, paired " +
-             describe_sub(se->passes.back().R, se->pair_sp);
+      s += " packed: " + half->describe() + "; untangle in the last pass, paired " +
+           describe_sub(se->passes.back().R, se->pair_sp);
+      if (half_inv) {
+        const StockhamEngine* si = static_cast<const StockhamEngine*>(half_inv.get());
+        s += "; inverse: " + half_inv->describe();
+        if (fused_first) s += "; pack in the first pass, paired " + describe_sub(si->passes.front().R, si->pair_first_sp);
+      }
+      return s;
     }
     return s + (even ? " packed: " : " promoted: ") + half->describe();
   }
@@ -2304,7 +2412,7 @@ class RealEngine {
       w.add(GOFFT_KERNEL_REAL, 1, batch * (inverse ? ib + rb : rb + sb));
       return;
     }
-    half->work(w, batch);
+    (inverse ? inv_half() : half.get())->work(w, batch);
     if (even) {
       if (fused_last && !inverse) {
         w.add(GOFFT_KERNEL_PASS, 0, batch * (sb - m * e));  // the last pass writes the bins
@@ -2413,11 +2521,12 @@ class RealEngine {
         rf.utw = utw.p;
         rf.bins = std::max<i64>(0, std::min<i64>(bins, m + 1));
         rf.w_mid = w_mid;
-        return static_cast<const StockhamEngine*>(half.get())
+        return static_cast<const StockhamEngine*>(inv_half())
             ->run_real_inv(in, istride, out, ostride / 2, gain / (double)m, rf, ic, lines);
       }
       HIP_TRY(launch_herm_pack(prec, in, Z, utw.p, m, lines, istride, bins, m, w_mid, c.s));
-      RC_TRY(half->run(Z, LineIO{nullptr, m}, out, LineIO{nullptr, ostride / 2}, 1, 1, gain / (double)m, ic, lines));
+      RC_TRY(inv_half()->run(Z, LineIO{nullptr, m}, out, LineIO{nullptr, ostride / 2}, 1, 1, gain / (double)m, ic,
+                             lines));
       return GOFFT_OK;
     }
     char* A = c.ws;

@@ -22,9 +22,11 @@ def dev():
 
 
 # multi-pass half-lengths (untangle fused into the last pass): 1 << 17,
-# 26880 (m = 2^7*3*5*7), 49152 (m = 2^13*3), 62500 (m = 2*5^6), 1 << 20
+# 26880 (m = 2^7*3*5*7), 49152 (m = 2^13*3), 62500 (m = 2*5^6), 1 << 20;
+# 1 << 22 (m = 2^21: two passes, 2048 x 1024 forward and 1024 x 2048 inverse,
+# each paired pass with half its columns)
 RLEN = [1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 15, 16, 17, 31, 32, 1000, 1024, 2046, 4096, 1 << 14, 1 << 17,
-        1 << 20, 20014, 2 * 1009, 26880, 49152, 62500,
+        1 << 20, 1 << 22, 20014, 2 * 1009, 26880, 49152, 62500,
         40028, 10007, 3 << 17, 2 * 3 ** 12]  # Bluestein half length, odd prime, multi-pass 3-smooth halves
 
 
@@ -44,7 +46,7 @@ def test_rfft_irfft_fp64(dev, oracle, n):
     assert rel_l2(back, wantb) <= bound64(n), (n, rel_l2(back, wantb))
 
 
-@pytest.mark.parametrize("n", [1024, 1 << 20, 1000, 20014, 1 << 17, 49152, 62500, 40028, 3 << 17])
+@pytest.mark.parametrize("n", [1024, 1 << 20, 1 << 22, 1000, 20014, 1 << 17, 49152, 62500, 40028, 3 << 17])
 def test_rfft_fp32(dev, oracle, n):
     D, torch = dev
     x = rgauss(n, seed=n)
