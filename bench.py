@@ -572,7 +572,41 @@ def workload_host(D, torch, dev, steps, warmup):
            "roundtrip_err": {"rel_l2": float("%.3g" % err)}})
 
 
-WORKLOADS = {"host": workload_host, "roundtrip4096": workload_roundtrip4096, "spectral": workload_spectral, "nd": workload_nd,
+def workload_dft16(D, torch, dev, steps, warmup):
+    """One radix-16 stage both ways (gofft_probe_dft16): VALU butterflies, one
+    transform per lane, against the dense DFT-matrix product on the matrix
+    cores (a 32 x 32 real matrix, sixteen v_mfma_f32_32x32x2f32 per 32
+    transforms), on 2^22 complex64 lines of 16 points held in registers and
+    transformed `reps` times between one load and one store. reps = 1 is
+    bound by memory; reps = 64 by the arithmetic. Rate: stages per second (a
+    stage = one 16-point DFT of one line); the matrix form does 2048 flops per
+    stage, the butterflies about 144."""
+    import ctypes
+    L = D.L
+    n = 1 << 22
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    x = torch.randn(n * 16, dtype=torch.complex64, device=dev, generator=g)
+    y = torch.empty_like(x)
+    lib = L.lib()
+    s = torch.cuda.current_stream()
+    for reps in (1, 16, 64):
+        res = {}
+        for method, name in ((0, "valu"), (1, "mfma")):
+            def run():
+                L.check(lib.gofft_probe_dft16(method, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                              n, reps, ctypes.c_void_p(s.cuda_stream)))
+            ms = time_events(run, steps, warmup, torch)
+            res[name] = ms
+        print(json.dumps({"workload": "dft16_stage", "reps": reps, "lines": n,
+                          "valu_ms": round(res["valu"], 4), "mfma_ms": round(res["mfma"], 4),
+                          "valu_gstages_s": round(n * reps / (res["valu"] * 1e6), 1),
+                          "mfma_gstages_s": round(n * reps / (res["mfma"] * 1e6), 1),
+                          "mfma_tflops": round(n * reps * 2048 / (res["mfma"] * 1e9), 1),
+                          "mfma_over_valu_time": round(res["mfma"] / res["valu"], 2)}), flush=True)
+
+
+WORKLOADS = {"host": workload_host, "dft16": workload_dft16, "roundtrip4096": workload_roundtrip4096, "spectral": workload_spectral, "nd": workload_nd,
              "blur": workload_blur, "groups": workload_groups, "twiddles": workload_twiddles}
 
 

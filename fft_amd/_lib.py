@@ -92,6 +92,7 @@ SIGNATURES = {
     "gofft_graph_launch": (ctypes.c_int, [_vp, _vp]),
     "gofft_graph_destroy": (ctypes.c_int, [_vp]),
     "gofft_sync": (ctypes.c_int, []),
+    "gofft_probe_dft16": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _i64, ctypes.c_int, _vp]),
 }
 
 _lib = None

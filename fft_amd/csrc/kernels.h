@@ -358,6 +358,9 @@ hipError_t launch_residual_loss(int prec, void* r, const void* y, long long per,
                                 void* loss, hipStream_t s);
 // Fill n real elements of T with a value (graph-capturable, unlike a memset).
 hipError_t launch_fill(int prec, void* out, long long n, double value, hipStream_t s);
+// Radix-16 stage probe (method 0 VALU butterflies, 1 matrix cores), kernels.hip.
+hipError_t launch_probe_dft16(int method, const void* in, void* out, long long transforms, int reps,
+                              const float* wmat, hipStream_t s);
 // Scale a real or complex array in place or out of place (n elements of T).
 hipError_t launch_scale(int prec, const void* in, void* out, long long n, double scale,
                         hipStream_t s);

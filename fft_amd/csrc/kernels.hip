@@ -2348,6 +2348,89 @@ hipError_t launch_scale(int prec, const void* in, void* out, long long n, double
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Probe: one radix-16 DFT stage on complex64 data held in registers, applied
+// `reps` times, as VALU butterflies (method 0) or as a dense DFT-matrix
+// product on the matrix cores (method 1). Each application is the unitary
+// DFT (scaled by 1/4), so four of them give the input back.
+//
+// Method 1 writes the 16-point complex DFT as a 32 x 32 real matrix W (rows
+// and columns: 16 real parts, then 16 imaginary parts) and applies it to 32
+// transforms per wave with sixteen v_mfma_f32_32x32x2f32 (K = 2 each). The
+// instruction's result D (32 x 32) sits in 16 registers per lane: lane l,
+// register r holds row 8*(r/4) + 4*(l/32) + r%4 of column l%32. Slice q of
+// the K dimension is taken to be the rows that register q holds in the two
+// lane halves, so lane half h supplies its own register q as the B operand
+// and W[l%32][row(q, h)] as the A operand: the result comes out in the
+// layout the next product reads, and repeated stages need no exchange.
+
+__global__ void __launch_bounds__(256) k_probe_dft16_valu(const float2* in, float2* out, long long transforms,
+                                                          int reps) {
+  const long long t = blockIdx.x * 256ll + threadIdx.x;
+  if (t >= transforms) return;
+  float2 v[16];
+  const float2* p = in + t * 16;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = p[i];
+  for (int r = 0; r < reps; ++r) {
+    dft<16>(v);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = cscale(v[i], 0.25f);
+  }
+  float2* o = out + t * 16;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) o[i] = v[i];
+}
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ void __launch_bounds__(256) k_probe_dft16_mfma(const float2* in, float2* out, long long transforms,
+                                                          int reps, const float* wmat) {
+  const int lane = threadIdx.x & 63;
+  const long long t0 = ((blockIdx.x * 256ll + threadIdx.x) >> 6) * 32;
+  if (t0 >= transforms) return;  // whole waves only
+  const int j = lane & 31, h = lane >> 5;
+  float a[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) a[q] = wmat[lane * 16 + q];
+  const long long t = t0 + j;
+  const bool live = t < transforms;
+  f32x16 d;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int pt = 8 * (q >> 2) + 4 * h + (q & 3);
+    const float2 x = live ? in[t * 16 + pt] : make_float2(0.f, 0.f);
+    d[q] = x.x;
+    d[q + 8] = x.y;
+  }
+  for (int r = 0; r < reps; ++r) {
+    f32x16 acc = {};
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q], d[q], acc, 0, 0, 0);
+    d = acc;
+  }
+  if (!live) return;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int pt = 8 * (q >> 2) + 4 * h + (q & 3);
+    out[t * 16 + pt] = make_float2(d[q], d[q + 8]);
+  }
+}
+
+hipError_t launch_probe_dft16(int method, const void* in, void* out, long long transforms, int reps,
+                              const float* wmat, hipStream_t s) {
+  if (method == 0) {
+    const unsigned g = (unsigned)((transforms + 255) / 256);
+    hipLaunchKernelGGL(k_probe_dft16_valu, dim3(g), dim3(256), 0, s, (const float2*)in, (float2*)out,
+                       transforms, reps);
+  } else {
+    const unsigned g = (unsigned)((transforms + 127) / 128);  // 4 waves of 32 transforms
+    hipLaunchKernelGGL(k_probe_dft16_mfma, dim3(g), dim3(256), 0, s, (const float2*)in, (float2*)out,
+                       transforms, reps, wmat);
+  }
+  return hipGetLastError();
+}
+
 #else  // GOFFT_QUEUE_TU
 
 // ---------------------------------------------------------------------------

@@ -4250,4 +4250,44 @@ int gofft_sync(void) {
   return GOFFT_OK;
 }
 
+int gofft_probe_dft16(int method, const void* in, void* out, int64_t transforms, int reps, void* stream) {
+  if (method != 0 && method != 1) return fail(GOFFT_ERR_INVALID_ARG, "method must be 0 (VALU) or 1 (matrix cores)");
+  if (transforms < 0 || reps < 0) return fail(GOFFT_ERR_INVALID_ARG, "negative count");
+  if (transforms == 0) return GOFFT_OK;
+  if (!in || !out) return fail(GOFFT_ERR_INVALID_ARG, "null data pointer");
+  if (transforms > ((int64_t)1 << 34)) return fail(GOFFT_ERR_INVALID_ARG, "too many transforms");
+  // The A operands of the matrix-core form (kernels.hip k_probe_dft16_mfma):
+  // lane l, slice q: W[l % 32][row(q, l / 32)], row(q, h) = 8*(q/4) + 4h + q%4,
+  // W the unitary 16-point forward DFT as a real 32 x 32 matrix.
+  static std::mutex mu;
+  static std::map<int, std::unique_ptr<DevBuf>> tabs;
+  DevInfo dev;
+  RC_TRY(current_device(&dev));
+  const float* w = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto& b = tabs[dev.id];
+    if (!b) {
+      std::vector<float> h(64 * 16);
+      for (int l = 0; l < 64; l++)
+        for (int q = 0; q < 16; q++) {
+          const int i = l % 32, c = 8 * (q / 4) + 4 * (l / 32) + q % 4;
+          const int k = i % 16, nn = c % 16;
+          const cld wr = root((i64)k * nn, 16);  // exp(-2 pi i k n / 16)
+          const long double cs = wr.real(), sn = -wr.imag();
+          long double v;
+          if (i < 16) v = c < 16 ? cs : sn;  // Re Y = cos Re x + sin Im x
+          else v = c < 16 ? -sn : cs;        // Im Y = -sin Re x + cos Im x
+          h[l * 16 + q] = (float)(v * 0.25L);
+        }
+      auto nb = std::make_unique<DevBuf>();
+      RC_TRY(nb->upload(h.data(), h.size() * sizeof(float)));
+      b = std::move(nb);
+    }
+    w = (const float*)b->p;
+  }
+  HIP_TRY(launch_probe_dft16(method, in, out, transforms, reps, w, (hipStream_t)stream));
+  return GOFFT_OK;
+}
+
 }  // extern "C"

@@ -200,6 +200,14 @@ enum {
 };
 int64_t gofft_plan_index_table(const gofft_plan* plan, int64_t n, int table, int64_t* out, int64_t cap);
 
+/* Measurement probe (no reference counterpart; no plan): applies the unitary
+   16-point forward DFT (scaled by 1/4) `reps` times to each of `transforms`
+   complex64 lines of 16 points (device memory, out may alias in) in
+   registers, as VALU butterflies (method 0) or as a dense 32 x 32 real
+   DFT-matrix product on the matrix cores (method 1). It exists to time one
+   radix-16 stage both ways; no transform uses method 1. */
+int gofft_probe_dft16(int method, const void* in, void* out, int64_t transforms, int reps, void* stream);
+
 /* --- execution on device memory -------------------------------------------- */
 
 /* c2c / c2c_many / nd-C2C plans. */
