@@ -409,4 +409,12 @@ SIM_EW(launch_fill, int, void*, long long, double, hipStream_t)
 SIM_EW(launch_scale, int, const void*, void*, long long, double, hipStream_t)
 SIM_EW(launch_filter_mul, int, void*, const void*, long long, long long, int, hipStream_t)
 SIM_EW(launch_residual_loss, int, void*, const void*, long long, long long, void*, void*, hipStream_t)
+// the radix-16 stage probe (kernels.hip k_probe_dft16_*): one launch over the
+// given lines; gofft_probe_dft16 checks its arguments before it gets here
+hipError_t launch_probe_dft16(int method, const void* in, void* out, long long transforms, int reps,
+                              const float* wmat, hipStream_t) {
+  if ((method != 0 && method != 1) || !in || !out || transforms <= 0 || reps < 0 || !wmat)
+    bad("launch_probe_dft16: arguments");
+  return launch("launch_probe_dft16");
+}
 }  // namespace gofft

@@ -459,6 +459,26 @@ static void queue_plans() {
   }
 }
 
+// The radix-16 stage probe: argument errors, and one launch per method.
+// (It keeps its 4 KB matrix table per device until the process ends.)
+static void probe_calls() {
+  Buf in(4096 * 16 * 8), out(4096 * 16 * 8);
+  std::string w;
+  if (gofft_probe_dft16(2, in.p, out.p, 4096, 1, nullptr) != GOFFT_ERR_INVALID_ARG) w = "method 2 accepted";
+  if (w.empty() && gofft_probe_dft16(0, nullptr, out.p, 4096, 1, nullptr) != GOFFT_ERR_INVALID_ARG)
+    w = "null input accepted";
+  if (w.empty() && gofft_probe_dft16(1, in.p, out.p, -1, 1, nullptr) != GOFFT_ERR_INVALID_ARG)
+    w = "negative count accepted";
+  if (w.empty() && gofft_probe_dft16(1, in.p, out.p, 0, 1, nullptr) != GOFFT_OK) w = "empty call: " + err();
+  for (int m = 0; m < 2 && w.empty(); m++) {
+    hipsim_reset_counts();
+    if (gofft_probe_dft16(m, in.p, out.p, 4096, 3, nullptr) != GOFFT_OK) w = "method " + std::to_string(m) + ": " + err();
+    else if (hipsim_launches() != 1) w = "launches " + std::to_string(hipsim_launches());
+  }
+  if (w.empty() && *hipsim_bad()) w = hipsim_bad();
+  report("probe dft16", w);
+}
+
 int main() {
   gofft_init();
   sweep();
@@ -468,6 +488,7 @@ int main() {
   argument_errors();
   index_tables();
   queue_plans();
+  probe_calls();  // last: its table stays allocated for the process (the leak checks count from zero)
   std::printf("%s: %d failed\n", g_failures ? "FAILED" : "PASSED", g_failures);
   return g_failures ? 1 : 0;
 }
