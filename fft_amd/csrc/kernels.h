@@ -202,6 +202,7 @@ struct ConvArgs {
   const void* spec;         // kernel spectrum B (length R, 1/R folded in)
   const void* chirp;        // Bluestein chirp (length n) or null
   const int* perm;          // Rader input gather perm[k] (length q) or null
+  const int* dlog;          // Rader input order: dlog[i - 1] = k with perm[k] = i, i = 1..q
   const int* oinv;          // Rader output order: oinv[p - 1] = k with iperm[k] = p, p = 1..q
   const long long* in_off;  // optional per-line element offsets (else l * stride)
   const long long* out_off;

@@ -1473,29 +1473,56 @@ __device__ __forceinline__ void conv_body(const ConvArgs& a, long long blk) {
       if (a.conj_in) x0.y = -x0.y;
     }
     const int rounds = MAXP == LB ? 1 : (R + LB * tpc - 1) / (LB * tpc);
-    for (int rd = 0; rd < rounds; ++rd) {
-      const int r0 = lt + rd * LB * tpc;
-      T2 buf[LB];
+    if (a.kind == 2) {
+      // Bluestein: x[r] times the chirp, zero past n
+      for (int rd = 0; rd < rounds; ++rd) {
+        const int r0 = lt + rd * LB * tpc;
+        T2 buf[LB];
 #pragma unroll
-      for (int u = 0; u < LB; ++u) {
-        const int r = r0 + u * tpc;
-        buf[u] = mkc<T2>(0, 0);
-        if (live && r < R) {
-          if (a.kind == 2) {
-            if (r < n) buf[u] = x[r];
-          } else if (r < q) {
-            buf[u] = x[a.perm[r]];
+        for (int u = 0; u < LB; ++u) {
+          const int r = r0 + u * tpc;
+          buf[u] = mkc<T2>(0, 0);
+          if (live && r < n) buf[u] = x[r];
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          const int r = r0 + u * tpc;
+          if (r < R) {
+            T2 v = buf[u];
+            if (a.conj_in) v.y = -v.y;
+            if (r < n) v = cmul(v, chirp[r]);
+            lds[c * Rp + lpm<MAXP>(r, f.psh)] = v;
           }
         }
       }
+    } else {
+      // Rader: point k of the convolution input is x[perm[k]] (k < q, zero
+      // beyond). The line is read in order, x[i] for i = 1..q, each value
+      // placed at k = dlog[i - 1]: contiguous loads instead of a gather
+      // that touches a different cache line per point.
+      for (int r = q + lt; r < R; r += tpc) lds[c * Rp + lpm<MAXP>(r, f.psh)] = mkc<T2>(0, 0);
+      for (int rd = 0; rd < rounds; ++rd) {
+        const int i0 = 1 + lt + rd * LB * tpc;
+        T2 buf[LB];
+        int pos[LB];
 #pragma unroll
-      for (int u = 0; u < LB; ++u) {
-        const int r = r0 + u * tpc;
-        if (r < R) {
-          T2 v = buf[u];
-          if (a.conj_in) v.y = -v.y;
-          if (a.kind == 2 && r < n) v = cmul(v, chirp[r]);
-          lds[c * Rp + lpm<MAXP>(r, f.psh)] = v;
+        for (int u = 0; u < LB; ++u) {
+          const int i = i0 + u * tpc;
+          buf[u] = mkc<T2>(0, 0);
+          pos[u] = 0;
+          if (live && i < n) {
+            buf[u] = x[i];
+            pos[u] = a.dlog[i - 1];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          const int i = i0 + u * tpc;
+          if (i < n) {
+            T2 v = buf[u];
+            if (a.conj_in) v.y = -v.y;
+            lds[c * Rp + lpm<MAXP>(pos[u], f.psh)] = v;
+          }
         }
       }
     }

@@ -1748,7 +1748,7 @@ class RaderEngine : public Engine {
  public:
   i64 q = 0, cl = 0, g = 0;  // g: the primitive root whose powers order the gather
   bool cyclic = true;
-  DevBuf perm, iperm, oinv, spec;
+  DevBuf perm, iperm, oinv, dlog, spec;
   // fused path
   bool fused = false;
   bool big_lds = false;  // its workgroup takes more than half a CU's LDS
@@ -1799,6 +1799,11 @@ class RaderEngine : public Engine {
       std::vector<int> oi(q);
       for (i64 k = 0; k < q; k++) oi[iph[k] - 1] = (int)k;
       RC_TRY(oinv.upload(oi.data(), oi.size() * sizeof(int)));
+      // and reads x[i], i = 1..q, in order: dlog[i - 1] is the k with
+      // perm[k] = i (the discrete logarithm of i to the base g)
+      std::vector<int> dl(q);
+      for (i64 k = 0; k < q; k++) dl[ph[k] - 1] = (int)k;
+      RC_TRY(dlog.upload(dl.data(), dl.size() * sizeof(int)));
     } else {
       cb = chunk_lines(2 * cl * (i64)elt(prec), batch);
       RC_TRY(chain.init(cl, cb, prec, false, force_cols));
@@ -1834,6 +1839,7 @@ class RaderEngine : public Engine {
       a.stw = stw.p;
       a.spec = spec.p;
       a.perm = (const int*)perm.p;
+      a.dlog = (const int*)dlog.p;
       a.oinv = (const int*)oinv.p;
       a.in_stride = li.stride;
       a.out_stride = lo.stride;
