@@ -54,6 +54,10 @@ struct Stage {
 // Column addressing: flat column g -> digits d0 = g % D0, d1 = (g / D0) % D1,
 // d2 = (g / (D0*D1)) % D2, d3 = rest. Element r of column g sits at
 // sum(d_i * a_i) + r * g_stride.
+// PassArgs::nt bits
+constexpr int kNtLoad = 1;
+constexpr int kNtStore = 2;
+
 struct ColMap {
   long long D0, D1, D2;
   long long a0, a1, a2, a3;
@@ -147,6 +151,13 @@ struct PassArgs {
   // for the inverse when ru_conj.
   int real_first;           // 1: this launch is that form
   int xcd_group;            // > 1: consecutive column groups share cache lines; keep them on one XCD
+  // kNtLoad: the input points go through non-temporal loads; kNtStore: the
+  // output points through non-temporal stores. Set only where the launch's
+  // lines cover whole 128-byte runs that no other workgroup touches (plan
+  // workspaces, aligned caller lines): there the streamed data no longer
+  // displaces the tables in L2. Where neighbouring workgroups share cache
+  // lines it costs a second fetch or a partial-line write.
+  int nt;
   int ru_conj;
   long long ru_m;           // m = n/2 complex points per line
   long long ru_lines;       // lines of this launch

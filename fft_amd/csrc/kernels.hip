@@ -41,6 +41,42 @@ template <> __device__ __forceinline__ float2 from_d2<float2>(double2 w) {
   return make_float2((float)w.x, (float)w.y);
 }
 
+// Line data that a launch reads once and writes once (the transform's input
+// and output points, never the tables) can go through the non-temporal forms
+// of the vector loads and stores (PassArgs::nt / ConvArgs::nt, chosen per
+// launch by the planner). with_nt turns the launch's flag into a
+// compile-time constant, so each access loop is straight-line code.
+template <bool B> struct BoolC { static constexpr bool value = B; };
+template <typename T2> struct NtVec;
+template <> struct NtVec<float2> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<double2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <bool NT, typename T2> __device__ __forceinline__ T2 ldl(const T2* p) {
+  if constexpr (NT) {
+    typedef typename NtVec<T2>::type V;
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+    return mkc<T2>(v.x, v.y);
+  } else {
+    return *p;
+  }
+}
+template <bool NT, typename T2> __device__ __forceinline__ void stl(T2* p, T2 v) {
+  if constexpr (NT) {
+    typedef typename NtVec<T2>::type V;
+    V w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<V*>(p));
+  } else {
+    *p = v;
+  }
+}
+template <typename F> __device__ __forceinline__ void with_nt(bool nt, F&& f) {
+  if (nt)
+    f(BoolC<true>{});
+  else
+    f(BoolC<false>{});
+}
+
 // ---------------------------------------------------------------------------
 // Forward DFT butterflies in registers, natural order in and out.
 
@@ -551,8 +587,10 @@ __device__ __forceinline__ void mid256(const PassArgs& a, typename Vec2<T>::type
     const T2* in = reinterpret_cast<const T2*>(a.in);
     const T2* p = in + (base >= 0 ? base + s_il[ic] : 0) + (long long)ir0 * a.im.g;
     const long long st = 16 * a.im.g;
+    with_nt(a.nt & kNtLoad, [&](auto nt) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = base >= 0 ? p[u * st] : mkc<T2>(0, 0);
+      for (int u = 0; u < 16; ++u) v[u] = base >= 0 ? ldl<decltype(nt)::value>(p + u * st) : mkc<T2>(0, 0);
+    });
     double2 tw_w = make_double2(1.0, 0.0), tw_step = make_double2(1.0, 0.0);
     const long long k = a.tw_M ? s_tk[ic] : 0;
     if (k) {
@@ -626,13 +664,15 @@ __device__ __forceinline__ void mid256(const PassArgs& a, typename Vec2<T>::type
   T2* out = reinterpret_cast<T2*>(a.out) + s_ob[c] + s_ol[c] + (long long)lt * a.om.g;
   const long long st = 16 * a.om.g;
   const T sc = (T)a.scale;
+  with_nt(a.nt & kNtStore, [&](auto nt) {
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    T2 w = v[i];
-    if (a.conj_out) w.y = -w.y;
-    if (a.scale != 1.0) w = cscale(w, sc);
-    out[i * st] = w;
-  }
+    for (int i = 0; i < 16; ++i) {
+      T2 w = v[i];
+      if (a.conj_out) w.y = -w.y;
+      if (a.scale != 1.0) w = cscale(w, sc);
+      stl<decltype(nt)::value>(out + i * st, w);
+    }
+  });
 }
 
 // Two-stage register form of a pass whose sub-FFT is R = A x 16 points, radix
@@ -667,8 +707,10 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
     T2 w[A];
     if (a.in_hook == 0) {
       const T2* p = in + (base >= 0 ? base + il : 0) + (long long)ir0 * g;
+      with_nt(a.nt & kNtLoad, [&](auto nt) {
 #pragma unroll
-      for (int u = 0; u < A; ++u) v[u] = base >= 0 ? p[u * 16 * g] : mkc<T2>(0, 0);
+        for (int u = 0; u < A; ++u) v[u] = base >= 0 ? ldl<decltype(nt)::value>(p + u * 16 * g) : mkc<T2>(0, 0);
+      });
     } else if (a.in_hook == 1) {
       const T2* hw = reinterpret_cast<const T2*>(a.hook_w);
 #pragma unroll
@@ -730,13 +772,15 @@ __device__ __forceinline__ void reg2(const PassArgs& a, typename Vec2<T>::type* 
   const T sc = (T)a.scale;
   if (a.out_hook == 0) {
     const bool do_scale = a.scale != 1.0;
+    with_nt(a.nt & kNtStore, [&](auto nt) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      T2 x = v[i];
-      if (a.conj_out) x.y = -x.y;
-      if (do_scale) x = cscale(x, sc);
-      out[ol + (long long)(j + A * i) * g] = x;
-    }
+      for (int i = 0; i < 16; ++i) {
+        T2 x = v[i];
+        if (a.conj_out) x.y = -x.y;
+        if (do_scale) x = cscale(x, sc);
+        stl<decltype(nt)::value>(out + ol + (long long)(j + A * i) * g, x);
+      }
+    });
   } else if (HK == 1) {
     // all chirp loads first (index clamped to 0 past nvalid), then the
     // stores of the points below nvalid
@@ -1077,8 +1121,11 @@ __device__ __forceinline__ void pass_body(const PassArgs& a, long long blk, long
     if (base >= 0 && in_hook == 0) {
       const T2* p = in + base + il + (long long)ir0 * a.im.g;
       const long long st = (long long)frs * a.im.g;
+      with_nt(a.nt & kNtLoad, [&](auto nt) {
 #pragma unroll
-      for (int u = 0; u < MAXP; ++u) buf[u] = ir0 + u * frs < R ? p[u * st] : mkc<T2>(0, 0);
+        for (int u = 0; u < MAXP; ++u)
+          buf[u] = ir0 + u * frs < R ? ldl<decltype(nt)::value>(p + u * st) : mkc<T2>(0, 0);
+      });
     } else {
 #pragma unroll
       for (int u = 0; u < MAXP; ++u) {
@@ -1195,13 +1242,15 @@ __device__ __forceinline__ void pass_body(const PassArgs& a, long long blk, long
     if (obase >= 0) {
       T2* po = out + obase + s_ol[oc] + (long long)or0 * a.om.g;
       const long long st = (long long)frs * a.om.g;
+      with_nt(a.nt & kNtStore, [&](auto nt) {
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        T2 w = v[i];
-        if (a.conj_out) w.y = -w.y;
-        if (do_scale) w = cscale(w, sc);
-        po[i * st] = w;
-      }
+        for (int i = 0; i < 16; ++i) {
+          T2 w = v[i];
+          if (a.conj_out) w.y = -w.y;
+          if (do_scale) w = cscale(w, sc);
+          stl<decltype(nt)::value>(po + i * st, w);
+        }
+      });
     }
     return;
   }
@@ -1209,16 +1258,18 @@ __device__ __forceinline__ void pass_body(const PassArgs& a, long long blk, long
   if (out_hook == 0) {
     T2* po = out + obase + s_ol[oc] + (long long)or0 * a.om.g;
     const long long st = (long long)frs * a.om.g;
+    with_nt(a.nt & kNtStore, [&](auto nt) {
 #pragma unroll
-    for (int u = 0; u < MAXP; ++u) {
-      const int r = or0 + u * frs;
-      if (r < R) {
-        T2 v = lds_aff ? ocol[lp(or0, f.psh) + u * lstep] : ocol[lp(r, f.psh)];
-        if (a.conj_out) v.y = -v.y;
-        if (do_scale) v = cscale(v, sc);
-        po[u * st] = v;
+      for (int u = 0; u < MAXP; ++u) {
+        const int r = or0 + u * frs;
+        if (r < R) {
+          T2 v = lds_aff ? ocol[lp(or0, f.psh) + u * lstep] : ocol[lp(r, f.psh)];
+          if (a.conj_out) v.y = -v.y;
+          if (do_scale) v = cscale(v, sc);
+          stl<decltype(nt)::value>(po + u * st, v);
+        }
       }
-    }
+    });
     return;
   }
   const T2 x0 = out_hook == 2 ? s_x0[oc] : mkc<T2>(0, 0);

@@ -835,6 +835,18 @@ i64 chunk_bytes() {
   return v;
 }
 
+// Where the pass kernels stream line data through non-temporal loads and
+// stores (PassArgs::nt): bits 1/2 a convolution chain's workspace reads and
+// writes, 4/8 a multi-pass transform's workspace reads and writes, 16/32 a
+// plain transform's reads and writes of aligned caller lines.
+#ifndef GOFFT_NT_POLICY
+#define GOFFT_NT_POLICY 63
+#endif
+constexpr int kNtPolicy = GOFFT_NT_POLICY;
+int nt_bits(bool load_ok, int load_class, bool store_ok, int store_class) {
+  return (load_ok && (kNtPolicy & load_class) ? kNtLoad : 0) | (store_ok && (kNtPolicy & store_class) ? kNtStore : 0);
+}
+
 i64 chunk_lines(i64 per_line_bytes, i64 batch) {
   i64 cb = std::max<i64>(1, chunk_bytes() / std::max<i64>(1, per_line_bytes));
   return std::max<i64>(1, std::min<i64>(cb, batch));
@@ -1446,6 +1458,17 @@ class StockhamEngine : public Engine {
         // inverse's fused pack gains nothing from it).
         a.xcd_group = 2;
       }
+      // Non-temporal line accesses where this launch's column groups own
+      // whole 128-byte runs: 1-D lines (a single pass: whole lines per
+      // workgroup; several: column groups that neither straddle lines nor
+      // split runs), on workspace buffers or aligned caller lines, outside
+      // the real transforms' fused forms.
+      if (inner == 1 && !a.real_first && !a.real_last && p.xcd_group <= 1 &&
+          (P() == 1 || ((n / p.R) % p.sp.f.C == 0 && (size_t)p.sp.f.C * elt(prec) >= 128))) {
+        const bool in_ws = i > 0, out_ws = i < P() - 1;
+        a.nt = nt_bits(in_ws || !unaligned_lines(in, li), in_ws ? 4 : 16, out_ws || !unaligned_lines(out, lo),
+                       out_ws ? 8 : 32);
+      }
       HIP_TRY(issue_pass(prec, kMaxP, a, threads, c.s));
     }
     return GOFFT_OK;
@@ -1684,6 +1707,9 @@ struct ConvChain {
       // complex64 0.277 -> 0.271 ms; the middle launches (aligned workspace
       // rows) gain nothing from it.
       if (k == 0 || k == K - 1) a.xcd_group = 2;
+      // workspace rows (m points, aligned) stream through non-temporal
+      // accesses; the caller's lines do not (see xcd_group)
+      a.nt = nt_bits(k > 0, 1, k < K - 1, 2);
       HIP_TRY(issue_pass(prec, kMaxP, a, threads, s));
     }
     return GOFFT_OK;
