@@ -425,10 +425,11 @@ __device__ __forceinline__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw,
       case 8: lds_stage<T2, 8, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       case 11: lds_stage<T2, 11, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       case 13: lds_stage<T2, 13, MAXP>(lds, f, s, stw, c, lt, cdig); break;
-      // composite prime-factor radices: planned for the 16-point form only
-      case 6: if constexpr (MAXP == 16) lds_stage<T2, 6, MAXP>(lds, f, s, stw, c, lt, cdig); break;
-      case 10: if constexpr (MAXP == 16) lds_stage<T2, 10, MAXP>(lds, f, s, stw, c, lt, cdig); break;
-      case 12: if constexpr (MAXP == 16) lds_stage<T2, 12, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      // composite prime-factor radices: planned for the 16-point form and
+      // the complex64 convolution's wide form
+      case 6: if constexpr (MAXP == 16 || MAXP == 24) lds_stage<T2, 6, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 10: if constexpr (MAXP == 16 || MAXP == 24) lds_stage<T2, 10, MAXP>(lds, f, s, stw, c, lt, cdig); break;
+      case 12: if constexpr (MAXP == 16 || MAXP == 24) lds_stage<T2, 12, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       // grouped radix-2 layers: only the 16-point form is planned with them
       case 102: if constexpr (MAXP == 16) lds_stage_rec<T2, 2, MAXP>(lds, f, s, stw, c, lt, cdig); break;
       case 104: if constexpr (MAXP == 16) lds_stage_rec<T2, 4, MAXP>(lds, f, s, stw, c, lt, cdig); break;

@@ -402,6 +402,69 @@ std::vector<int> composite_schedule(int R) {
 constexpr int kWideP = 24;  // the complex64 fused convolution's wide form
 constexpr int kNoPad = 16;  // pad shift of an unpadded column (every point < 2^16)
 
+// Schedule of the wide form (one column, tpc threads, at most 24 points per
+// thread per stage, so a radix-r stage's R/r butterflies need
+// ceil(R/r/tpc) <= 24/r): the fewest stages over radices 2-8, 10, 12, 11
+// and 13, ties broken by an estimate of the work per point (the butterfly's
+// and twiddle products' flops, plus 16 instructions of addressing per
+// butterfly); odd radices first (kernels.hip lpm: their writes land at odd
+// strides, which spread over the LDS banks), then the others from the
+// largest. Empty when R has a prime factor above 13. Measured on Rader
+// 10007 x 583 complex64 (20160 points): 7,5,3,3,8,8 0.143-0.147 ms;
+// 7,12,12,10,2 0.141-0.143; 7,5,12,12,4 0.140-0.142; 7,12,10,6,4 (the
+// choice) 0.135-0.138.
+std::vector<int> wide_schedule(int R, int tpc) {
+  static const int kRad[] = {12, 10, 8, 7, 6, 5, 4, 3, 2, 11, 13};
+  auto cost = [](int r) {
+    double flops;
+    switch (r) {
+      case 2: flops = 5.0; break;
+      case 3: flops = 9.3; break;
+      case 4: flops = 8.5; break;
+      case 5: flops = 15.0; break;
+      case 6: flops = 12.3; break;
+      case 7: flops = 20.0; break;
+      case 8: flops = 12.0; break;
+      case 10: flops = 17.4; break;
+      case 12: flops = 14.8; break;
+      default: flops = 30.0; break;
+    }
+    return flops + 16.0 / r;
+  };
+  auto ok = [&](int r) {
+    const int nb = R / r;
+    return r <= kWideP && (nb + tpc - 1) / tpc <= kWideP / r;
+  };
+  std::vector<int> best, cur;
+  double best_cost = 0;
+  std::function<void(int, double)> go = [&](int x, double c) {
+    if (!best.empty() && cur.size() > best.size()) return;
+    if (x == 1) {
+      if (best.empty() || cur.size() < best.size() || c < best_cost) {
+        best = cur;
+        best_cost = c;
+      }
+      return;
+    }
+    // radices in non-increasing order of kRad's list position, so each
+    // multiset is visited once
+    const int last = cur.empty() ? 0 : (int)(std::find(std::begin(kRad), std::end(kRad), cur.back()) - std::begin(kRad));
+    for (int i = last; i < (int)(sizeof(kRad) / sizeof(kRad[0])); i++) {
+      const int r = kRad[i];
+      if (x % r || !ok(r)) continue;
+      cur.push_back(r);
+      go(x / r, c + cost(r));
+      cur.pop_back();
+    }
+  };
+  go(R, 0.0);
+  std::stable_sort(best.begin(), best.end(), [](int a, int b) {
+    const bool oa = a % 2, ob = b % 2;
+    return oa != ob ? oa : a > b;
+  });
+  return best;
+}
+
 // LDS column pitch for R points (kernels.hip: one pad slot per 16 points plus
 // one, odd so adjacent columns fall in different banks).
 i64 pitch(i64 R, int sh = 4) { return R + (R >> sh) + 1; }
@@ -471,6 +534,10 @@ bool make_sub(int R, int C, SubPlan& sp, int min_threads = 0, int sh = 4, int ma
   // Fewer stages with composite radices when they fit the same threads.
   if (maxp == kMaxP && maxrad >= 16 && R > 1) {
     std::vector<int> comp = composite_schedule(R);
+    if (!comp.empty() && comp.size() < rads.size() && feasible(comp, T)) rads = comp;
+  }
+  if (maxp == kWideP && R > 1) {
+    std::vector<int> comp = wide_schedule(R, T / C);
     if (!comp.empty() && comp.size() < rads.size() && feasible(comp, T)) rads = comp;
   }
   // A x 16 points with 16 threads per column: radix A first, the order the

@@ -92,7 +92,7 @@ void check_sub(const char* k, const gofft::SubFFT& f, int threads, int maxp, lon
     // butterfly j < nb reached by j = lt + m * tpc, m < max(maxp / radix, 1)
     if (f.st[i].radix < 100) {
       static const int k16[] = {2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 13, 16};
-      static const int k24[] = {2, 3, 4, 5, 7, 8, 11, 13, 16};
+      static const int k24[] = {2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 13, 16};
       bool known = false;
       for (int x : k16) known |= maxp == 16 && x == r;
       for (int x : k24) known |= maxp == 24 && x == r;
