@@ -1528,13 +1528,19 @@ class StockhamEngine : public Engine {
       // Non-temporal line accesses where this launch's column groups own
       // whole 128-byte runs: 1-D lines (a single pass: whole lines per
       // workgroup; several: column groups that neither straddle lines nor
-      // split runs), on workspace buffers or aligned caller lines, outside
-      // the real transforms' fused forms.
-      if (inner == 1 && !a.real_first && !a.real_last && p.xcd_group <= 1 &&
+      // split runs), on workspace buffers or aligned caller lines.
+      // (Strided N-D axes stay off: FFTN 256^3 complex64 0.150 -> 0.168 ms
+      // with their aligned passes streamed.)
+      if (inner == 1 && p.xcd_group <= 1 &&
           (P() == 1 || ((n / p.R) % p.sp.f.C == 0 && (size_t)p.sp.f.C * elt(prec) >= 128))) {
         const bool in_ws = i > 0, out_ws = i < P() - 1;
-        a.nt = nt_bits(in_ws || !unaligned_lines(in, li), in_ws ? 4 : 16, out_ws || !unaligned_lines(out, lo),
-                       out_ws ? 8 : 32);
+        // The fused real forms stream only their workspace side: the
+        // untangle's and the pack's bins go through their own code.
+        // Measured: RFFT / IRFFT 2^24 float64 0.254 / 0.262 -> 0.232 / 0.240
+        // ms (float32 unchanged).
+        const bool ld = in_ws ? !a.real_first : !a.real_first && !a.real_last && !unaligned_lines(in, li);
+        const bool st = out_ws ? !a.real_last : !a.real_first && !a.real_last && !unaligned_lines(out, lo);
+        a.nt = nt_bits(ld, in_ws ? 4 : 16, st, out_ws ? 8 : 32);
       }
       HIP_TRY(issue_pass(prec, kMaxP, a, threads, c.s));
     }
