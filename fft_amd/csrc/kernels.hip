@@ -2106,6 +2106,12 @@ hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipSt
         case 4: GOFFT_PASS_LB(double, 4, 256); break;
         default: GOFFT_PASS_LB(double, 0, 256); break;
       }
+    } else if (threads <= 512 && (mode == 0 || mode == 3 || mode == 4)) {
+      switch (mode) {
+        case 3: GOFFT_PASS_LB(double, 3, 512); break;
+        case 4: GOFFT_PASS_LB(double, 4, 512); break;
+        default: GOFFT_PASS_LB(double, 0, 512); break;
+      }
     } else {
       switch (mode) {
         case 1: GOFFT_PASS(double, 1); break;
@@ -2197,6 +2203,8 @@ hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipSt
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<double>();
     if (threads <= 256)
       hipLaunchKernelGGL((k_conv<double, 16, 256>), dim3(groups), dim3(threads), bytes, s, a);
+    else if (threads <= 512)
+      hipLaunchKernelGGL((k_conv<double, 16, 512>), dim3(groups), dim3(threads), bytes, s, a);
     else
       GOFFT_LAUNCH16(k_conv, double, a, bytes, threads, groups, s);
   } else {
