@@ -18,8 +18,9 @@ timed on this host).
 
 Roofline of the pass kernel: the step runs its length groups side by side on
 several streams, so its launches overlap. Each length group is therefore also
-timed on its own, on one stream, with the same warm-up and step count,
-through a plan of exactly that group; the groups whose launches are all
+timed on its own, on one stream, through a plan of exactly that group
+replayed as its captured graph (the step's count, at least 20 warm-up
+replays); the groups whose launches are all
 pass-kernel launches give the kernel's time per step. The library reports
 each launch's data bytes (its input and output lines once each), so achieved
 = bytes / time. `trace_window` names those isolated dispatches among the run's
@@ -64,6 +65,9 @@ def gaussian_complex(total, seed=0):
 
 def flops_of(lengths):
     return float(sum(5.0 * n * math.log2(n) for n in lengths))
+
+
+GROUP_WARMUP = 20  # warm-up replays of each length group alone (roofline)
 
 
 def time_events(fn, steps, warmup, torch):
@@ -118,15 +122,22 @@ def run_dtype(D, torch, dev, name, lengths, x64, steps, warmup):
     counts = {}
     for n in lengths:
         counts[n] = counts.get(n, 0) + 1
+    # Each group alone, replayed as its plan's captured graph like the step,
+    # after at least GROUP_WARMUP replays (a fresh plan on fresh buffers runs
+    # a few percent slower for its first few dozen calls).
+    gwarm = max(warmup, GROUP_WARMUP)
     for n in sorted(counts):
         cnt = counts[n]
         gp = D.C2CPlan(n, cnt, tdtype)
-        gx = torch.from_numpy(x64[: n * cnt].astype(ndtype)).to(dev)
-        gy = torch.empty_like(gx)
-        gms = time_events(lambda: gp.exec(gx, out=gy), steps, warmup, torch)
+        with torch.cuda.stream(stream):
+            gx = torch.from_numpy(x64[: n * cnt].astype(ndtype)).to(dev)
+            gy = torch.empty_like(gx)
+            gg = D.Graph(gp, gx, gy, stream=stream)
+            gms = time_events(lambda: gg.launch(stream=stream), steps, gwarm, torch)
+        torch.cuda.synchronize()
         gw = {k: gp.kernel_work(k) for k in (L.KERNEL_PASS, L.KERNEL_CONV, L.KERNEL_REAL, L.KERNEL_AUX)}
-        res["groups"][n] = {"count": cnt, "ms": gms, "launches": gp.num_launches(), "work": gw}
-        del gp, gx, gy
+        res["groups"][n] = {"count": cnt, "ms": gms, "launches": gp.num_launches(), "work": gw, "warmup": gwarm}
+        del gg, gp, gx, gy
     torch.cuda.empty_cache()
     return res
 
@@ -144,6 +155,7 @@ def pass_roofline(r, L, steps, warmup):
             launches += pl
             nbytes += pb
     step_launches = r["work"][L.KERNEL_PASS][0]
+    gwarm = max(warmup, GROUP_WARMUP)
     if launches != step_launches:
         raise RuntimeError("pass-kernel launches of the groups (%d) differ from the step's (%d)"
                            % (launches, step_launches))
@@ -155,11 +167,12 @@ def pass_roofline(r, L, steps, warmup):
         "kernel": PASS_KERNEL, "launches_per_step": launches,
         "alg_bytes_per_launch": round(nbytes / launches), "avg_launch_ms": round(avg_ms, 5),
         "kernel_ms_per_step": round(ms, 4),
-        "measured_on": "each length group alone on one stream (the step overlaps them)",
+        "measured_on": "each length group alone on one stream, replayed as its captured graph after %d "
+                       "warm-up replays (the step overlaps the groups)" % gwarm,
         # the run's k_pass<float, ...> dispatches: (warmup + steps) * launches
-        # in the eager step, the same in its graph replays (plus the capture,
-        # which launches nothing), then the per-group runs
-        "trace_window": {"skip": 2 * (steps + warmup) * step_launches, "take": (steps + warmup) * launches},
+        # in the eager step, the same in its graph replays (the captures
+        # launch nothing), then the per-group replays
+        "trace_window": {"skip": 2 * (steps + warmup) * step_launches, "take": (steps + gwarm) * launches},
     }
 
 

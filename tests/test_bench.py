@@ -74,7 +74,9 @@ def test_pass_roofline_uses_pass_only_groups():
     assert roof["avg_launch_ms"] == pytest.approx(0.1)
     assert roof["achieved"] == pytest.approx(1e9 / 0.1e6, rel=1e-6)  # 10000 GB/s
     assert roof["frac"] == pytest.approx(roof["achieved"] / bench.HBM_PEAK_GBS, rel=1e-3)
-    assert roof["trace_window"] == {"skip": 2 * 23 * 3, "take": 23 * 3}
+    # eager and graph-replayed steps (3 + 20 calls each), then each group's
+    # 20 + max(3, 20) replays
+    assert roof["trace_window"] == {"skip": 2 * 23 * 3, "take": (20 + bench.GROUP_WARMUP) * 3}
     # a group mixing pass and convolution launches is left out, and then the
     # launch counts no longer agree with the step's
     L, groups = _groups({1024: (0.1, 1, 1e9, 0), 20014: (0.3, 2, 2e9, 1)})
