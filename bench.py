@@ -319,6 +319,24 @@ def workload_roundtrip4096(D, torch, dev, steps, warmup):
     _line("roundtrip4096", ms, 2 * batch * 5.0 * n * 12, 4 * n * batch * 16, "c128",
           "FFT + IFFT, N=4096, batch 256, complex128 (64 MiB in total, resident in the 256 MB MALL)",
           {"roundtrip_err": _err(z, x, "the input (IFFT(FFT(x)) = x)")})
+    # The same step from HBM: before each timed step a 1 GiB fill (outside the
+    # timed events) evicts the data from the memory-side cache.
+    flush = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+    times = []
+    for i in range(warmup + steps):
+        flush.fill_(float(i))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        step()
+        e1.record()
+        torch.cuda.synchronize()
+        if i >= warmup:
+            times.append(e0.elapsed_time(e1))
+    del flush
+    _line("roundtrip4096_cold", sum(times) / len(times), 2 * batch * 5.0 * n * 12, 4 * n * batch * 16, "c128",
+          "FFT + IFFT, N=4096, batch 256, complex128, each step timed alone after a 1 GiB fill that evicts "
+          "its data from the memory-side cache")
 
 
 def workload_spectral(D, torch, dev, steps, warmup, cpu=True):
