@@ -1534,12 +1534,16 @@ class StockhamEngine : public Engine {
       if (inner == 1 && p.xcd_group <= 1 &&
           (P() == 1 || ((n / p.R) % p.sp.f.C == 0 && (size_t)p.sp.f.C * elt(prec) >= 128))) {
         const bool in_ws = i > 0, out_ws = i < P() - 1;
+        // workspace lines: n points each, back to back
+        const LineIO wio{nullptr, n};
         // The fused real forms stream only their workspace side: the
         // untangle's and the pack's bins go through their own code.
         // Measured: RFFT / IRFFT 2^24 float64 0.254 / 0.262 -> 0.232 / 0.240
         // ms (float32 unchanged).
-        const bool ld = in_ws ? !a.real_first : !a.real_first && !a.real_last && !unaligned_lines(in, li);
-        const bool st = out_ws ? !a.real_last : !a.real_first && !a.real_last && !unaligned_lines(out, lo);
+        const bool ld = in_ws ? !a.real_first && !unaligned_lines(a.in, wio)
+                              : !a.real_first && !a.real_last && !unaligned_lines(in, li);
+        const bool st = out_ws ? !a.real_last && !unaligned_lines(a.out, wio)
+                               : !a.real_first && !a.real_last && !unaligned_lines(out, lo);
         a.nt = nt_bits(ld, in_ws ? 4 : 16, st, out_ws ? 8 : 32);
       }
       HIP_TRY(issue_pass(prec, kMaxP, a, threads, c.s));
@@ -1780,9 +1784,11 @@ struct ConvChain {
       // complex64 0.277 -> 0.271 ms; the middle launches (aligned workspace
       // rows) gain nothing from it.
       if (k == 0 || k == K - 1) a.xcd_group = 2;
-      // workspace rows (m points, aligned) stream through non-temporal
-      // accesses; the caller's lines do not (see xcd_group)
-      a.nt = nt_bits(k > 0, 1, k < K - 1, 2);
+      // workspace rows (m points each) stream through non-temporal accesses
+      // when they lie on the 128-byte grid; the caller's lines do not (see
+      // xcd_group)
+      auto grid = [&](const void* q) { return (uintptr_t)q % 128 == 0 && ((size_t)m * elt(prec)) % 128 == 0; };
+      a.nt = nt_bits(k > 0 && grid(a.in), 1, k < K - 1 && grid(a.out), 2);
       HIP_TRY(issue_pass(prec, kMaxP, a, threads, s));
     }
     return GOFFT_OK;
