@@ -1380,32 +1380,62 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
   lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
   if (!live) return;
   const T2* utw = reinterpret_cast<const T2*>(a.utw);
-  for (int k = lt; k <= R; k += tpc) {
-    T2 X;
+  auto bin = [&](int k) -> T2 {
     if (k == 0 || k == R) {
-      T2 z0 = col[0];
-      X = mkc<T2>(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0);
-    } else {
-      T2 zk = col[lp(k, sh)], zmk = col[lp(R - k, sh)];
-      T xer = (zk.x + zmk.x) * (T)0.5;
-      T xei = (zk.y - zmk.y) * (T)0.5;
-      T xor_ = (zk.y + zmk.y) * (T)0.5;
-      T xoi = -(zk.x - zmk.x) * (T)0.5;
-      T2 w = utw[k];
-      T tr = w.x * xor_ - w.y * xoi;
-      T ti = w.x * xoi + w.y * xor_;
-      X = mkc<T2>(xer + tr, xei + ti);
+      const T2 z0 = col[0];
+      return mkc<T2>(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0);
     }
-    if (a.mode == 0) {
-      reinterpret_cast<T2*>(a.out)[l * a.out_stride + k] = X;
-    } else if (a.mode == 2) {
-      const bool edge = k == 0 || k == R;  // imaginary part exactly 0 at DC and Nyquist
-      reinterpret_cast<T2*>(a.out)[l * a.out_stride + k] =
-          edge ? mkc<T2>(X.x * (T)a.w_edge, 0) : cscale(X, (T)a.w_mid);
-    } else {
+    const T2 zk = col[lp(k, sh)], zmk = col[lp(R - k, sh)];
+    const T xer = (zk.x + zmk.x) * (T)0.5;
+    const T xei = (zk.y - zmk.y) * (T)0.5;
+    const T xor_ = (zk.y + zmk.y) * (T)0.5;
+    const T xoi = -(zk.x - zmk.x) * (T)0.5;
+    const T2 w = utw[k];
+    const T tr = w.x * xor_ - w.y * xoi;
+    const T ti = w.x * xoi + w.y * xor_;
+    return mkc<T2>(xer + tr, xei + ti);
+  };
+  auto finish = [&](int k, T2 X) -> T2 {  // mode 2 weights; mode 1 power in .x
+    const bool edge = k == 0 || k == R;  // imaginary part exactly 0 at DC and Nyquist
+    if (a.mode == 2) return edge ? mkc<T2>(X.x * (T)a.w_edge, 0) : cscale(X, (T)a.w_mid);
+    if (a.mode == 1) {
       T pw = (X.x * X.x + X.y * X.y) * (T)a.psd_scale;
-      if (k != 0 && k != R) pw *= (T)2;
-      reinterpret_cast<T*>(a.out)[l * a.out_stride + k] = pw;
+      if (!edge) pw *= (T)2;
+      return mkc<T2>(pw, 0);
+    }
+    return X;
+  };
+  // Bins k0 and k0 + 1 per thread and step, written by one store: adjacent
+  // lanes cover adjacent pairs (the rows need not be aligned to the pair;
+  // gfx950 vector stores need only element alignment).
+  typedef T Tu2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+  typedef T Tu4 __attribute__((ext_vector_type(4), aligned(sizeof(T))));
+  for (int k0 = 2 * lt; k0 <= R; k0 += 2 * tpc) {
+    const bool two = k0 + 1 <= R;
+    const T2 X0 = finish(k0, bin(k0));
+    const T2 X1 = two ? finish(k0 + 1, bin(k0 + 1)) : mkc<T2>(0, 0);
+    if (a.mode == 1) {
+      T* o = reinterpret_cast<T*>(a.out) + l * a.out_stride + k0;
+      if (two) {
+        Tu2 v;
+        v.x = X0.x;
+        v.y = X1.x;
+        *reinterpret_cast<Tu2*>(o) = v;
+      } else {
+        o[0] = X0.x;
+      }
+    } else {
+      T2* o = reinterpret_cast<T2*>(a.out) + l * a.out_stride + k0;
+      if (two) {
+        Tu4 v;
+        v.x = X0.x;
+        v.y = X0.y;
+        v.z = X1.x;
+        v.w = X1.y;
+        *reinterpret_cast<Tu4*>(o) = v;
+      } else {
+        o[0] = X0;
+      }
     }
   }
 }
