@@ -184,6 +184,7 @@ struct RealFwdArgs {
   double psd_scale;         // d / n
   double w_edge, w_mid;     // mode 2: DC/Nyquist bins times w_edge, others times w_mid
   int vec_in;               // sample pairs are 2T-aligned (even stride, aligned base)
+  int nt;                   // kNtLoad: the lines are read through non-temporal loads
   FastDiv nbd;              // division by R + 1 (bins per line)
   SubFFT f;
 };
@@ -200,6 +201,7 @@ struct RealInvArgs {
   long long out_stride;     // real samples between consecutive lines
   double scale;             // 1/m
   double w_mid;             // weight of the bins other than DC and Nyquist (1: IRFFT)
+  int nt;                   // kNtStore: the lines are written through non-temporal stores
   FastDiv halfd;            // division by R/2 + 1 (bin pairs per line)
   SubFFT f;
 };

@@ -1349,15 +1349,17 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
   const T2* win = reinterpret_cast<const T2*>(a.window);
   T2* const col = lds + c * Rp;
   T2 buf[MAXP];
+  with_nt(a.nt & kNtLoad, [&](auto nt) {
 #pragma unroll
-  for (int u = 0; u < MAXP; ++u) {
-    const int r = lt + u * tpc;
-    buf[u] = mkc<T2>(0, 0);
-    if (live && r < R) {
-      const T* q = p + 2 * (long long)u * tpc;
-      buf[u] = a.vec_in ? *reinterpret_cast<const T2*>(q) : mkc<T2>(q[0], q[1]);
+    for (int u = 0; u < MAXP; ++u) {
+      const int r = lt + u * tpc;
+      buf[u] = mkc<T2>(0, 0);
+      if (live && r < R) {
+        const T* q = p + 2 * (long long)u * tpc;
+        buf[u] = a.vec_in ? ldl<decltype(nt)::value>(reinterpret_cast<const T2*>(q)) : mkc<T2>(q[0], q[1]);
+      }
     }
-  }
+  });
   // lp(lt + u * tpc) is affine in u when tpc is a whole number of pad blocks
   const bool aff = (tpc & ((1 << sh) - 1)) == 0;
   const int step = tpc + (tpc >> sh);
@@ -1511,10 +1513,12 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
   // out_stride is even (n = 2R), so each sample pair is one aligned T2
   T2* out = reinterpret_cast<T2*>(reinterpret_cast<T*>(a.out) + l * a.out_stride);
   const T sc = (T)a.scale;
-  for (int r = lt; r < R; r += tpc) {
-    const T2 z = X[lp(r, sh)];
-    out[r] = mkc<T2>(z.x * sc, -z.y * sc);
-  }
+  with_nt(a.nt & kNtStore, [&](auto nt) {
+    for (int r = lt; r < R; r += tpc) {
+      const T2 z = X[lp(r, sh)];
+      stl<decltype(nt)::value>(out + r, mkc<T2>(z.x * sc, -z.y * sc));
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------

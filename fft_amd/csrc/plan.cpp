@@ -2562,6 +2562,11 @@ class RealEngine {
       a.w_edge = w_edge;
       a.w_mid = w_mid;
       a.vec_in = istride % 2 == 0 && (uintptr_t)in % (2 * relt(prec)) == 0;
+      // whole rows of n samples on the 128-byte grid, not shared with the
+      // next line (frames that do not overlap): non-temporal loads
+      if (a.vec_in && istride >= n && ((size_t)istride * relt(prec)) % 128 == 0 && (uintptr_t)in % 128 == 0 &&
+          ((size_t)n * relt(prec)) % 128 == 0)
+        a.nt = nt_bits(true, 16, false, 0);
       a.nbd = fastdiv_make((uint32_t)(sp.f.R + 1));
       a.f = sp.f;
       HIP_TRY(launch_real_fwd(prec, kMaxP, a, sp.threads, c.s));
@@ -2620,6 +2625,11 @@ class RealEngine {
       a.w_mid = w_mid;
       a.halfd = fastdiv_make((uint32_t)(sp.f.R / 2 + 1));
       a.f = sp.f;
+      // whole rows of n samples on the 128-byte grid, one workgroup each:
+      // written through non-temporal stores
+      if (ostride >= n && ((size_t)ostride * relt(prec)) % 128 == 0 && (uintptr_t)out % 128 == 0 &&
+          ((size_t)n * relt(prec)) % 128 == 0)
+        a.nt = nt_bits(false, 0, true, 32);
       HIP_TRY(launch_real_inv(prec, kMaxP, a, sp.threads, c.s));
       return GOFFT_OK;
     }
