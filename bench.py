@@ -483,6 +483,15 @@ def workload_blur(D, torch, dev, steps, warmup):
           "loss ||IRFFT2(RFFT2(x) H) - y||^2 and its gradient, 64 x 1024x1024 float32, Gaussian H sigma 3 px; "
           "one step = RFFT2, IRFFT2 and the two adjoint transforms plus elementwise work",
           {"err_vs_fp64": err, "adjoint_err_vs_fp64": err_adj})
+    # The same loss and gradient in one call (gofft_filter2_loss_grad): the
+    # filter multiplies, residual and loss sums run inside, and the spectrum
+    # is recomputed per chunk of arrays instead of kept (here one chunk).
+    Hc = H[: rows * rc].to(torch.complex64)
+    ms_f = time_events(lambda: p.filter_loss_grad(x, Hc, y, chunk=0), steps, warmup, torch)
+    _, g_f = p.filter_loss_grad(x, Hc, y, chunk=0)
+    _line("blur_grad_fused", ms_f, 4 * per, 4 * passb, "f32",
+          "the same loss and gradient through gofft_filter2_loss_grad (one call per step)",
+          {"vs_composed_step": _err(g_f, gx, "the composed step's gradient, same input")})
 
 
 def workload_groups(D, torch, dev, steps, warmup, lengths=None, precisions=("c64", "c128")):
