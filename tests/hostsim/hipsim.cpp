@@ -23,6 +23,7 @@
 namespace {
 std::mutex g_mu;
 long g_allocs = 0, g_fail_alloc = 0;    // fail allocation number g_fail_alloc (1-based; 0: never)
+long g_nt = 0;  // launches that take a non-temporal form (hipsim_nt_launches)
 long g_launches = 0, g_fail_launch = 0; // likewise for kernel launches
 long g_hostallocs = 0, g_fail_host = 0;
 long g_live = 0;                        // live device allocations
@@ -124,6 +125,7 @@ void hipsim_reset_counts() {
 }
 long hipsim_allocs() { return g_allocs; }
 long hipsim_launches() { return g_launches; }
+long hipsim_nt_launches() { return g_nt; }
 long hipsim_live() { return g_live; }
 void hipsim_region(void* p, size_t n) {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -252,8 +254,27 @@ hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
 namespace gofft {
 static const long kLds = 160 * 1024;
 
+// Non-temporal line accesses (PassArgs::nt, RealFwdArgs::nt, RealInvArgs::nt)
+// only on data the launch reads or writes in whole 128-byte runs through the
+// plain load and store code: no offset table, no hook, a 128-byte-aligned
+// base, and not the real forms' bins.
+void count_nt(int nt) {
+  if (nt) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_nt++;
+  }
+}
+void check_nt_pass(const PassArgs& a) {
+  count_nt(a.nt);
+  if ((a.nt & kNtLoad) && (a.in_off || a.in_hook || a.real_first || (uintptr_t)a.in % 128))
+    bad("k_pass: non-temporal loads of an offset-table, hooked, real-pack or unaligned input");
+  if ((a.nt & kNtStore) && (a.out_off || a.out_hook || a.real_last || (uintptr_t)a.out % 128))
+    bad("k_pass: non-temporal stores to an offset-table, hooked, real-untangle or unaligned output");
+}
+
 hipError_t launch_pass(int prec, int, const PassArgs& a, int threads, hipStream_t) {
   const size_t e = prec == 0 ? 16 : 8;
+  check_nt_pass(a);
   check_sub("k_pass", a.f, threads, 16, kLds - kPassStaticLds, e);
   if (a.mid_spec) check_sub("k_pass (second sub-FFT)", a.f2, threads, 16, kLds - kPassStaticLds, e);
   if (!a.real_last && !a.real_first && a.ncols <= 0) bad("k_pass: no columns");
@@ -261,10 +282,21 @@ hipError_t launch_pass(int prec, int, const PassArgs& a, int threads, hipStream_
   return launch("pass");
 }
 hipError_t launch_real_fwd(int prec, int, const RealFwdArgs& a, int threads, hipStream_t) {
+  const size_t re = prec == 0 ? 8 : 4;
+  count_nt(a.nt);
+  if ((a.nt & kNtLoad) && (!a.vec_in || (uintptr_t)a.in % 128 || (a.in_stride * re) % 128 ||
+                           a.in_stride < 2 * (long long)a.f.R))
+    bad("k_real_fwd: non-temporal loads of unaligned or overlapping rows");
+  if (a.nt & kNtStore) bad("k_real_fwd: non-temporal stores are not used");
   check_sub("k_real_fwd", a.f, threads, 16, kLds, prec == 0 ? 16 : 8);
   return launch("real_fwd");
 }
 hipError_t launch_real_inv(int prec, int, const RealInvArgs& a, int threads, hipStream_t) {
+  const size_t re = prec == 0 ? 8 : 4;
+  count_nt(a.nt);
+  if ((a.nt & kNtStore) && ((uintptr_t)a.out % 128 || (a.out_stride * re) % 128 || a.out_stride < 2 * (long long)a.f.R))
+    bad("k_real_inv: non-temporal stores to unaligned or overlapping rows");
+  if (a.nt & kNtLoad) bad("k_real_inv: non-temporal loads are not used");
   check_sub("k_real_inv", a.f, threads, 16, kLds, prec == 0 ? 16 : 8);
   return launch("real_inv");
 }

@@ -21,6 +21,7 @@ void hipsim_fail_launch(long n);
 void hipsim_reset_counts();
 long hipsim_allocs();
 long hipsim_launches();
+long hipsim_nt_launches();
 long hipsim_live();
 const char* hipsim_bad();
 void hipsim_region(void* p, size_t n);
@@ -489,6 +490,9 @@ int main() {
   index_tables();
   queue_plans();
   probe_calls();  // last: its table stays allocated for the process (the leak checks count from zero)
+  // the sweep's aligned workspaces and buffers took the non-temporal forms
+  // (each such launch was checked above)
+  report("non-temporal launches seen", hipsim_nt_launches() > 0 ? "" : "none");
   std::printf("%s: %d failed\n", g_failures ? "FAILED" : "PASSED", g_failures);
   return g_failures ? 1 : 0;
 }
