@@ -163,6 +163,11 @@ struct PassArgs {
   long long ru_lines;       // lines of this launch
   long long ru_wpl;         // workgroups per line
   const void* ru_utw;       // W_n^k, k = 0..m
+  // or, when ru_uhi is set, W_n^k = ru_uhi[k >> ru_ushift] * ru_ulo[k mod
+  // 2^ru_ushift], formed in FP64 (two tables of about sqrt(m) entries)
+  const double2* ru_uhi;
+  const double2* ru_ulo;
+  int ru_ushift;
   long long ru_ostride;     // output elements between lines
   int ru_mode;              // 0 bins, 1 one-sided PSD, 2 bins weighted (DC/Nyquist, others)
   double ru_psd, ru_w_edge, ru_w_mid;  // (real_first: w_mid scales bins 1..m-1)

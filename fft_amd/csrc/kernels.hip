@@ -847,6 +847,9 @@ __device__ __forceinline__ void untangle_store(const PassArgs& a, const typename
   const int H = f.C >> 1, log2H = f.log2C - 1, R = f.R, Rp = f.Rp, sh = f.psh;
   const long long m = a.ru_m, Ns = m / R;
   const T2* utw = reinterpret_cast<const T2*>(a.ru_utw);
+  auto ut = [&](long long k) -> T2 {
+    return a.ru_uhi ? from_d2<T2>(pass_twiddle(a.ru_uhi, a.ru_ulo, k, a.ru_ushift)) : utw[k];
+  };
   const T half = (T)0.5;
   auto untangle = [&](T2 za, T2 zb, T2 w) {
     const T xer = (za.x + zb.x) * half, xei = (za.y - zb.y) * half;
@@ -891,8 +894,8 @@ __device__ __forceinline__ void untangle_store(const PassArgs& a, const typename
     }
     const T2 zp = lds[cp * Rp + lp(rp, sh)];
     const long long kp = m - k;
-    emit(line, k, untangle(zk, zp, utw[k]));
-    if (kp != k) emit(line, kp, untangle(zp, zk, utw[kp]));
+    emit(line, k, untangle(zk, zp, ut(k)));
+    if (kp != k) emit(line, kp, untangle(zp, zk, ut(kp)));
   }
 }
 
@@ -910,6 +913,9 @@ __device__ __forceinline__ void pack_step(const PassArgs& a, typename Vec2<T>::t
   const int H = f.C >> 1, log2H = f.log2C - 1, R = f.R, Rp = f.Rp, sh = f.psh;
   const long long m = a.ru_m, Mc = m / R;
   const T2* utw = reinterpret_cast<const T2*>(a.ru_utw);
+  auto ut = [&](long long k) -> T2 {
+    return a.ru_uhi ? from_d2<T2>(pass_twiddle(a.ru_uhi, a.ru_ulo, k, a.ru_ushift)) : utw[k];
+  };
   const T half = (T)0.5, wm = (T)a.ru_w_mid;
   auto pack = [&](T2 xk, T2 xmk, T2 w) {
     xk = cscale(xk, wm);
@@ -950,8 +956,8 @@ __device__ __forceinline__ void pack_step(const PassArgs& a, typename Vec2<T>::t
     T2* const pp = lds + cp * Rp + lp(rp, sh);
     const T2 xk = *pk, xmk = *pp;
     const long long kp = m - k;
-    *pk = pack(xk, xmk, utw[k]);
-    if (kp != k) *pp = pack(xmk, xk, utw[kp]);
+    *pk = pack(xk, xmk, ut(k));
+    if (kp != k) *pp = pack(xmk, xk, ut(kp));
   }
 }
 

@@ -837,6 +837,9 @@ struct Ctx {
 // half-length multi-pass transform (kernels.hip pack_step).
 struct RealFirst {
   const void* utw = nullptr;  // W_n^k, k = 0..m
+  const double2* uhi = nullptr;  // or hi[k >> ushift] * lo[k mod 2^ushift] (FP64)
+  const double2* ulo = nullptr;
+  int ushift = 0;
   i64 bins = 0;               // bins per line present in the input
   double w_mid = 1.0;         // scale of bins 1..m-1
 };
@@ -845,6 +848,9 @@ struct RealFirst {
 // half-length multi-pass transform (kernels.hip untangle_store).
 struct RealLast {
   const void* utw = nullptr;  // W_n^k, k = 0..m
+  const double2* uhi = nullptr;  // or hi[k >> ushift] * lo[k mod 2^ushift] (FP64)
+  const double2* ulo = nullptr;
+  int ushift = 0;
   i64 ostride = 0;            // output elements between lines
   int mode = 0;               // 0 bins, 1 one-sided PSD, 2 weighted bins
   double psd = 1.0, w_edge = 1.0, w_mid = 1.0;
@@ -1500,6 +1506,9 @@ class StockhamEngine : public Engine {
         a.ru_lines = lines;
         a.ru_wpl = (Mc / 2 + 1 + H - 1) / H;
         a.ru_utw = rf->utw;
+        a.ru_uhi = rf->uhi;
+        a.ru_ulo = rf->ulo;
+        a.ru_ushift = rf->ushift;
         a.ru_w_mid = rf->w_mid;
         a.in_off = nullptr;
       }
@@ -1512,6 +1521,9 @@ class StockhamEngine : public Engine {
         a.ru_lines = lines;
         a.ru_wpl = (Ns / 2 + 1 + H - 1) / H;
         a.ru_utw = rl->utw;
+        a.ru_uhi = rl->uhi;
+        a.ru_ulo = rl->ulo;
+        a.ru_ushift = rl->ushift;
         a.ru_ostride = rl->ostride;
         a.ru_mode = rl->mode;
         a.ru_psd = rl->psd;
@@ -2415,6 +2427,11 @@ class RealEngine {
   bool even = false, single = false;
   SubPlan sp;
   DevBuf stw, utw;
+  // The fused untangle and pack's twiddles W_n^k (k = 0..m) as two FP64
+  // tables: hi[h] = W_n^(h * 2^ushift), lo[l] = W_n^l (l < 2^ushift).
+  DevBuf utw2;
+  int ushift = 0;
+  i64 unh = 0;  // entries of hi
   std::unique_ptr<Engine> half;  // m-point (even, multi) or n-point (odd) complex engine
 
   int init(unsigned flags) {
@@ -2456,6 +2473,18 @@ class RealEngine {
       }
       StockhamEngine* si = dynamic_cast<StockhamEngine*>(inv_half());
       fused_first = si && si->prepare_pack();
+      if (fused_last || fused_first) {
+        // Read from two tables of about sqrt(m) entries instead of the m + 1
+        // of utw: for PSD 2^26 that is 268 MB (complex64) of HBM reads less
+        // per call.
+        ushift = (ilog2((int)std::min<i64>(m + 1, 1 << 30)) + 1) / 2;
+        const i64 S = 1ll << ushift;
+        unh = m / S + 1;
+        std::vector<cld> t;
+        for (i64 h = 0; h < unh; h++) t.push_back(root(h * S, n));
+        for (i64 l = 0; l < S; l++) t.push_back(root(l, n));
+        RC_TRY(upload_complex(utw2, t, GOFFT_C128));
+      }
       return GOFFT_OK;
     }
     int rc = 0;
@@ -2580,6 +2609,9 @@ class RealEngine {
       if (fused_last) {
         RealLast rl;
         rl.utw = utw.p;
+        rl.uhi = (const double2*)utw2.p;
+        rl.ulo = rl.uhi ? rl.uhi + unh : nullptr;
+        rl.ushift = ushift;
         rl.ostride = ostride;
         rl.mode = mode;
         rl.psd = psd_scale;
@@ -2640,6 +2672,9 @@ class RealEngine {
       if (fused_first) {
         RealFirst rf;
         rf.utw = utw.p;
+        rf.uhi = (const double2*)utw2.p;
+        rf.ulo = rf.uhi ? rf.uhi + unh : nullptr;
+        rf.ushift = ushift;
         rf.bins = std::max<i64>(0, std::min<i64>(bins, m + 1));
         rf.w_mid = w_mid;
         return static_cast<const StockhamEngine*>(inv_half())
