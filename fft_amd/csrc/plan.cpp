@@ -2484,6 +2484,8 @@ class RealEngine {
         for (i64 h = 0; h < unh; h++) t.push_back(root(h * S, n));
         for (i64 l = 0; l < S; l++) t.push_back(root(l, n));
         RC_TRY(upload_complex(utw2, t, GOFFT_C128));
+        // with both directions fused nothing reads the (m+1)-entry table
+        if (fused_last && fused_first) RC_TRY(utw.alloc(0));
       }
       return GOFFT_OK;
     }
