@@ -2444,12 +2444,15 @@ class RealEngine {
       DevInfo dev;
       RC_TRY(current_device(&dev));
       const i64 pts = dev.lds_bytes / (i64)elt(prec);
-      // complex64 takes about 16 KB of LDS per workgroup (two waves for
-      // 512-point halves): measured on the 2^26 Spectrogram (131071 frames
-      // of 1024), 4 frames per workgroup 0.296 ms against 8 frames 0.308 ms,
-      // 2 frames 0.309 ms and 1 frame 0.347 ms. complex128 keeps the larger
-      // budget (real_cols then picks by resident waves).
-      const i64 soft = prec == GOFFT_C64 ? 2100 : soft_pts(prec);
+      // complex64 takes about 17 KB of LDS per workgroup: two waves for
+      // 256-, 512- and 1024-point halves (8, 4 and 2 lines of pitch 273,
+      // 545 and 1089 points). Measured on the 2^26 Spectrogram (131071
+      // frames of 1024), 4 frames per workgroup 0.304-0.305 ms against 2
+      // frames 0.316-0.317 ms, 8 frames 0.312-0.324 ms and 16 frames
+      // 0.318-0.319 ms (with the pair stores: 4 frames 0.290 ms, 2 frames
+      // 0.291-0.300 ms). complex128 keeps the larger budget (real_cols then
+      // picks by resident waves).
+      const i64 soft = prec == GOFFT_C64 ? 2200 : soft_pts(prec);
       const i64 budget = std::min<i64>(pts, std::max<i64>(pitch(m), soft));
       int C = 0;
       const int cap = (int)std::min<i64>(kMaxCols, std::max<i64>(1, batch));
