@@ -190,6 +190,13 @@ struct RealFwdArgs {
   double w_edge, w_mid;     // mode 2: DC/Nyquist bins times w_edge, others times w_mid
   int vec_in;               // sample pairs are 2T-aligned (even stride, aligned base)
   int nt;                   // kNtLoad: the lines are read through non-temporal loads
+  // Table form (complex64): the stage twiddles (stw_n entries), the R + 1
+  // untangle twiddles and the window are copied to LDS once per workgroup,
+  // and `wgs` workgroups loop over the line groups, loading the next
+  // group's samples while the current one is transformed.
+  int lds_tab;
+  int stw_n;
+  long long wgs;
   FastDiv nbd;              // division by R + 1 (bins per line)
   SubFFT f;
 };
@@ -207,6 +214,11 @@ struct RealInvArgs {
   double scale;             // 1/m
   double w_mid;             // weight of the bins other than DC and Nyquist (1: IRFFT)
   int nt;                   // kNtStore: the lines are written through non-temporal stores
+  // Table form (complex64), as RealFwdArgs: stage twiddles (stw_n entries)
+  // and the R/2 + 1 pack twiddles in LDS, `wgs` looping workgroups.
+  int lds_tab;
+  int stw_n;
+  long long wgs;
   FastDiv halfd;            // division by R/2 + 1 (bin pairs per line)
   SubFFT f;
 };
@@ -302,9 +314,12 @@ hipError_t launch_queue(int prec, const QueueArgs& a, int threads, size_t lds, l
 hipError_t launch_pass(int prec, int maxp, const PassArgs& a, int threads, hipStream_t s);
 hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads, hipStream_t s);
 hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads, hipStream_t s);
-// Vector registers per thread of the single-pass real kernels (the larger of
-// k_real_fwd and k_real_inv) at precision prec, from the code object.
-int real_kernel_vgprs(int prec);
+// Vector registers per thread of the single-pass real kernels at precision
+// prec, from the code object: dir 0 k_real_fwd, 1 k_real_inv, -1 the larger.
+int real_kernel_vgprs(int prec, int dir = -1);
+// Registers per thread of the complex64 table forms of the real forward
+// (inverse = 0) and inverse (inverse = 1) kernels.
+int real_tab_vgprs(int inverse);
 hipError_t launch_conv(int prec, int maxp, const ConvArgs& a, int threads, hipStream_t s);
 
 // Element-wise helpers for the multi-pass convolution and real paths.

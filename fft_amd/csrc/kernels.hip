@@ -440,6 +440,24 @@ __device__ __forceinline__ void lds_fft(T2* lds, const SubFFT& f, const T2* stw,
   }
 }
 
+// lds_fft for sub-FFTs whose stages are all radix 2, 4, 8 or 16 (fewer
+// live registers than the general dispatch).
+template <typename T2, int MAXP>
+__device__ __forceinline__ void lds_fft_pow2(T2* lds, const SubFFT& f, const T2* stw) {
+  const int tid = threadIdx.x;
+  const int c = tid >> f.log2tpc;
+  const int lt = tid & (f.tpc - 1);
+  for (int si = 0; si < f.nstages; ++si) {
+    const Stage& s = f.st[si];
+    switch (s.radix) {
+      case 2: lds_stage<T2, 2, MAXP>(lds, f, s, stw, c, lt, 0); break;
+      case 4: lds_stage<T2, 4, MAXP>(lds, f, s, stw, c, lt, 0); break;
+      case 8: lds_stage<T2, 8, MAXP>(lds, f, s, stw, c, lt, 0); break;
+      default: if constexpr (MAXP >= 16) lds_stage<T2, 16, MAXP>(lds, f, s, stw, c, lt, 0); break;
+    }
+  }
+}
+
 // Column g's digits: returns the in-line part sum(d_i * a_i, i < 3); *line
 // gets the line part (offset table entry or d3 * a3), *d3 the line index.
 __device__ __forceinline__ unsigned long long udiv(unsigned long long n, long long d,
@@ -1338,56 +1356,12 @@ __global__ void __launch_bounds__(LB) k_pass(PassArgs a) {
 // ---------------------------------------------------------------------------
 // Real forward, single pass: line l holds 2R real samples; Z = FFT_R(pairs),
 // then the untangle to R+1 bins (mode 0) or the one-sided PSD (mode 1).
-template <typename T, int MAXP>
-__global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
-  typedef typename Vec2<T>::type T2;
-  T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
-  const SubFFT& f = a.f;
-  const int tid = threadIdx.x;
-  const int R = f.R, Rp = f.Rp, tpc = f.tpc, sh = f.psh;
-  // Each thread keeps one line: c = tid / tpc, points r = lt + u * tpc
-  // (adjacent lanes read adjacent sample pairs).
-  const int c = tid >> f.log2tpc;
-  const int lt = tid & (tpc - 1);
-  const long long l = (long long)blockIdx.x * f.C + c;
-  const bool live = l < a.nlines;
-  const T* p = reinterpret_cast<const T*>(a.in) + (live ? l * a.in_stride : 0) + 2 * (long long)lt;
-  const T2* win = reinterpret_cast<const T2*>(a.window);
-  T2* const col = lds + c * Rp;
-  T2 buf[MAXP];
-  with_nt(a.nt & kNtLoad, [&](auto nt) {
-#pragma unroll
-    for (int u = 0; u < MAXP; ++u) {
-      const int r = lt + u * tpc;
-      buf[u] = mkc<T2>(0, 0);
-      if (live && r < R) {
-        const T* q = p + 2 * (long long)u * tpc;
-        buf[u] = a.vec_in ? ldl<decltype(nt)::value>(reinterpret_cast<const T2*>(q)) : mkc<T2>(q[0], q[1]);
-      }
-    }
-  });
-  // lp(lt + u * tpc) is affine in u when tpc is a whole number of pad blocks
-  const bool aff = (tpc & ((1 << sh) - 1)) == 0;
-  const int step = tpc + (tpc >> sh);
-#pragma unroll
-  for (int u = 0; u < MAXP; ++u) {
-    const int r = lt + u * tpc;
-    if (r < R) {
-      T2 v = buf[u];
-      if (win) {
-        const T2 w = win[r];
-        v = mkc<T2>(v.x * w.x, v.y * w.y);
-      }
-      if (aff)
-        col[lp(lt, sh) + u * step] = v;
-      else
-        col[lp(r, sh)] = v;
-    }
-  }
-  __syncthreads();
-  lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
-  if (!live) return;
-  const T2* utw = reinterpret_cast<const T2*>(a.utw);
+
+// Untangle of the transformed column `col` (line l) into R + 1 bins, and
+// their stores: X_k = (Z_k + conj Z_(R-k)) / 2 + W_n^k (Z_k - conj Z_(R-k)) / 2i.
+template <typename T, typename T2>
+__device__ __forceinline__ void real_fwd_bins(const RealFwdArgs& a, const T2* col, const T2* utw, long long l,
+                                              int lt, int tpc, int R, int sh) {
   auto bin = [&](int k) -> T2 {
     if (k == 0 || k == R) {
       const T2 z0 = col[0];
@@ -1448,9 +1422,201 @@ __global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
   }
 }
 
+// Loads line l's sample pairs r = lt + u * tpc into buf (zeros past R and
+// for lines past the batch).
+template <typename T, typename T2, int MAXP>
+__device__ __forceinline__ void real_fwd_load(const RealFwdArgs& a, long long l, int lt, int tpc, int R,
+                                              T2 (&buf)[MAXP]) {
+  const bool live = l < a.nlines;
+  const T* p = reinterpret_cast<const T*>(a.in) + (live ? l * a.in_stride : 0) + 2 * (long long)lt;
+  with_nt(a.nt & kNtLoad, [&](auto nt) {
+#pragma unroll
+    for (int u = 0; u < MAXP; ++u) {
+      const int r = lt + u * tpc;
+      buf[u] = mkc<T2>(0, 0);
+      if (live && r < R) {
+        const T* q = p + 2 * (long long)u * tpc;
+        buf[u] = a.vec_in ? ldl<decltype(nt)::value>(reinterpret_cast<const T2*>(q)) : mkc<T2>(q[0], q[1]);
+      }
+    }
+  });
+}
+
+// Windowed sample pairs into the thread's column of LDS.
+template <typename T2, int MAXP>
+__device__ __forceinline__ void real_fwd_put(T2* col, const T2 (&buf)[MAXP], const T2* win, bool has_win, int lt,
+                                             int tpc, int R, int sh) {
+  // lp(lt + u * tpc) is affine in u when tpc is a whole number of pad blocks
+  const bool aff = (tpc & ((1 << sh) - 1)) == 0;
+  const int step = tpc + (tpc >> sh);
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int r = lt + u * tpc;
+    if (r < R) {
+      T2 v = buf[u];
+      if (has_win) {
+        const T2 w = win[r];
+        v = mkc<T2>(v.x * w.x, v.y * w.y);
+      }
+      if (aff)
+        col[lp(lt, sh) + u * step] = v;
+      else
+        col[lp(r, sh)] = v;
+    }
+  }
+}
+
+template <typename T, int MAXP>
+__global__ void __launch_bounds__(1024) k_real_fwd(RealFwdArgs a) {
+  typedef typename Vec2<T>::type T2;
+  T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
+  const SubFFT& f = a.f;
+  const int tid = threadIdx.x;
+  const int R = f.R, Rp = f.Rp, tpc = f.tpc, sh = f.psh;
+  // Each thread keeps one line: c = tid / tpc, points r = lt + u * tpc
+  // (adjacent lanes read adjacent sample pairs).
+  const int c = tid >> f.log2tpc;
+  const int lt = tid & (tpc - 1);
+  const long long l = (long long)blockIdx.x * f.C + c;
+  T2* const col = lds + c * Rp;
+  T2 buf[MAXP];
+  real_fwd_load<T, T2, MAXP>(a, l, lt, tpc, R, buf);
+  real_fwd_put<T2, MAXP>(col, buf, reinterpret_cast<const T2*>(a.window), a.window != nullptr, lt, tpc, R, sh);
+  __syncthreads();
+  lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
+  if (l >= a.nlines) return;
+  real_fwd_bins<T, T2>(a, col, reinterpret_cast<const T2*>(a.utw), l, lt, tpc, R, sh);
+}
+
+// Table form of k_real_fwd (RealFwdArgs::lds_tab): the stage and untangle
+// twiddles and the window are read from LDS, so the only global loads inside
+// the loop are the next line group's samples, which stay in flight while the
+// current group is transformed.
+template <typename T, int MAXP>
+__global__ void __launch_bounds__(512) k_real_fwd_tab(RealFwdArgs a) {
+  typedef typename Vec2<T>::type T2;
+  T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
+  const SubFFT& f = a.f;
+  const int tid = threadIdx.x;
+  const int R = f.R, Rp = f.Rp, tpc = f.tpc, sh = f.psh;
+  const int c = tid >> f.log2tpc;
+  const int lt = tid & (tpc - 1);
+  const long long ngroups = (a.nlines + f.C - 1) / f.C;
+  const long long gstep = gridDim.x;
+  long long g = blockIdx.x;
+  T2* const col = lds + c * Rp;
+  T2 buf[MAXP];
+  real_fwd_load<T, T2, MAXP>(a, g * f.C + c, lt, tpc, R, buf);
+  // tables after the columns: stage twiddles, untangle twiddles, window
+  T2* const tab = lds + f.C * Rp;
+  const bool has_win = a.window != nullptr;
+  {
+    const T2* stw = reinterpret_cast<const T2*>(a.stw);
+    const T2* utw = reinterpret_cast<const T2*>(a.utw);
+    const T2* win = reinterpret_cast<const T2*>(a.window);
+    const int nthr = blockDim.x;
+    for (int i = tid; i < a.stw_n; i += nthr) tab[i] = stw[i];
+    for (int i = tid; i <= R; i += nthr) tab[a.stw_n + i] = utw[i];
+    if (has_win)
+      for (int i = tid; i < R; i += nthr) tab[a.stw_n + R + 1 + i] = win[i];
+  }
+  const T2* const stw_l = tab;
+  const T2* const utw_l = tab + a.stw_n;
+  const T2* const win_l = tab + a.stw_n + R + 1;
+  __syncthreads();
+  for (; g < ngroups; g += gstep) {
+    real_fwd_put<T2, MAXP>(col, buf, win_l, has_win, lt, tpc, R, sh);
+    const long long l = g * f.C + c;
+    if (g + gstep < ngroups) real_fwd_load<T, T2, MAXP>(a, (g + gstep) * f.C + c, lt, tpc, R, buf);
+    __syncthreads();
+    lds_fft_pow2<T2, MAXP>(lds, f, stw_l);
+    if (l < a.nlines) real_fwd_bins<T, T2>(a, col, utw_l, l, lt, tpc, R, sh);
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Real inverse, single pass: R+1 bins -> packed spectrum Z -> IFFT_R (via
 // conj(FFT(conj))) -> 2R real samples scaled by 1/R.
+// Loads line l's bins r = lt + u * tpc < R into buf, bins past in_bins and
+// lines past the batch as zero, every bin but DC times w_mid; *xm gets the
+// Nyquist bin's real part (lt = 0 only).
+template <typename T, typename T2, int MAXP>
+__device__ __forceinline__ void real_inv_load(const RealInvArgs& a, long long l, int lt, int tpc, int R,
+                                              T2 (&buf)[MAXP], T* xm) {
+  const bool live = l < a.nlines;
+  const T2* in = reinterpret_cast<const T2*>(a.in) + (live ? l * a.in_stride : 0);
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int r = lt + u * tpc;
+    buf[u] = mkc<T2>(0, 0);
+    if (live && r < R && r < a.in_bins) buf[u] = in[r];
+    if (r > 0) buf[u] = cscale(buf[u], (T)a.w_mid);
+  }
+  *xm = 0;
+  if (lt == 0 && live && R < a.in_bins) *xm = in[R].x;
+}
+
+template <typename T2, int MAXP>
+__device__ __forceinline__ void real_inv_put(T2* X, const T2 (&buf)[MAXP], int lt, int tpc, int R, int sh) {
+  const bool aff = (tpc & ((1 << sh) - 1)) == 0;
+  const int step = tpc + (tpc >> sh);
+#pragma unroll
+  for (int u = 0; u < MAXP; ++u) {
+    const int r = lt + u * tpc;
+    if (r < R) {
+      if (aff)
+        X[lp(lt, sh) + u * step] = buf[u];
+      else
+        X[lp(r, sh)] = buf[u];
+    }
+  }
+}
+
+// Pairs (k, R-k) for k = 0 .. R/2, each owned by one thread; k = 0 pairs DC
+// with the Nyquist bin's real part xm. Leaves conj(Z) in X, Z the packed
+// half-length spectrum, for the inverse by a forward transform.
+template <typename T, typename T2, typename XM>
+__device__ __forceinline__ void real_inv_pack(T2* X, const T2* utw, XM nyq, int lt, int tpc, int R, int sh) {
+  for (int k = lt; k <= R / 2; k += tpc) {
+    if (k == 0) {
+      const T x0 = X[0].x;
+      const T xm = nyq();
+      X[0] = mkc<T2>((x0 + xm) * (T)0.5, -((x0 - xm) * (T)0.5));
+      continue;
+    }
+    const int mk_ = R - k;
+    T2 xk = X[lp(k, sh)], xmk = X[lp(mk_, sh)];
+    // Z[k] = xe + i*xo with xe = (Xk + conj Xmk)/2, d = (Xk - conj Xmk)/2,
+    // xo = conj(W^k) d; Z[m-k] symmetric.
+    T2 w = utw[k];
+    T xer = (xk.x + xmk.x) * (T)0.5, xei = (xk.y - xmk.y) * (T)0.5;
+    T dr = (xk.x - xmk.x) * (T)0.5, di = (xk.y + xmk.y) * (T)0.5;
+    T xor_ = w.x * dr + w.y * di, xoi = w.x * di - w.y * dr;
+    T2 zk = mkc<T2>(xer - xoi, xei + xor_);
+    if (k != mk_) {
+      T2 zmk = mkc<T2>(xer + xoi, -(xei - xor_));
+      X[lp(mk_, sh)] = cconj(zmk);
+    }
+    X[lp(k, sh)] = cconj(zk);
+  }
+}
+
+// Samples of line l from the transformed column: conj, times 1/m.
+template <typename T, typename T2>
+__device__ __forceinline__ void real_inv_store(const RealInvArgs& a, const T2* X, long long l, int lt, int tpc,
+                                               int R, int sh) {
+  // out_stride is even (n = 2R), so each sample pair is one aligned T2
+  T2* out = reinterpret_cast<T2*>(reinterpret_cast<T*>(a.out) + l * a.out_stride);
+  const T sc = (T)a.scale;
+  with_nt(a.nt & kNtStore, [&](auto nt) {
+    for (int r = lt; r < R; r += tpc) {
+      const T2 z = X[lp(r, sh)];
+      stl<decltype(nt)::value>(out + r, mkc<T2>(z.x * sc, -z.y * sc));
+    }
+  });
+}
+
 template <typename T, int MAXP>
 __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
   typedef typename Vec2<T>::type T2;
@@ -1474,57 +1640,59 @@ __global__ void __launch_bounds__(1024) k_real_inv(RealInvArgs a) {
     if (live && r < R && r < a.in_bins) buf[u] = in[r];
     if (r > 0) buf[u] = cscale(buf[u], (T)a.w_mid);
   }
-  const bool aff = (tpc & ((1 << sh) - 1)) == 0;
-  const int step = tpc + (tpc >> sh);
-#pragma unroll
-  for (int u = 0; u < MAXP; ++u) {
-    const int r = lt + u * tpc;
-    if (r < R) {
-      if (aff)
-        X[lp(lt, sh) + u * step] = buf[u];
-      else
-        X[lp(r, sh)] = buf[u];
-    }
-  }
+  real_inv_put<T2, MAXP>(X, buf, lt, tpc, R, sh);
   __syncthreads();
-  // Pairs (k, R-k) for k = 0 .. R/2, each owned by one thread; k = 0 pairs DC
-  // with the Nyquist bin X[R] read from memory.
-  for (int k = lt; k <= R / 2; k += tpc) {
-    if (k == 0) {
-      T x0 = X[0].x;
-      T xm = 0;
-      if (live && R < a.in_bins) xm = in[R].x;
-      // conj for the inverse-by-forward trick
-      X[0] = mkc<T2>((x0 + xm) * (T)0.5, -((x0 - xm) * (T)0.5));
-      continue;
-    }
-    const int mk_ = R - k;
-    T2 xk = X[lp(k, sh)], xmk = X[lp(mk_, sh)];
-    // Z[k] = xe + i*xo with xe = (Xk + conj Xmk)/2, d = (Xk - conj Xmk)/2,
-    // xo = conj(W^k) d; Z[m-k] symmetric.
-    T2 w = utw[k];
-    T xer = (xk.x + xmk.x) * (T)0.5, xei = (xk.y - xmk.y) * (T)0.5;
-    T dr = (xk.x - xmk.x) * (T)0.5, di = (xk.y + xmk.y) * (T)0.5;
-    T xor_ = w.x * dr + w.y * di, xoi = w.x * di - w.y * dr;
-    T2 zk = mkc<T2>(xer - xoi, xei + xor_);
-    if (k != mk_) {
-      T2 zmk = mkc<T2>(xer + xoi, -(xei - xor_));
-      X[lp(mk_, sh)] = cconj(zmk);
-    }
-    X[lp(k, sh)] = cconj(zk);
-  }
+  real_inv_pack<T, T2>(
+      X, utw, [&] { return live && R < a.in_bins ? in[R].x : (T)0; }, lt, tpc, R, sh);
   __syncthreads();
   lds_fft<T2, MAXP>(lds, f, reinterpret_cast<const T2*>(a.stw));
   if (!live) return;
-  // out_stride is even (n = 2R), so each sample pair is one aligned T2
-  T2* out = reinterpret_cast<T2*>(reinterpret_cast<T*>(a.out) + l * a.out_stride);
-  const T sc = (T)a.scale;
-  with_nt(a.nt & kNtStore, [&](auto nt) {
-    for (int r = lt; r < R; r += tpc) {
-      const T2 z = X[lp(r, sh)];
-      stl<decltype(nt)::value>(out + r, mkc<T2>(z.x * sc, -z.y * sc));
-    }
-  });
+  real_inv_store<T, T2>(a, X, l, lt, tpc, R, sh);
+}
+
+// Table form of k_real_inv (RealInvArgs::lds_tab), as k_real_fwd_tab: the
+// stage and pack twiddles in LDS, the next line group's bins loaded while
+// the current group is transformed.
+template <typename T, int MAXP>
+__global__ void __launch_bounds__(512) k_real_inv_tab(RealInvArgs a) {
+  typedef typename Vec2<T>::type T2;
+  T2* lds = reinterpret_cast<T2*>(g_dyn_lds);
+  const SubFFT& f = a.f;
+  const int tid = threadIdx.x;
+  const int R = f.R, Rp = f.Rp, tpc = f.tpc, sh = f.psh;
+  const int c = tid >> f.log2tpc;
+  const int lt = tid & (tpc - 1);
+  const long long ngroups = (a.nlines + f.C - 1) / f.C;
+  const long long gstep = gridDim.x;
+  long long g = blockIdx.x;
+  T2* const X = lds + c * Rp;
+  T2 buf[MAXP];
+  T xm;
+  real_inv_load<T, T2, MAXP>(a, g * f.C + c, lt, tpc, R, buf, &xm);
+  // tables after the columns: stage twiddles, then pack twiddles k <= R/2
+  T2* const tab = lds + f.C * Rp;
+  {
+    const T2* stw = reinterpret_cast<const T2*>(a.stw);
+    const T2* utw = reinterpret_cast<const T2*>(a.utw);
+    const int nthr = blockDim.x;
+    for (int i = tid; i < a.stw_n; i += nthr) tab[i] = stw[i];
+    for (int i = tid; i <= R / 2; i += nthr) tab[a.stw_n + i] = utw[i];
+  }
+  const T2* const stw_l = tab;
+  const T2* const utw_l = tab + a.stw_n;
+  __syncthreads();
+  for (; g < ngroups; g += gstep) {
+    real_inv_put<T2, MAXP>(X, buf, lt, tpc, R, sh);
+    const T xmc = xm;
+    const long long l = g * f.C + c;
+    if (g + gstep < ngroups) real_inv_load<T, T2, MAXP>(a, (g + gstep) * f.C + c, lt, tpc, R, buf, &xm);
+    __syncthreads();
+    real_inv_pack<T, T2>(X, utw_l, [&] { return xmc; }, lt, tpc, R, sh);
+    __syncthreads();
+    lds_fft_pow2<T2, MAXP>(lds, f, stw_l);
+    if (l < a.nlines) real_inv_store<T, T2>(a, X, l, lt, tpc, R, sh);
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2205,12 +2373,17 @@ hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads
     GOFFT_LAUNCH16(k_real_fwd, double, a, bytes, threads, groups, s);
   } else {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
-    GOFFT_LAUNCH16(k_real_fwd, float, a, bytes, threads, groups, s);
+    if (a.lds_tab) {
+      bytes += (size_t)(a.stw_n + 2 * a.f.R + 1) * elt2<float>();
+      hipLaunchKernelGGL((k_real_fwd_tab<float, 8>), dim3(a.wgs < groups ? a.wgs : groups), dim3(threads), bytes, s, a);
+    } else {
+      GOFFT_LAUNCH16(k_real_fwd, float, a, bytes, threads, groups, s);
+    }
   }
   return hipGetLastError();
 }
 
-int real_kernel_vgprs(int prec) {
+int real_kernel_vgprs(int prec, int dir) {
   hipFuncAttributes fa{}, ia{};
   hipError_t e1, e2;
   if (prec == 0) {
@@ -2221,7 +2394,17 @@ int real_kernel_vgprs(int prec) {
     e2 = hipFuncGetAttributes(&ia, reinterpret_cast<const void*>(&k_real_inv<float, 16>));
   }
   if (e1 != hipSuccess || e2 != hipSuccess) return 0;
+  if (dir == 0) return fa.numRegs;
+  if (dir == 1) return ia.numRegs;
   return fa.numRegs > ia.numRegs ? fa.numRegs : ia.numRegs;
+}
+
+int real_tab_vgprs(int inverse) {
+  hipFuncAttributes fa{};
+  const void* k = inverse ? reinterpret_cast<const void*>(&k_real_inv_tab<float, 8>)
+                          : reinterpret_cast<const void*>(&k_real_fwd_tab<float, 8>);
+  if (hipFuncGetAttributes(&fa, k) != hipSuccess) return 0;
+  return fa.numRegs;
 }
 
 hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads, hipStream_t s) {
@@ -2232,7 +2415,12 @@ hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads
     GOFFT_LAUNCH16(k_real_inv, double, a, bytes, threads, groups, s);
   } else {
     size_t bytes = (size_t)a.f.C * a.f.Rp * elt2<float>();
-    GOFFT_LAUNCH16(k_real_inv, float, a, bytes, threads, groups, s);
+    if (a.lds_tab) {
+      bytes += (size_t)(a.stw_n + a.f.R / 2 + 1) * elt2<float>();
+      hipLaunchKernelGGL((k_real_inv_tab<float, 8>), dim3(a.wgs < groups ? a.wgs : groups), dim3(threads), bytes, s, a);
+    } else {
+      GOFFT_LAUNCH16(k_real_inv, float, a, bytes, threads, groups, s);
+    }
   }
   return hipGetLastError();
 }

@@ -912,6 +912,11 @@ i64 chunk_bytes() {
 // stores (PassArgs::nt): bits 1/2 a convolution chain's workspace reads and
 // writes, 4/8 a multi-pass transform's workspace reads and writes, 16/32 a
 // plain transform's reads and writes of aligned caller lines.
+#ifndef GOFFT_REAL_TAB
+#define GOFFT_REAL_TAB 1
+#endif
+constexpr bool kRealTab = GOFFT_REAL_TAB != 0;
+
 #ifndef GOFFT_NT_POLICY
 #define GOFFT_NT_POLICY 63
 #endif
@@ -2460,6 +2465,7 @@ class RealEngine {
           real_cols((int)m, cap, pts, budget, prec, dev, sp, &C)) {
         single = true;
         RC_TRY(upload_complex(stw, sp.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp.tw, prec));
+        if (prec == GOFFT_C64 && kRealTab) RC_TRY(plan_tab(dev, pts));
         return GOFFT_OK;
       }
       int rc = 0;
@@ -2496,6 +2502,71 @@ class RealEngine {
     half = make_engine(n, batch, prec, flags, &rc);
     return half ? GOFFT_OK : rc;
   }
+  // Table forms of the complex64 single-pass real kernels (kernels.hip
+  // k_real_fwd_tab, k_real_inv_tab): 8 points per thread, radices 2, 4 and
+  // 8, the twiddles (and the forward's window) in LDS, tab_wgs[dir]
+  // workgroups looping over the line groups with the next group's loads in
+  // flight. dir 0 forward, 1 inverse; tab_wgs 0 keeps the plain kernel.
+  SubPlan sp_tab;
+  DevBuf stw_tab;
+  i64 tab_wgs[2] = {0, 0};
+
+  int plan_tab(const DevInfo& dev, i64 pts) {
+    if (m < 64 || (m & (m - 1)) || batch < 2 * (i64)dev.cus) return GOFFT_OK;
+    for (int dir = 0; dir < 2; dir++) {
+      const int vg = real_tab_vgprs(dir);
+      if (vg <= 0) return GOFFT_OK;
+      const int wave_cap = 4 * std::min(8, 512 / ((vg + 7) / 8 * 8));
+      // LDS points of the tables beside the lines: forward stage + untangle
+      // twiddles + window, inverse stage + pack twiddles
+      auto tab_pts = [&](const SubPlan& t) {
+        return (i64)std::max<size_t>(t.tw.size(), 1) + (dir == 0 ? 2 * m + 1 : m / 2 + 1);
+      };
+      auto waves = [&](const SubPlan& t) {
+        const i64 wg_pts = (i64)t.f.C * pitch(m) + tab_pts(t);
+        if (wg_pts > pts) return 0;
+        const int waves_wg = (t.threads + 63) / 64;
+        const i64 by_lds = dev.lds_full / (wg_pts * (i64)elt(prec));
+        return (int)std::min<i64>(by_lds, wave_cap / waves_wg) * waves_wg;
+      };
+      if (dir == 0) {
+        // The column count, chosen for the forward: the most waves resident
+        // among the counts whose line groups fill the resident grid at least
+        // once (smaller batches keep the plain kernel); ties go to more
+        // lines per wave.
+        int best = 0;
+        for (int C = 1; C <= kMaxCols && C <= batch; C *= 2) {
+          SubPlan t;
+          if (!make_sub((int)m, C, t, 0, 4, 8, 8) || t.threads > 512) continue;
+          const int w = waves(t);
+          const i64 grid = (i64)std::max(dev.cus, 1) * (w / ((t.threads + 63) / 64));
+          if (!w || (batch + C - 1) / C < grid) continue;
+          if (w >= best) {
+            best = w;
+            sp_tab = t;
+          }
+        }
+        if (!best) return GOFFT_OK;
+      }
+      const int w = waves(sp_tab);
+      if (!w) continue;
+      // The tables take LDS from the lines (about 6 KB per 1024 points of
+      // half length in the forward): keep the plain kernel when it keeps
+      // more waves resident.
+      const int vr = real_kernel_vgprs(prec, dir);
+      if (vr > 0) {
+        const int cap_r = 4 * std::min(8, 512 / ((vr + 7) / 8 * 8));
+        const int wr = (sp.threads + 63) / 64;
+        const i64 by = dev.lds_full / ((i64)sp.f.C * pitch(m) * (i64)elt(prec));
+        if (w < (int)std::min<i64>(by, cap_r / wr) * wr) continue;
+      }
+      tab_wgs[dir] = (i64)std::max(dev.cus, 1) * (w / ((sp_tab.threads + 63) / 64));
+    }
+    if (tab_wgs[0] || tab_wgs[1])
+      RC_TRY(upload_complex(stw_tab, sp_tab.tw.empty() ? std::vector<cld>(1, cld(1, 0)) : sp_tab.tw, prec));
+    return GOFFT_OK;
+  }
+
   bool fused_last = false;   // forward untangle done by the half transform's last pass
   bool fused_first = false;  // inverse pack done by the half transform's first pass
   std::unique_ptr<Engine> half_inv;  // the inverse's own half transform (two-pass halves)
@@ -2532,7 +2603,10 @@ class RealEngine {
   }
   std::string describe() const {
     std::string s = "real n=" + std::to_string(n) + " lines=" + std::to_string(batch);
-    if (single) return s + " fused " + describe_sub(m, sp);
+    if (single)
+      return s + " fused " + describe_sub(m, sp) +
+             (tab_wgs[0] ? " forward table form " + describe_sub(m, sp_tab) + " x" + std::to_string(tab_wgs[0]) : "") +
+             (tab_wgs[1] ? " inverse table form " + describe_sub(m, sp_tab) + " x" + std::to_string(tab_wgs[1]) : "");
     if (even && fused_last) {
       const StockhamEngine* se = static_cast<const StockhamEngine*>(half.get());
       s += " packed: " + half->describe() + "; untangle in the last pass, paired " +
@@ -2603,6 +2677,15 @@ class RealEngine {
         a.nt = nt_bits(true, 16, false, 0);
       a.nbd = fastdiv_make((uint32_t)(sp.f.R + 1));
       a.f = sp.f;
+      if (tab_wgs[0]) {
+        a.lds_tab = 1;
+        a.stw = stw_tab.p;
+        a.stw_n = (int)std::max<size_t>(sp_tab.tw.size(), 1);
+        a.wgs = tab_wgs[0];
+        a.f = sp_tab.f;
+        HIP_TRY(launch_real_fwd(prec, 8, a, sp_tab.threads, c.s));
+        return GOFFT_OK;
+      }
       HIP_TRY(launch_real_fwd(prec, kMaxP, a, sp.threads, c.s));
       return GOFFT_OK;
     }
@@ -2667,6 +2750,15 @@ class RealEngine {
       if (ostride >= n && ((size_t)ostride * relt(prec)) % 128 == 0 && (uintptr_t)out % 128 == 0 &&
           ((size_t)n * relt(prec)) % 128 == 0)
         a.nt = nt_bits(false, 0, true, 32);
+      if (tab_wgs[1]) {
+        a.lds_tab = 1;
+        a.stw = stw_tab.p;
+        a.stw_n = (int)std::max<size_t>(sp_tab.tw.size(), 1);
+        a.wgs = tab_wgs[1];
+        a.f = sp_tab.f;
+        HIP_TRY(launch_real_inv(prec, 8, a, sp_tab.threads, c.s));
+        return GOFFT_OK;
+      }
       HIP_TRY(launch_real_inv(prec, kMaxP, a, sp.threads, c.s));
       return GOFFT_OK;
     }

@@ -97,6 +97,7 @@ void check_sub(const char* k, const gofft::SubFFT& f, int threads, int maxp, lon
       bool known = false;
       for (int x : k16) known |= maxp == 16 && x == r;
       for (int x : k24) known |= maxp == 24 && x == r;
+      known |= maxp == 8 && (r == 2 || r == 4 || r == 8);  // lds_fft_pow2
       const int cap = maxp / r > 0 ? maxp / r : 1;
       if (!known || (f.st[i].nb + f.tpc - 1) / f.tpc > cap) {
         snprintf(b, sizeof b, "%s: stage %d radix %d (%d butterflies, %d threads per column) not run by the %d-point form",
@@ -281,9 +282,24 @@ hipError_t launch_pass(int prec, int, const PassArgs& a, int threads, hipStream_
   if ((a.in_hook == 2 || a.out_hook == 2) && !a.x0buf) bad("k_pass: Rader hooks without x0 slots");
   return launch("pass");
 }
-hipError_t launch_real_fwd(int prec, int, const RealFwdArgs& a, int threads, hipStream_t) {
+hipError_t launch_real_fwd(int prec, int maxp, const RealFwdArgs& a, int threads, hipStream_t) {
   const size_t re = prec == 0 ? 8 : 4;
   count_nt(a.nt);
+  if (a.lds_tab) {
+    // table form: complex64 only, 8 points per thread, radices 2/4/8, the
+    // tables beside the columns, at most 512 threads, a positive grid
+    if (prec != 1 || maxp != 8) bad("k_real_fwd_tab: complex64 8-point form only");
+    for (int i = 0; i < a.f.nstages; i++) {
+      const int r = a.f.st[i].radix;
+      if (r != 2 && r != 4 && r != 8) bad("k_real_fwd_tab: radix other than 2, 4, 8");
+    }
+    if (threads > 512) bad("k_real_fwd_tab: more than 512 threads");
+    if (a.wgs <= 0) bad("k_real_fwd_tab: empty grid");
+    if (a.stw_n < 1) bad("k_real_fwd_tab: no stage twiddle entries");
+    const long tab = (long)(a.stw_n + 2 * a.f.R + 1) * 8;
+    check_sub("k_real_fwd_tab", a.f, threads, 8, kLds - tab, 8);
+    return launch("real_fwd");
+  }
   if ((a.nt & kNtLoad) && (!a.vec_in || (uintptr_t)a.in % 128 || (a.in_stride * re) % 128 ||
                            a.in_stride < 2 * (long long)a.f.R))
     bad("k_real_fwd: non-temporal loads of unaligned or overlapping rows");
@@ -291,9 +307,24 @@ hipError_t launch_real_fwd(int prec, int, const RealFwdArgs& a, int threads, hip
   check_sub("k_real_fwd", a.f, threads, 16, kLds, prec == 0 ? 16 : 8);
   return launch("real_fwd");
 }
-hipError_t launch_real_inv(int prec, int, const RealInvArgs& a, int threads, hipStream_t) {
+hipError_t launch_real_inv(int prec, int maxp, const RealInvArgs& a, int threads, hipStream_t) {
   const size_t re = prec == 0 ? 8 : 4;
   count_nt(a.nt);
+  if ((a.nt & kNtStore) && ((uintptr_t)a.out % 128 || (a.out_stride * re) % 128 || a.out_stride < 2 * (long long)a.f.R))
+    bad("k_real_inv: non-temporal stores to unaligned or overlapping rows");
+  if (a.lds_tab) {
+    if (prec != 1 || maxp != 8) bad("k_real_inv_tab: complex64 8-point form only");
+    for (int i = 0; i < a.f.nstages; i++) {
+      const int r = a.f.st[i].radix;
+      if (r != 2 && r != 4 && r != 8) bad("k_real_inv_tab: radix other than 2, 4, 8");
+    }
+    if (threads > 512) bad("k_real_inv_tab: more than 512 threads");
+    if (a.wgs <= 0) bad("k_real_inv_tab: empty grid");
+    if (a.stw_n < 1) bad("k_real_inv_tab: no stage twiddle entries");
+    const long tab = (long)(a.stw_n + a.f.R / 2 + 1) * 8;
+    check_sub("k_real_inv_tab", a.f, threads, 8, kLds - tab, 8);
+    return launch("real_inv");
+  }
   if ((a.nt & kNtStore) && ((uintptr_t)a.out % 128 || (a.out_stride * re) % 128 || a.out_stride < 2 * (long long)a.f.R))
     bad("k_real_inv: non-temporal stores to unaligned or overlapping rows");
   if (a.nt & kNtLoad) bad("k_real_inv: non-temporal loads are not used");
@@ -409,7 +440,8 @@ hipError_t launch_queue(int prec, const QueueArgs& a, int threads, size_t lds, l
   if (start != a.nitems) bad("k_queue: item count");
   return launch("queue");
 }
-int real_kernel_vgprs(int prec) { return prec == 0 ? 126 : 84; }
+int real_kernel_vgprs(int prec, int dir) { return prec == 0 ? 126 : dir == 1 ? 78 : 88; }
+int real_tab_vgprs(int) { return 89; }
 long long residual_parts(long long per) {
   long long p = (per + 2047) / 2048;
   return p < 1 ? 1 : p > 1024 ? 1024 : p;

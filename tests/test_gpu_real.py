@@ -95,6 +95,42 @@ def test_spectrogram(dev, oracle, seg, ov):
         assert rel_l2(got, want) <= bar, (seg, ov, dtype, rel_l2(got, want))
 
 
+# Complex64 batches of power-of-two half length whose line groups fill the
+# resident grid take the table forms (kernels.hip k_real_fwd_tab,
+# k_real_inv_tab): persistent workgroups, a last group of partial lines,
+# windows, odd hops (scalar sample loads), the PSD, plain bins and the
+# inverse.
+@pytest.mark.parametrize("n,batch", [(128, 50000), (1024, 6000), (512, 12000)])
+def test_real_table_forms(dev, oracle, n, batch):
+    D, torch = dev
+    x = rgauss(n * batch, seed=n + batch).reshape(batch, n)
+    p = D.R2CPlan(n, batch, torch.float32)
+    assert "forward table form" in p.describe() and "inverse table form" in p.describe(), p.describe()
+    got = p.forward(torch.from_numpy(x.astype(np.float32).ravel()).to("cuda")).cpu().numpy()
+    want = np.stack([oracle.rfft(r) for r in x])
+    assert rel_l2(got.reshape(batch, n // 2 + 1), want) <= 1e-6
+    back = p.inverse(torch.from_numpy(want.astype(np.complex64).ravel()).to("cuda")).cpu().numpy()
+    wantb = np.stack([oracle.irfft(w, n) for w in want])
+    assert rel_l2(back.reshape(batch, n), wantb) <= 1e-6
+    q = D.PSDPlan(n, batch, 0.1, torch.float32)
+    assert "forward table form" in q.describe(), q.describe()
+    got = q.exec(torch.from_numpy(x.astype(np.float32).ravel()).to("cuda")).cpu().numpy()
+    want = np.stack([oracle.psd(r, 0.1) for r in x])
+    assert rel_l2(got.reshape(batch, n // 2 + 1), want) <= 1e-6
+
+
+@pytest.mark.parametrize("seg,ov", [(1024, 512), (256, 100), (256, 101), (128, 64)])
+def test_spectrogram_table_form(dev, oracle, seg, ov):
+    D, torch = dev
+    x = rgauss(1 << 22, seed=seg + ov)
+    w = oracle.window("hann", seg)
+    p = D.SpectrogramPlan(len(x), seg, ov, w, 2.0, torch.float32)
+    assert "forward table form" in p.describe(), p.describe()
+    got = p.exec(torch.from_numpy(x.astype(np.float32)).to("cuda")).cpu().numpy()[: p.frames * p.bins]
+    want = oracle.spectrogram(x, seg, ov, w, 2.0)
+    assert rel_l2(got.reshape(p.frames, p.bins), want) <= 1e-6, (seg, ov)
+
+
 def test_spectrogram_argument_errors(dev):
     D, torch = dev
     from fft_amd import GofftError
