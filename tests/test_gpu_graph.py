@@ -155,3 +155,32 @@ def test_graph_replay_overlaps_execs_on_other_streams(dev):
         gc.launch(stream=s3)
         torch.cuda.synchronize()
         assert torch.equal(ya, wa) and torch.equal(yb, wb) and torch.equal(yc, wc)
+
+
+def test_graph_real_table_forms(dev):
+    # graphs of complex64 real plans that run the table forms (resident
+    # workgroups looping over the line groups): RFFT, IRFFT with a short
+    # spectrum, both VJPs and a Spectrogram
+    D, torch = dev
+    from fft_amd import _lib as L
+    from oracle import oracle as O
+
+    def rin(shape, seed):
+        g = torch.Generator().manual_seed(seed)
+        return torch.randn(shape, generator=g, dtype=torch.float64).to("cuda", torch.float32)
+
+    def cin(shape, seed):
+        g = torch.Generator().manual_seed(seed)
+        return torch.randn(shape, generator=g, dtype=torch.complex128).to("cuda", torch.complex64)
+
+    n, batch = 1024, 6000
+    p = D.R2CPlan(n, batch, torch.float32)
+    assert "forward table form" in p.describe() and "inverse table form" in p.describe(), p.describe()
+    nb = n // 2 + 1
+    _replay_op(D, torch, p, L.OP_R2C, lambda k: rin(n * batch, k), batch * nb, torch.complex64)
+    _replay_op(D, torch, p, L.OP_C2R, lambda k: cin(batch * 100, k), batch * n, torch.float32, bins=100)
+    _replay_op(D, torch, p, L.OP_VJP_R2C, lambda k: cin(batch * nb, k), batch * n, torch.float32)
+    _replay_op(D, torch, p, L.OP_VJP_C2R, lambda k: rin(batch * n, k), batch * nb, torch.complex64)
+    sp = D.SpectrogramPlan(1 << 22, 1024, 512, list(O.window("hann", 1024)), 1.0, torch.float32)
+    assert "forward table form" in sp.describe(), sp.describe()
+    _replay_op(D, torch, sp, L.OP_SPECTROGRAM, lambda k: rin(1 << 22, k), sp.frames * sp.bins, torch.float32)
